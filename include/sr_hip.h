@@ -1,0 +1,128 @@
+/*
+ * sr_hip.h -- C ABI of the MI355X symbolic-regression evaluator (libsrhip.so).
+ *
+ * Plain C types only.  Every call returns an sr_status (0 = SR_OK); on error
+ * sr_last_error(ctx) holds a message.  No call exits the process.
+ *
+ * Reference interfaces each entry point replaces (phi-jkim/symbolic_regression_gpu):
+ *   sr_ctx_create / sr_ctx_destroy
+ *       create_gpu_context / destroy_gpu_context      src/cleaned/cl_gpu_eval.cu:569-575
+ *       create_gpu_simple_context / destroy_...        src/cleaned/cl_gpu_simple.cu:315-321
+ *       create_gpu_nvrtc_context / destroy_...         src/cleaned/cl_gpu_nvrtc.cu:449-450
+ *   sr_set_data / sr_set_data_device
+ *       the upload_X flag + flatten_X_col_major_full   src/cleaned/cl_gpu_eval.cu:249-257,474-489
+ *   sr_eval_mse (mode PLAIN / BATCHED / CACHED / JIT)
+ *       evaluate_gpu_simple_wrapper                    src/cleaned/cl_gpu_simple.cu:324-336
+ *       evaluate_gpu_mse_wrapper (subtree cache)       src/cleaned/cl_gpu_eval.cu:578-592
+ *       evaluate_gpu_nvrtc_wrapper                     src/cleaned/cl_gpu_nvrtc.cu:452-548
+ *       evaluate_cpu_mse (semantics)                   src/cleaned/cl_cpu_eval.cpp:94-168
+ *   sr_eval_predictions
+ *       MultiEvalFunc (predictions per expression)     src/utils/utils.h:141
+ *   sr_cache_clear
+ *       the clear_cache flag of evaluate_gpu_mse       src/cleaned/cl_gpu_eval.cu:344-347
+ *   sr_detect
+ *       detect_common_subtrees + mark_cached_subtrees_from_state
+ *                                                      src/utils/detect.cpp:120-387,
+ *                                                      src/cleaned/cl_gpu_eval.cu:294-329
+ */
+#ifndef SR_HIP_H
+#define SR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sr_ctx sr_ctx;
+
+typedef enum {
+    SR_OK = 0,
+    SR_ERR_INVALID_ARG = -1,
+    SR_ERR_HIP = -2,
+    SR_ERR_BAD_EXPR = -3,  /* malformed token stream (underflow, bad token, bad var index) */
+    SR_ERR_NO_DATA = -4,   /* no X|y uploaded */
+    SR_ERR_CAPACITY = -5,  /* a fixed-capacity buffer would overflow */
+    SR_ERR_JIT = -6,       /* hipRTC compile / load failure */
+    SR_ERR_UNSUPPORTED = -7,
+    SR_ERR_NO_DEVICE = -8
+} sr_status;
+
+typedef enum {
+    SR_EVAL_PLAIN = 0,   /* one interpreter launch per expression */
+    SR_EVAL_BATCHED = 1, /* one persistent length-ordered launch for the population */
+    SR_EVAL_CACHED = 2,  /* batched + cross-population subtree cache */
+    SR_EVAL_JIT = 3      /* hipRTC straight-line kernels */
+} sr_eval_mode;
+
+/* Packed prefix-token population (the layout of cl_gpu_simple.cu:213-227).
+ * Expression e is tokens[offsets[e] .. offsets[e] + lengths[e]).
+ * values: the doubles the reference InputInfo carries (hash fidelity); when
+ * NULL, values_f32 is used. */
+typedef struct {
+    int32_t num_exprs;
+    const int32_t *tokens;
+    const double *values;
+    const float *values_f32;
+    const int32_t *offsets;
+    const int32_t *lengths;
+} sr_population;
+
+/* Mirrors RunStats (utils.h:103-114) + EvalMetrics (utils.h:67-73). */
+typedef struct {
+    double total_eval_time_ms;
+    double detect_time_ms;
+    double transfer_time_ms;
+    double kernel_time_ms;
+    double jit_compile_time_ms;
+    int32_t num_subtrees;
+    double avg_subtree_size;
+    double coverage;
+    int32_t num_new_subtrees;
+    int32_t num_cached;
+    int32_t num_kernel_launches;
+    int64_t logical_node_evals;  /* sum_e len_e * N */
+    int64_t executed_node_evals; /* token evaluations actually run */
+} sr_stats;
+
+/* Launch shape of the batched interpreter. */
+typedef struct {
+    int32_t dpt;           /* datapoints per lane: 1, 2, 4, 8 */
+    int32_t chunk;         /* datapoints per work item (multiple of 64*dpt) */
+    int32_t n_queues;      /* 1, or 8 = one queue per XCD */
+    int32_t blocks_per_cu; /* persistent workgroups per CU (0 = occupancy limit) */
+} sr_tuning;
+
+int sr_ctx_create(int device, sr_ctx **out);
+int sr_ctx_destroy(sr_ctx *ctx);
+const char *sr_last_error(const sr_ctx *ctx);
+/* Run on a caller-owned hipStream_t (NULL = the context's own stream). */
+int sr_set_stream(sr_ctx *ctx, void *hip_stream);
+
+/* Upload X|y, column-major [nv+1][N] fp32, y = column nv (host pointer). */
+int sr_set_data(sr_ctx *ctx, const float *X_colmajor, int32_t nv, int64_t N);
+/* Same layout, already in device memory; borrowed, caller keeps it alive. */
+int sr_set_data_device(sr_ctx *ctx, const float *d_X_colmajor, int32_t nv, int64_t N);
+
+/* Fitness: MSE over finite predictions (NaN when none), per expression.
+ * mse_out: host [num_exprs]; valid_out (optional): host [num_exprs] finite
+ * prediction counts; stats optional. */
+int sr_eval_mse(sr_ctx *ctx, const sr_population *pop, int mode, float *mse_out,
+                int64_t *valid_out, sr_stats *stats);
+/* Predictions, host [num_exprs][N]. */
+int sr_eval_predictions(sr_ctx *ctx, const sr_population *pop, float *pred_out);
+
+/* Compile one expression to the batched interpreter's bytecode (host only, no
+ * GPU needed).  out_code/out_arg: [len]; returns the instruction count, or a
+ * negative sr_status.  *stack_need = per-lane stack slots the program uses.
+ * A test and inspection hook; no counterpart in the reference. */
+int sr_compile_expr(const int32_t *tokens, const float *values, int32_t len, int32_t nv,
+                    int32_t *out_code, int32_t *out_arg, int32_t *stack_need);
+
+int sr_get_tuning(sr_ctx *ctx, sr_tuning *out);
+int sr_set_tuning(sr_ctx *ctx, const sr_tuning *t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,326 @@
+"""ctypes front end of the CPU oracle (oracle/sr_oracle.c) and, when it has been
+built in this container, of the reference harness (oracle/_ref/libref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the symbolic_regression_gpu_amd package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref.so")
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class GaParams(C.Structure):
+    _fields_ = [
+        ("num_vars", C.c_int), ("max_tokens", C.c_int), ("max_depth", C.c_int),
+        ("const_min", C.c_float), ("const_max", C.c_float),
+        ("prob_leaf", C.c_float), ("prob_const_leaf", C.c_float), ("prob_unary", C.c_float),
+        ("n_unary", C.c_int), ("n_binary", C.c_int),
+        ("unary_ops", C.c_int * 32), ("binary_ops", C.c_int * 32),
+    ]
+
+
+class SelectParams(C.Structure):
+    _fields_ = [("tournament", C.c_int), ("p_mutation", C.c_float), ("elite_frac", C.c_double)]
+
+
+class DetectResult(C.Structure):
+    _fields_ = [
+        ("num_subs", C.c_int),
+        ("sub_expr", C.POINTER(C.c_int)), ("sub_start", C.POINTER(C.c_int)),
+        ("sub_size", C.POINTER(C.c_int)), ("sub_hash", C.POINTER(C.c_uint64)),
+        ("occ_offset", C.POINTER(C.c_int)), ("occ_expr", C.POINTER(C.c_int)),
+        ("occ_start", C.POINTER(C.c_int)), ("hints", C.POINTER(C.c_int)),
+        ("covered_tokens", C.c_int64), ("avg_subtree_size", C.c_double),
+        ("coverage", C.c_double),
+    ]
+
+
+def build(ref: bool = False) -> None:
+    subprocess.run(["make", "-s", "-C", HERE] + (["ref"] if ref else []), check=True)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.sro_op32.restype = C.c_float
+        L.sro_op32.argtypes = [C.c_int, C.c_float, C.c_float]
+        L.sro_op64.restype = C.c_double
+        L.sro_op64.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.sro_predict32.argtypes = [_i32p, _f64p, C.c_int, _f32p, C.c_int, C.c_int64, _f32p]
+        L.sro_predict64.argtypes = [_i32p, _f64p, C.c_int, _f32p, C.c_int, C.c_int64, _f64p]
+        L.sro_mse32.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, _f32p, C.c_int, C.c_int64,
+                                C.c_int, _f32p, _i64p]
+        L.sro_mse64.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, _f32p, C.c_int, C.c_int64,
+                                C.c_int, _f64p]
+        L.sro_stack_depth.argtypes = [_i32p, C.c_int]
+        L.sro_subtree_size.argtypes = [_i32p, C.c_int, C.c_int]
+        L.sro_hash_flat.restype = C.c_uint64
+        L.sro_hash_flat.argtypes = [_i32p, _f64p, C.c_int]
+        L.sro_subtree_merkle.argtypes = [_i32p, _f64p, C.c_int, _u64p, _i32p]
+        L.sro_detect.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, C.c_int, C.c_int,
+                                 C.POINTER(DetectResult)]
+        L.sro_detect_free.argtypes = [C.POINTER(DetectResult)]
+        L.sro_mark_cached.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, _u64p, C.c_int,
+                                      C.c_int, _i32p]
+        L.sro_skeleton.restype = C.c_int
+        L.sro_skeleton.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, _i32p, _i32p, _f64p,
+                                   _i32p, _i32p]
+        L.sro_predict32_cached.argtypes = [_i32p, _f64p, C.c_int, _f32p, C.c_int, C.c_int64,
+                                           _f32p, _f32p]
+        L.sro_philox4x32_10.argtypes = [np.ctypeslib.ndpointer(dtype=np.uint32)] * 3
+        L.sro_ga_params_default.argtypes = [C.POINTER(GaParams)]
+        L.sro_ga_next_generation.argtypes = [
+            C.POINTER(GaParams), C.POINTER(SelectParams), C.c_int, C.c_int, _i32p, _f32p, _i32p,
+            _f32p, C.c_uint64, C.c_uint32, _i32p, _f32p, _i32p]
+        L.sro_ga_init_population.argtypes = [C.POINTER(GaParams), C.c_int, C.c_int, C.c_uint64,
+                                             _i32p, _f32p, _i32p]
+        L.sro_fitness_order.argtypes = [_f32p, C.c_int, _i32p]
+        L.sro_ga_subtree_size.argtypes = [_i32p, C.c_int, C.c_int]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    """Reference harness (only in a container holding the reference tree)."""
+    global _ref
+    if _ref is None:
+        R = C.CDLL(REF_PATH)
+        R.ref_op32.restype = C.c_float
+        R.ref_op32.argtypes = [C.c_int, C.c_float, C.c_float]
+        R.ref_op64.restype = C.c_double
+        R.ref_op64.argtypes = [C.c_int, C.c_double, C.c_double]
+        R.ref_eval_point32.restype = C.c_float
+        R.ref_eval_point32.argtypes = [_i32p, _f64p, C.c_int, _f32p, C.c_int]
+        R.ref_mse32.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, _f32p, C.c_int, C.c_int64,
+                                _f32p]
+        R.ref_hash_flat.restype = C.c_uint64
+        R.ref_hash_flat.argtypes = [_i32p, _f64p, C.c_int]
+        R.ref_detect.restype = C.c_int
+        R.ref_detect.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, C.c_int, C.c_int, _i32p,
+                                 _i32p, _u64p, _i32p, C.c_int]
+        R.ref_ga_op.restype = C.c_int
+        R.ref_ga_op.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(GaParams), _i32p, _f32p,
+                                C.c_int, _i32p, _f32p, C.c_int, _i32p, _f32p]
+        _ref = R
+    return _ref
+
+
+# --------------------------------------------------------------------------
+# numpy-level helpers
+# --------------------------------------------------------------------------
+
+class Population:
+    """Packed prefix-token population: expression e is tok[off[e]:off[e]+len[e]]."""
+
+    def __init__(self, exprs):
+        toks, vals, offs, lens = [], [], [], []
+        n = 0
+        for t, v in exprs:
+            offs.append(n)
+            lens.append(len(t))
+            toks.extend(int(x) for x in t)
+            vals.extend(float(x) for x in v)
+            n += len(t)
+        self.tok = np.asarray(toks, dtype=np.int32)
+        self.val = np.asarray(vals, dtype=np.float64)
+        self.off = np.asarray(offs, dtype=np.int32)
+        self.len = np.asarray(lens, dtype=np.int32)
+
+    @property
+    def E(self):
+        return int(self.len.shape[0])
+
+    def expr(self, e):
+        a, b = int(self.off[e]), int(self.off[e] + self.len[e])
+        return self.tok[a:b], self.val[a:b]
+
+
+def op32(op, a, b=0.0):
+    return lib().sro_op32(op, a, b)
+
+
+def op64(op, a, b=0.0):
+    return lib().sro_op64(op, a, b)
+
+
+def predict32(tok, val, X, nv, N):
+    out = np.empty(N, dtype=np.float32)
+    lib().sro_predict32(np.ascontiguousarray(tok, np.int32), np.ascontiguousarray(val, np.float64),
+                        len(tok), np.ascontiguousarray(X, np.float32), nv, N, out)
+    return out
+
+
+def predict64(tok, val, X, nv, N):
+    out = np.empty(N, dtype=np.float64)
+    lib().sro_predict64(np.ascontiguousarray(tok, np.int32), np.ascontiguousarray(val, np.float64),
+                        len(tok), np.ascontiguousarray(X, np.float32), nv, N, out)
+    return out
+
+
+def mse32(pop: Population, X, nv, N, threads=1):
+    out = np.empty(pop.E, dtype=np.float32)
+    valid = np.empty(pop.E, dtype=np.int64)
+    lib().sro_mse32(pop.tok, pop.val, pop.off, pop.len, pop.E, np.ascontiguousarray(X, np.float32),
+                    nv, N, threads, out, valid)
+    return out, valid
+
+
+def mse64(pop: Population, X, nv, N, threads=1):
+    out = np.empty(pop.E, dtype=np.float64)
+    lib().sro_mse64(pop.tok, pop.val, pop.off, pop.len, pop.E, np.ascontiguousarray(X, np.float32),
+                    nv, N, threads, out)
+    return out
+
+
+def stack_depth(tok):
+    t = np.ascontiguousarray(tok, np.int32)
+    return lib().sro_stack_depth(t, len(t))
+
+
+def hash_flat(tok, val):
+    t = np.ascontiguousarray(tok, np.int32)
+    return int(lib().sro_hash_flat(t, np.ascontiguousarray(val, np.float64), len(t)))
+
+
+def merkle(tok, val):
+    t = np.ascontiguousarray(tok, np.int32)
+    h = np.empty(len(t), dtype=np.uint64)
+    s = np.empty(len(t), dtype=np.int32)
+    lib().sro_subtree_merkle(t, np.ascontiguousarray(val, np.float64), len(t), h, s)
+    return h, s
+
+
+def detect(pop: Population, min_size=3, min_freq=2):
+    r = DetectResult()
+    lib().sro_detect(pop.tok, pop.val, pop.off, pop.len, pop.E, min_size, min_freq, C.byref(r))
+    n = r.num_subs
+    T = int(pop.len.sum())
+    occ_off = [r.occ_offset[i] for i in range(n + 1)]
+    out = {
+        "num_subs": n,
+        "sub_expr": np.array([r.sub_expr[i] for i in range(n)], np.int32),
+        "sub_start": np.array([r.sub_start[i] for i in range(n)], np.int32),
+        "sub_size": np.array([r.sub_size[i] for i in range(n)], np.int32),
+        "sub_hash": np.array([r.sub_hash[i] for i in range(n)], np.uint64),
+        "occurrences": [
+            [(r.occ_expr[k], r.occ_start[k]) for k in range(occ_off[i], occ_off[i + 1])]
+            for i in range(n)
+        ],
+        "hints": np.array([r.hints[i] for i in range(T)], np.int32),
+        "covered_tokens": r.covered_tokens,
+        "avg_subtree_size": r.avg_subtree_size,
+        "coverage": r.coverage,
+    }
+    lib().sro_detect_free(C.byref(r))
+    out["sub_flat"] = np.array(
+        [hash_flat(*_span(pop, out["sub_expr"][i], out["sub_start"][i], out["sub_size"][i]))
+         for i in range(n)], np.uint64)
+    return out
+
+
+def _span(pop, e, start, size):
+    a = int(pop.off[e]) + int(start)
+    return pop.tok[a:a + size], pop.val[a:a + size]
+
+
+def mark_cached(pop: Population, keys, min_size=3):
+    keys = np.ascontiguousarray(keys, np.uint64)
+    hints = np.empty(int(pop.len.sum()), dtype=np.int32)
+    lib().sro_mark_cached(pop.tok, pop.val, pop.off, pop.len, pop.E,
+                          keys if len(keys) else np.zeros(1, np.uint64), len(keys), min_size, hints)
+    return hints
+
+
+def skeleton(pop: Population, hints):
+    T = int(pop.len.sum())
+    sk_tok = np.empty(max(T, 1), np.int32)
+    sk_val = np.empty(max(T, 1), np.float64)
+    sk_off = np.empty(pop.E, np.int32)
+    sk_len = np.empty(pop.E, np.int32)
+    n = lib().sro_skeleton(pop.tok, pop.val, pop.off, pop.len, pop.E,
+                           np.ascontiguousarray(hints, np.int32), sk_tok, sk_val, sk_off, sk_len)
+    return sk_tok[:n], sk_val[:n], sk_off, sk_len
+
+
+def predict32_cached(tok, val, X, nv, N, cache):
+    out = np.empty(N, dtype=np.float32)
+    lib().sro_predict32_cached(np.ascontiguousarray(tok, np.int32),
+                               np.ascontiguousarray(val, np.float64), len(tok),
+                               np.ascontiguousarray(X, np.float32), nv, N,
+                               np.ascontiguousarray(cache, np.float32), out)
+    return out
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    o = np.empty(4, np.uint32)
+    lib().sro_philox4x32_10(c, k, o)
+    return o
+
+
+def ga_params(**kw) -> GaParams:
+    p = GaParams()
+    lib().sro_ga_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k in ("unary_ops", "binary_ops"):
+            arr = getattr(p, k)
+            for i, x in enumerate(v):
+                arr[i] = int(x)
+            setattr(p, "n_unary" if k == "unary_ops" else "n_binary", len(v))
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def ga_init_population(p: GaParams, pop, cap, seed):
+    tok = np.zeros(pop * cap, np.int32)
+    val = np.zeros(pop * cap, np.float32)
+    ln = np.zeros(pop, np.int32)
+    rc = lib().sro_ga_init_population(C.byref(p), pop, cap, seed, tok, val, ln)
+    return rc, tok, val, ln
+
+
+def ga_next_generation(p: GaParams, tournament, p_mutation, elite_frac, pop, cap, tok, val, ln,
+                       fit, seed, gen):
+    sp = SelectParams(tournament, p_mutation, elite_frac)
+    otok = np.zeros(pop * cap, np.int32)
+    oval = np.zeros(pop * cap, np.float32)
+    oln = np.zeros(pop, np.int32)
+    rc = lib().sro_ga_next_generation(
+        C.byref(p), C.byref(sp), pop, cap, np.ascontiguousarray(tok, np.int32),
+        np.ascontiguousarray(val, np.float32), np.ascontiguousarray(ln, np.int32),
+        np.ascontiguousarray(fit, np.float32), seed, gen, otok, oval, oln)
+    return rc, otok, oval, oln
+
+
+def fitness_order(fit):
+    f = np.ascontiguousarray(fit, np.float32)
+    o = np.empty(len(f), np.int32)
+    lib().sro_fitness_order(f, len(f), o)
+    return o

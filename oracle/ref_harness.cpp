@@ -1,0 +1,162 @@
+// ref_harness.cpp -- C entry points around the reference's own CPU sources,
+// compiled where they lie (oracle/Makefile adds -I$(REF)/src).  Output goes to
+// oracle/_ref/libref.so only.  Used to pin the oracle restatement and to write
+// the golden fixtures under tests/golden/.  Test infrastructure only.
+//
+// Translation units pulled in:
+//   cleaned/cl_cpu_eval.cpp   eval_op_cpu / evaluate_single_expr / evaluate_cpu_mse
+//   utils/detect.cpp          compute_hash / hash_node / detect_common_subtrees
+//   eval/cpu_simple_single.cpp eval_op (fp64 twin)           [linked separately]
+//   utils/utils.cpp           GA operators + set_seed        [linked separately]
+#include "cleaned/cl_cpu_eval.cpp"
+#include "utils/detect.cpp"
+
+#include <cstdint>
+#include <random>
+#include <vector>
+
+#include "sr_oracle.h"
+
+double eval_op(int op, double val1, double val2);  // cpu_simple_single.cpp:6
+
+extern "C" {
+
+float ref_op32(int op, float a, float b) { return eval_op_cpu(op, a, b); }
+double ref_op64(int op, double a, double b) { return eval_op(op, a, b); }
+
+float ref_eval_point32(const int *tok, const double *val, int len, const float *x, int nv)
+{
+    return evaluate_single_expr(tok, val, len, x, nv);
+}
+
+// evaluate_cpu_mse over a packed population sharing one column-major X|y.
+void ref_mse32(const int *tok, const double *val, const int *off, const int *len, int E,
+               const float *X, int nv, int64_t N, float *mse_out)
+{
+    InputInfo info{};
+    info.num_exprs = E;
+    std::vector<int> nvars(E, nv), ndps(E, (int)N), ntok(E);
+    std::vector<int *> toks(E);
+    std::vector<double *> vals(E);
+    for (int e = 0; e < E; e++) {
+        ntok[e] = len[e];
+        toks[e] = const_cast<int *>(tok + off[e]);
+        vals[e] = const_cast<double *>(val + off[e]);
+    }
+    info.num_vars = nvars.data();
+    info.num_dps = ndps.data();
+    info.num_tokens = ntok.data();
+    info.tokens = toks.data();
+    info.values = vals.data();
+    std::vector<float *> cols(nv + 1);
+    for (int v = 0; v <= nv; v++) cols[v] = const_cast<float *>(X + (int64_t)v * N);
+    std::vector<float **> all(E, cols.data());
+    std::vector<float> mses;
+    RunStats st;
+    evaluate_cpu_mse(info, all.data(), mses, st);
+    for (int e = 0; e < E; e++) mse_out[e] = mses[e];
+}
+
+uint64_t ref_hash_flat(const int *tok, const double *val, int len)
+{
+    return compute_hash(tok, val, len);
+}
+
+uint64_t ref_hash_node(int token, double value, const uint64_t *children, int nch)
+{
+    std::vector<uint64_t> ch(children, children + nch);
+    return hash_node(token, value, ch);
+}
+
+// detect_common_subtrees: returns num_subs; fills per-sub size/occ count, the
+// flat hash of each detected subtree, and the hint map (packed like tok).
+int ref_detect(const int *tok, const double *val, const int *off, const int *len, int E,
+               int min_size, int min_freq, int *sub_size, int *sub_occ, uint64_t *sub_flat,
+               int *hints, int max_subs)
+{
+    std::vector<int> ntok(len, len + E);
+    std::vector<const int *> toks(E);
+    std::vector<const double *> vals(E);
+    for (int e = 0; e < E; e++) {
+        toks[e] = tok + off[e];
+        vals[e] = val + off[e];
+    }
+    SubtreeDetectionResult r = detect_common_subtrees(E, 1, ntok.data(), toks.data(),
+                                                      vals.data(), min_size, min_freq);
+    int n = r.num_subs;
+    for (int s = 0; s < n && s < max_subs; s++) {
+        sub_size[s] = r.num_sub_tokens[s];
+        sub_occ[s] = r.num_occurrences[s];
+        sub_flat[s] = compute_hash(r.sub_tokens[s], r.sub_values[s], r.num_sub_tokens[s]);
+    }
+    for (int e = 0; e < E; e++)
+        for (int i = 0; i < len[e]; i++) hints[off[e] + i] = r.expr_sub_hints[e][i];
+    free_subtree_detection_result(r);
+    return n;
+}
+
+// ---- GA: reference operators vs the oracle restatement under one mt19937 ----
+
+static std::mt19937 g_twin;
+
+static float twin_uniform(sro_rng *, float lo, float hi)
+{
+    std::uniform_real_distribution<float> d(lo, hi);
+    return d(g_twin);
+}
+
+static int twin_randint(sro_rng *, int lo, int hi)
+{
+    if (hi <= lo) return lo;
+    std::uniform_int_distribution<int> d(lo, hi);
+    return d(g_twin);
+}
+
+static EvolutionParams to_ref(const sro_ga_params *p)
+{
+    EvolutionParams e;
+    e.num_vars = p->num_vars;
+    e.max_tokens = p->max_tokens;
+    e.max_depth = p->max_depth;
+    e.const_min = p->const_min;
+    e.const_max = p->const_max;
+    e.prob_leaf = p->prob_leaf;
+    e.prob_const_leaf = p->prob_const_leaf;
+    e.prob_unary = p->prob_unary;
+    return e;
+}
+
+static int out(const std::vector<int> &t, const std::vector<float> &v, int *tok, float *val)
+{
+    for (size_t i = 0; i < t.size(); i++) {
+        tok[i] = t[i];
+        val[i] = v[i];
+    }
+    return (int)t.size();
+}
+
+// op: 0 generate, 1 mutate(a), 2 crossover(a, b).  which: 0 reference, 1 oracle.
+int ref_ga_op(int op, int which, int seed, const sro_ga_params *p, const int *at,
+              const float *av, int alen, const int *bt, const float *bv, int blen, int *tok,
+              float *val)
+{
+    if (which == 0) {
+        set_seed(seed);
+        EvolutionParams e = to_ref(p);
+        std::vector<int> t;
+        std::vector<float> v;
+        std::vector<int> a(at, at + alen), b(bt, bt + blen);
+        std::vector<float> fa(av, av + alen), fb(bv, bv + blen);
+        if (op == 0) generate_random_expression(e, t, v);
+        else if (op == 1) mutate_expression(e, a, fa, t, v);
+        else crossover_expressions(e, a, fa, b, fb, t, v);
+        return out(t, v, tok, val);
+    }
+    g_twin.seed(seed);
+    sro_rng r{twin_uniform, twin_randint, nullptr};
+    if (op == 0) return sro_ga_generate(p, p->max_tokens, &r, tok, val);
+    if (op == 1) return sro_ga_mutate(p, at, av, alen, &r, tok, val);
+    return sro_ga_crossover(p, at, av, alen, bt, bv, blen, &r, tok, val);
+}
+
+}  // extern "C"

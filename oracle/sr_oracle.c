@@ -1,0 +1,1023 @@
+/*
+ * sr_oracle.c -- CPU restatement of the reference hot path (test infrastructure).
+ *
+ * This is the checker for the HIP product path and its CPU timing baseline.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+ * See sr_oracle.h for what pins each function.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fPIC -shared (oracle/Makefile).  FP
+ * contraction is off so every fp32 operation rounds exactly as written.
+ */
+#include "sr_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* Op semantics                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* cl_cpu_eval.cpp:16-56 (fp32 op table).  Opcode 29 (exp clamped) follows the
+ * clamped exp of evolution_tests.cpp:40: a > 10 -> 1e9, else expf(a). */
+float sro_op32(int op, float a, float b)
+{
+    const float DELTA = 1e-9f;
+    const float MAX_VAL = 1e9f;
+    switch (op) {
+    case 1: return a + b;
+    case 2: return a - b;
+    case 3: return a * b;
+    case 4: return (b == 0.0f) ? NAN : a / b;
+    case 5: return powf(a, b);
+    case 6: return fminf(a, b);
+    case 7: return fmaxf(a, b);
+    case 8: {
+        float d = fabsf(b) <= DELTA ? (b < 0.0f ? -DELTA : DELTA) : b;
+        return a / d;
+    }
+    case 9: return (a == 0.0f && b == 0.0f) ? 0.0f : powf(fabsf(a), b);
+    case 10: return sinf(a);
+    case 11: return cosf(a);
+    case 12: return tanf(a);
+    case 13: return sinhf(a);
+    case 14: return coshf(a);
+    case 15: return tanhf(a);
+    case 16: return expf(a);
+    case 17: return logf(a);
+    case 18: return 1.0f / a;
+    case 19: return asinf(a);
+    case 20: return acosf(a);
+    case 21: return atanf(a);
+    case 22: return (a == 0.0f) ? -MAX_VAL : logf(fabsf(a));
+    case 23: {
+        float d = fabsf(a) <= DELTA ? (a < 0.0f ? -DELTA : DELTA) : a;
+        return 1.0f / d;
+    }
+    case 24: return fabsf(a);
+    case 25: return -a;
+    case 26: return sqrtf(a);
+    case 27: return sqrtf(fabsf(a));
+    case SRO_OP_EXP_CLAMPED: return (a > 10.0f) ? MAX_VAL : expf(a);
+    default: return 0.0f;
+    }
+}
+
+/* cpu_simple_single.cpp:6-79 (fp64 twin; MIN/MAX via <= / >=). */
+double sro_op64(int op, double a, double b)
+{
+    const double DELTA = 1e-9;
+    const double MAX_VAL = 1e9;
+    switch (op) {
+    case 1: return a + b;
+    case 2: return a - b;
+    case 3: return a * b;
+    case 4: return (b == 0.0) ? NAN : a / b;
+    case 5: return pow(a, b);
+    case 6: return (a <= b) ? a : b;
+    case 7: return (a >= b) ? a : b;
+    case 8: {
+        double d = fabs(b) <= DELTA ? (b < 0 ? -DELTA : DELTA) : b;
+        return a / d;
+    }
+    case 9: return (a == 0.0 && b == 0.0) ? 0.0 : pow(fabs(a), b);
+    case 10: return sin(a);
+    case 11: return cos(a);
+    case 12: return tan(a);
+    case 13: return sinh(a);
+    case 14: return cosh(a);
+    case 15: return tanh(a);
+    case 16: return exp(a);
+    case 17: return log(a);
+    case 18: return 1.0 / a;
+    case 19: return asin(a);
+    case 20: return acos(a);
+    case 21: return atan(a);
+    case 22: return (a == 0.0) ? -MAX_VAL : log(fabs(a));
+    case 23: {
+        double d = fabs(a) <= DELTA ? (a < 0 ? -DELTA : DELTA) : a;
+        return 1.0 / d;
+    }
+    case 24: return fabs(a);
+    case 25: return -a;
+    case 26: return sqrt(a);
+    case 27: return sqrt(fabs(a));
+    case SRO_OP_EXP_CLAMPED: return (a > 10.0) ? MAX_VAL : exp(a);
+    default: return 0.0;
+    }
+}
+
+/* cl_cpu_eval.cpp:82 -- tokens 1..9 pop two operands, every other positive
+ * token pops one. */
+int sro_is_binary(int tok) { return tok >= 1 && tok <= 9; }
+
+/* cl_cpu_eval.cpp:59-90: right-to-left prefix scan. */
+float sro_eval_point32(const int *tok, const double *val, int len, const float *x, int nv)
+{
+    float stk[SRO_MAX_STACK];
+    int sp = 0;
+    for (int i = len - 1; i >= 0; i--) {
+        int t = tok[i];
+        if (t == SRO_TOK_CONST) {
+            stk[sp++] = (float)val[i];
+        } else if (t == SRO_TOK_VAR) {
+            int v = (int)val[i];
+            stk[sp++] = (v < nv) ? x[v] : 0.0f;
+        } else {
+            float a = stk[--sp];
+            float b = 0.0f;
+            if (sro_is_binary(t)) b = stk[--sp];
+            stk[sp++] = sro_op32(t, a, b);
+        }
+    }
+    return stk[0];
+}
+
+/* cpu_simple_single.cpp:96-127 (fp64 interpreter). */
+double sro_eval_point64(const int *tok, const double *val, int len, const double *x, int nv)
+{
+    double stk[SRO_MAX_STACK];
+    int sp = 0;
+    for (int i = len - 1; i >= 0; i--) {
+        int t = tok[i];
+        if (t == SRO_TOK_CONST) {
+            stk[sp++] = val[i];
+        } else if (t == SRO_TOK_VAR) {
+            int v = (int)val[i];
+            stk[sp++] = (v < nv) ? x[v] : 0.0;
+        } else {
+            double a = stk[--sp];
+            double b = 0.0;
+            if (sro_is_binary(t)) b = stk[--sp];
+            stk[sp++] = sro_op64(t, a, b);
+        }
+    }
+    return stk[0];
+}
+
+int sro_stack_depth(const int *tok, int len)
+{
+    int sp = 0, mx = 0;
+    for (int i = len - 1; i >= 0; i--) {
+        int t = tok[i];
+        if (t == SRO_TOK_CONST || t == SRO_TOK_VAR || t == SRO_OP_REF) {
+            sp++;
+        } else if (t > 0) {
+            int need = sro_is_binary(t) ? 2 : 1;
+            if (sp < need) return -1;
+            sp -= need - 1;
+        } else {
+            return -1;
+        }
+        if (sp > mx) mx = sp;
+    }
+    return sp == 1 ? mx : -1;
+}
+
+void sro_predict32(const int *tok, const double *val, int len, const float *X, int nv,
+                   int64_t N, float *out)
+{
+    float x[64];
+    for (int64_t dp = 0; dp < N; dp++) {
+        for (int v = 0; v < nv && v < 64; v++) x[v] = X[(int64_t)v * N + dp];
+        out[dp] = sro_eval_point32(tok, val, len, x, nv);
+    }
+}
+
+void sro_predict64(const int *tok, const double *val, int len, const float *X, int nv,
+                   int64_t N, double *out)
+{
+    double x[64];
+    for (int64_t dp = 0; dp < N; dp++) {
+        for (int v = 0; v < nv && v < 64; v++) x[v] = (double)X[(int64_t)v * N + dp];
+        out[dp] = sro_eval_point64(tok, val, len, x, nv);
+    }
+}
+
+/* ---- MSE over a population, threads statically partitioned over exprs ---- */
+
+typedef struct {
+    const int *tok;
+    const double *val;
+    const int *off;
+    const int *len;
+    int e0, e1;
+    const float *X;
+    int nv;
+    int64_t N;
+    float *mse32;
+    int64_t *valid;
+    double *mse64;
+} mse_job;
+
+/* cl_cpu_eval.cpp:117-158 */
+static void *mse32_worker(void *arg)
+{
+    mse_job *j = (mse_job *)arg;
+    float x[64];
+    const float *y = j->X + (int64_t)j->nv * j->N;
+    for (int e = j->e0; e < j->e1; e++) {
+        const int *t = j->tok + j->off[e];
+        const double *v = j->val + j->off[e];
+        int L = j->len[e];
+        double total = 0.0;
+        int64_t valid = 0;
+        for (int64_t dp = 0; dp < j->N; dp++) {
+            for (int k = 0; k < j->nv && k < 64; k++) x[k] = j->X[(int64_t)k * j->N + dp];
+            float pred = sro_eval_point32(t, v, L, x, j->nv);
+            if (!isnan(pred) && !isinf(pred)) {
+                float diff = pred - y[dp];
+                float sq = diff * diff;
+                total += sq;
+                valid++;
+            }
+        }
+        j->mse32[e] = valid > 0 ? (float)(total / (double)valid) : NAN;
+        if (j->valid) j->valid[e] = valid;
+    }
+    return NULL;
+}
+
+static void *mse64_worker(void *arg)
+{
+    mse_job *j = (mse_job *)arg;
+    double x[64];
+    const float *y = j->X + (int64_t)j->nv * j->N;
+    for (int e = j->e0; e < j->e1; e++) {
+        const int *t = j->tok + j->off[e];
+        const double *v = j->val + j->off[e];
+        int L = j->len[e];
+        double total = 0.0;
+        int64_t valid = 0;
+        for (int64_t dp = 0; dp < j->N; dp++) {
+            for (int k = 0; k < j->nv && k < 64; k++) x[k] = (double)j->X[(int64_t)k * j->N + dp];
+            double pred = sro_eval_point64(t, v, L, x, j->nv);
+            if (!isnan(pred) && !isinf(pred)) {
+                double diff = pred - (double)y[dp];
+                total += diff * diff;
+                valid++;
+            }
+        }
+        j->mse64[e] = valid > 0 ? total / (double)valid : NAN;
+    }
+    return NULL;
+}
+
+static void run_mse(const int *tok, const double *val, const int *off, const int *len, int E,
+                    const float *X, int nv, int64_t N, int threads, float *m32,
+                    int64_t *valid, double *m64)
+{
+    if (threads < 1) threads = 1;
+    if (threads > E) threads = E > 0 ? E : 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+    mse_job *jobs = (mse_job *)malloc(sizeof(mse_job) * threads);
+    for (int t = 0; t < threads; t++) {
+        mse_job *j = &jobs[t];
+        j->tok = tok; j->val = val; j->off = off; j->len = len;
+        j->e0 = (int)((int64_t)E * t / threads);
+        j->e1 = (int)((int64_t)E * (t + 1) / threads);
+        j->X = X; j->nv = nv; j->N = N;
+        j->mse32 = m32; j->valid = valid; j->mse64 = m64;
+        pthread_create(&th[t], NULL, m32 ? mse32_worker : mse64_worker, j);
+    }
+    for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    free(th);
+    free(jobs);
+}
+
+void sro_mse32(const int *tok, const double *val, const int *off, const int *len, int E,
+               const float *X, int nv, int64_t N, int threads, float *mse_out,
+               int64_t *valid_out)
+{
+    run_mse(tok, val, off, len, E, X, nv, N, threads, mse_out, valid_out, NULL);
+}
+
+void sro_mse64(const int *tok, const double *val, const int *off, const int *len, int E,
+               const float *X, int nv, int64_t N, int threads, double *mse_out)
+{
+    run_mse(tok, val, off, len, E, X, nv, N, threads, NULL, NULL, mse_out);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Hashing, detection, cache marking                                         */
+/* ------------------------------------------------------------------------ */
+
+#define FNV_OFFSET 14695981039346656037ULL
+#define FNV_PRIME 1099511628211ULL
+
+static int detect_arity(int t) { return t < 10 ? 2 : 1; } /* detect.cpp:16-19 */
+
+/* detect.cpp:23-47 (recursive size; terminals are tokens <= 0). */
+int sro_subtree_size(const int *tok, int start, int total)
+{
+    if (start >= total) return 0;
+    int t = tok[start];
+    if (t <= 0) return 1;
+    int ar = detect_arity(t), size = 1, pos = start + 1;
+    for (int i = 0; i < ar; i++) {
+        if (pos >= total) break;
+        int s = sro_subtree_size(tok, pos, total);
+        size += s;
+        pos += s;
+    }
+    return size;
+}
+
+static uint64_t dbl_bits(double d)
+{
+    uint64_t u;
+    memcpy(&u, &d, sizeof u);
+    return u;
+}
+
+/* detect.cpp:50-68 == cl_gpu_eval.cu:259-272.  The token is widened as a
+ * signed int, so -1 hashes as 0xffffffffffffffff. */
+uint64_t sro_hash_flat(const int *tok, const double *val, int len)
+{
+    uint64_t h = FNV_OFFSET;
+    for (int i = 0; i < len; i++) {
+        h ^= (uint64_t)(int64_t)tok[i];
+        h *= FNV_PRIME;
+        h ^= dbl_bits(val[i]);
+        h *= FNV_PRIME;
+    }
+    return h;
+}
+
+/* detect.cpp:97-118 and the reverse pass at :170-191. */
+void sro_subtree_merkle(const int *tok, const double *val, int len, uint64_t *hash_out,
+                        int *size_out)
+{
+    for (int i = len - 1; i >= 0; i--) {
+        int t = tok[i];
+        uint64_t h = FNV_OFFSET;
+        h ^= (uint64_t)(int64_t)t;
+        h *= FNV_PRIME;
+        h ^= dbl_bits(val[i]);
+        h *= FNV_PRIME;
+        int size = 1;
+        if (t > 0) {
+            int pos = i + 1;
+            for (int k = 0; k < detect_arity(t); k++) {
+                if (pos >= len) break;
+                uint64_t ch = hash_out[pos];
+                h = h ^ (ch + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2));
+                size += size_out[pos];
+                pos += size_out[pos];
+            }
+        }
+        hash_out[i] = h;
+        size_out[i] = size;
+    }
+}
+
+typedef struct {
+    uint64_t hash;
+    int expr, start, size;
+} cand_t;
+
+static int cand_cmp(const void *a, const void *b)
+{
+    const cand_t *x = (const cand_t *)a, *y = (const cand_t *)b;
+    if (x->hash != y->hash) return x->hash < y->hash ? -1 : 1;
+    if (x->expr != y->expr) return x->expr < y->expr ? -1 : 1;
+    return (x->start > y->start) - (x->start < y->start);
+}
+
+typedef struct {
+    int first, count, size; /* into verified-candidate array */
+    uint64_t hash;
+} group_t;
+
+static int group_cmp(const void *a, const void *b)
+{
+    const group_t *x = (const group_t *)a, *y = (const group_t *)b;
+    if (x->size != y->size) return x->size > y->size ? -1 : 1;
+    if (x->hash != y->hash) return x->hash < y->hash ? -1 : 1;
+    return 0;
+}
+
+/* detect_common_subtrees, detect.cpp:120-387. */
+void sro_detect(const int *tok, const double *val, const int *off, const int *len, int E,
+                int min_size, int min_freq, sro_detect_result *res)
+{
+    memset(res, 0, sizeof *res);
+    int64_t T = 0;
+    int maxlen = 0;
+    for (int e = 0; e < E; e++) {
+        T += len[e];
+        if (len[e] > maxlen) maxlen = len[e];
+    }
+    res->hints = (int *)malloc(sizeof(int) * (T > 0 ? T : 1));
+    for (int64_t i = 0; i < T; i++) res->hints[i] = -1;
+
+    /* Step 1: reverse pass, candidates of size >= min_size. */
+    cand_t *cands = (cand_t *)malloc(sizeof(cand_t) * (T > 0 ? T : 1));
+    int64_t nc = 0;
+    uint64_t *hs = (uint64_t *)malloc(sizeof(uint64_t) * (maxlen > 0 ? maxlen : 1));
+    int *sz = (int *)malloc(sizeof(int) * (maxlen > 0 ? maxlen : 1));
+    for (int e = 0; e < E; e++) {
+        sro_subtree_merkle(tok + off[e], val + off[e], len[e], hs, sz);
+        for (int i = len[e] - 1; i >= 0; i--) {
+            if (sz[i] >= min_size) {
+                cands[nc].hash = hs[i];
+                cands[nc].expr = e;
+                cands[nc].start = i;
+                cands[nc].size = sz[i];
+                nc++;
+            }
+        }
+    }
+    free(hs);
+    free(sz);
+    qsort(cands, (size_t)nc, sizeof(cand_t), cand_cmp);
+
+    /* Step 2: group by hash, verify against the group's first member. */
+    cand_t *ver = (cand_t *)malloc(sizeof(cand_t) * (nc > 0 ? nc : 1));
+    int64_t nver = 0;
+    group_t *groups = (group_t *)malloc(sizeof(group_t) * (nc > 0 ? nc : 1));
+    int ng = 0;
+    int64_t i = 0;
+    while (i < nc) {
+        int64_t j = i + 1;
+        while (j < nc && cands[j].hash == cands[i].hash) j++;
+        if (j - i >= min_freq) {
+            const cand_t *r = &cands[i];
+            const int *rt = tok + off[r->expr] + r->start;
+            const double *rv = val + off[r->expr] + r->start;
+            int64_t first = nver;
+            for (int64_t k = i; k < j; k++) {
+                const cand_t *c = &cands[k];
+                if (c->size != r->size) continue;
+                const int *ct = tok + off[c->expr] + c->start;
+                const double *cv = val + off[c->expr] + c->start;
+                int same = 1;
+                for (int q = 0; q < r->size; q++) {
+                    if (ct[q] != rt[q] || cv[q] != rv[q]) { same = 0; break; }
+                }
+                if (same) ver[nver++] = *c;
+            }
+            if (nver - first >= min_freq) {
+                groups[ng].first = (int)first;
+                groups[ng].count = (int)(nver - first);
+                groups[ng].size = r->size;
+                groups[ng].hash = r->hash;
+                ng++;
+            } else {
+                nver = first;
+            }
+        }
+        i = j;
+    }
+
+    /* Step 3: size descending. */
+    qsort(groups, (size_t)ng, sizeof(group_t), group_cmp);
+
+    /* Step 4: result + greedy non-overlapping marking. */
+    res->num_subs = ng;
+    res->sub_expr = (int *)malloc(sizeof(int) * (ng > 0 ? ng : 1));
+    res->sub_start = (int *)malloc(sizeof(int) * (ng > 0 ? ng : 1));
+    res->sub_size = (int *)malloc(sizeof(int) * (ng > 0 ? ng : 1));
+    res->sub_hash = (uint64_t *)malloc(sizeof(uint64_t) * (ng > 0 ? ng : 1));
+    res->occ_offset = (int *)malloc(sizeof(int) * (ng + 1));
+    res->occ_expr = (int *)malloc(sizeof(int) * (nver > 0 ? nver : 1));
+    res->occ_start = (int *)malloc(sizeof(int) * (nver > 0 ? nver : 1));
+    int o = 0;
+    int64_t size_sum = 0;
+    for (int s = 0; s < ng; s++) {
+        const group_t *g = &groups[s];
+        const cand_t *r = &ver[g->first];
+        res->sub_expr[s] = r->expr;
+        res->sub_start[s] = r->start;
+        res->sub_size[s] = r->size;
+        res->sub_hash[s] = g->hash;
+        size_sum += r->size;
+        res->occ_offset[s] = o;
+        for (int k = 0; k < g->count; k++) {
+            const cand_t *c = &ver[g->first + k];
+            res->occ_expr[o] = c->expr;
+            res->occ_start[o] = c->start;
+            o++;
+            int *h = res->hints + off[c->expr] + c->start;
+            int overlap = 0;
+            for (int q = 0; q < r->size; q++) {
+                if (h[q] != -1) { overlap = 1; break; }
+            }
+            if (!overlap) {
+                h[0] = s;
+                for (int q = 1; q < r->size; q++) h[q] = -2;
+            }
+        }
+    }
+    res->occ_offset[ng] = o;
+    int64_t covered = 0;
+    for (int64_t q = 0; q < T; q++) covered += res->hints[q] != -1;
+    res->covered_tokens = covered;
+    res->avg_subtree_size = ng > 0 ? (double)size_sum / ng : 0.0;
+    res->coverage = T > 0 ? (double)covered / (double)T : 0.0;
+    free(cands);
+    free(ver);
+    free(groups);
+}
+
+void sro_detect_free(sro_detect_result *res)
+{
+    free(res->sub_expr);
+    free(res->sub_start);
+    free(res->sub_size);
+    free(res->sub_hash);
+    free(res->occ_offset);
+    free(res->occ_expr);
+    free(res->occ_start);
+    free(res->hints);
+    memset(res, 0, sizeof *res);
+}
+
+static int find_key(const uint64_t *keys, int n, uint64_t h)
+{
+    for (int i = 0; i < n; i++)
+        if (keys[i] == h) return i;
+    return -1;
+}
+
+/* cl_gpu_eval.cu:384-397 */
+int sro_cache_update(const sro_detect_result *det, const int *tok, const double *val,
+                     const int *off, uint64_t *keys, int *num_cached, int capacity,
+                     int *new_sub, int *new_slot)
+{
+    int nn = 0;
+    for (int s = 0; s < det->num_subs; s++) {
+        const int *t = tok + off[det->sub_expr[s]] + det->sub_start[s];
+        const double *v = val + off[det->sub_expr[s]] + det->sub_start[s];
+        uint64_t h = sro_hash_flat(t, v, det->sub_size[s]);
+        if (find_key(keys, *num_cached, h) >= 0) continue;
+        if (*num_cached >= capacity) continue;
+        int slot = (*num_cached)++;
+        keys[slot] = h;
+        new_sub[nn] = s;
+        new_slot[nn] = slot;
+        nn++;
+    }
+    return nn;
+}
+
+/* cl_gpu_eval.cu:279-292 (size with arity 2 for tokens < 10, 1 otherwise). */
+void sro_mark_cached(const int *tok, const double *val, const int *off, const int *len,
+                     int E, const uint64_t *keys, int num_cached, int min_size, int *hints)
+{
+    for (int e = 0; e < E; e++) {
+        const int *t = tok + off[e];
+        const double *v = val + off[e];
+        int L = len[e];
+        int *h = hints + off[e];
+        for (int i = 0; i < L; i++) h[i] = -1;
+        for (int pos = 0; pos < L; pos++) {
+            if (h[pos] != -1) continue;
+            int size = sro_subtree_size(t, pos, L);
+            if (size < min_size) continue;
+            int slot = find_key(keys, num_cached, sro_hash_flat(t + pos, v + pos, size));
+            if (slot < 0) continue;
+            int overlap = 0;
+            for (int k = 0; k < size; k++)
+                if (h[pos + k] != -1) { overlap = 1; break; }
+            if (overlap) continue;
+            h[pos] = slot;
+            for (int k = 1; k < size && pos + k < L; k++) h[pos + k] = -2;
+        }
+    }
+}
+
+/* cl_gpu_eval.cu:433-459 */
+int sro_skeleton(const int *tok, const double *val, const int *off, const int *len, int E,
+                 const int *hints, int *sk_tok, double *sk_val, int *sk_off, int *sk_len)
+{
+    int n = 0;
+    for (int e = 0; e < E; e++) {
+        sk_off[e] = n;
+        const int *t = tok + off[e];
+        const double *v = val + off[e];
+        const int *h = hints + off[e];
+        for (int k = 0; k < len[e]; k++) {
+            if (h[k] >= 0) {
+                int s = sro_subtree_size(t, k, len[e]);
+                sk_tok[n] = SRO_OP_REF;
+                sk_val[n] = (double)(float)h[k];
+                n++;
+                k += s - 1;
+            } else if (h[k] == -2) {
+                continue;
+            } else {
+                sk_tok[n] = t[k];
+                sk_val[n] = (double)(float)v[k];
+                n++;
+            }
+        }
+        sk_len[e] = n - sk_off[e];
+    }
+    return n;
+}
+
+/* cl_gpu_eval.cu:165-186 (skeleton interpreter; OP_REF loads a column). */
+void sro_predict32_cached(const int *tok, const double *val, int len, const float *X, int nv,
+                          int64_t N, const float *cache, float *out)
+{
+    float stk[SRO_MAX_STACK];
+    for (int64_t dp = 0; dp < N; dp++) {
+        int sp = 0;
+        for (int i = len - 1; i >= 0; i--) {
+            int t = tok[i];
+            if (t == SRO_OP_REF) {
+                stk[sp++] = cache[(int64_t)(int)val[i] * N + dp];
+            } else if (t == SRO_TOK_CONST) {
+                stk[sp++] = (float)val[i];
+            } else if (t == SRO_TOK_VAR) {
+                int v = (int)val[i];
+                stk[sp++] = v < nv ? X[(int64_t)v * N + dp] : 0.0f;
+            } else if (t > 0) {
+                float a = stk[--sp], b = 0.0f;
+                if (sro_is_binary(t)) b = stk[--sp];
+                stk[sp++] = sro_op32(t, a, b);
+            }
+        }
+        out[dp] = stk[0];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), written from the paper  */
+/* ------------------------------------------------------------------------ */
+
+void sro_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+void sro_philox_stream_init(sro_philox_stream *s, uint64_t seed, uint32_t stream,
+                            uint32_t gen, uint32_t domain)
+{
+    s->key[0] = (uint32_t)seed;
+    s->key[1] = (uint32_t)(seed >> 32);
+    s->c1 = stream;
+    s->c2 = gen;
+    s->c3 = domain;
+    s->word = 0;
+}
+
+uint32_t sro_philox_next(sro_philox_stream *s)
+{
+    uint32_t ctr[4] = {s->word >> 2, s->c1, s->c2, s->c3};
+    uint32_t out[4];
+    sro_philox4x32_10(ctr, s->key, out);
+    uint32_t w = out[s->word & 3];
+    s->word++;
+    return w;
+}
+
+/* uniform in [lo, hi): lo + (hi - lo) * u, u = top 24 bits / 2^24.  Each
+ * product and sum rounds once (no contraction), identically on the GPU. */
+static float philox_uniform(sro_rng *r, float lo, float hi)
+{
+    uint32_t w = sro_philox_next((sro_philox_stream *)r->state);
+    float u = (float)(w >> 8) * (1.0f / 16777216.0f);
+    float span = hi - lo;
+    float t = span * u;
+    return lo + t;
+}
+
+/* inclusive [lo, hi] by multiply-shift; no draw when hi <= lo (utils.cpp:973-979). */
+static int philox_randint(sro_rng *r, int lo, int hi)
+{
+    if (hi <= lo) return lo;
+    uint32_t w = sro_philox_next((sro_philox_stream *)r->state);
+    uint32_t range = (uint32_t)(hi - lo) + 1u;
+    return lo + (int)(((uint64_t)w * range) >> 32);
+}
+
+void sro_rng_bind_philox(sro_rng *r, sro_philox_stream *s)
+{
+    r->uniform = philox_uniform;
+    r->randint = philox_randint;
+    r->state = s;
+}
+
+/* ------------------------------------------------------------------------ */
+/* GA operators (utils.cpp:934-1272)                                         */
+/* ------------------------------------------------------------------------ */
+
+void sro_ga_params_default(sro_ga_params *p)
+{
+    static const int un[18] = {10, 11, 12, 13, 14, 15, 16, 17, 18,
+                               19, 20, 21, 22, 23, 24, 25, 26, 27};
+    static const int bi[9] = {1, 2, 3, 4, 5, 6, 7, 8, 9};
+    memset(p, 0, sizeof *p);
+    p->num_vars = 1;
+    p->max_tokens = 64;
+    p->max_depth = 5;
+    p->const_min = -5.0f;
+    p->const_max = 5.0f;
+    p->prob_leaf = 0.1f;
+    p->prob_const_leaf = 0.5f;
+    p->prob_unary = 0.35f;
+    p->n_unary = 18;
+    p->n_binary = 9;
+    memcpy(p->unary_ops, un, sizeof un);
+    memcpy(p->binary_ops, bi, sizeof bi);
+}
+
+/* opcodes.h:42-78: unknown opcodes count as binary; opcode 29 is unary. */
+int sro_op_arity(int op)
+{
+    if (op >= 10 && op <= 27) return 1;
+    if (op == SRO_OP_EXP_CLAMPED) return 1;
+    return 2;
+}
+
+typedef struct {
+    const sro_ga_params *p;
+    int max_tokens;
+    int left;
+    int num_vars;
+    int *tok;
+    float *val;
+    int n;
+    sro_rng *r;
+} build_ctx;
+
+/* utils.cpp:995-1016 */
+static void emit_leaf(build_ctx *c)
+{
+    if (c->left <= 0) return;
+    int choose_const = c->r->uniform(c->r, 0.0f, 1.0f) < c->p->prob_const_leaf;
+    if (c->num_vars <= 0) choose_const = 1;
+    if (choose_const) {
+        c->tok[c->n] = SRO_TOK_CONST;
+        c->val[c->n] = c->r->uniform(c->r, c->p->const_min, c->p->const_max);
+    } else {
+        int v = c->r->randint(c->r, 0, c->num_vars - 1);
+        c->tok[c->n] = SRO_TOK_VAR;
+        c->val[c->n] = (float)v;
+    }
+    c->n++;
+    c->left -= 1;
+}
+
+/* utils.cpp:1018-1084 */
+static void build_node(build_ctx *c, int depth)
+{
+    if (c->left <= 0) return;
+    int must_leaf = depth >= c->p->max_depth || c->left == 1;
+    if (!must_leaf) {
+        if (c->r->uniform(c->r, 0.0f, 1.0f) < c->p->prob_leaf) must_leaf = 1;
+    }
+    if (must_leaf) {
+        emit_leaf(c);
+        return;
+    }
+    int allow_unary = c->left >= 2;
+    int allow_binary = c->left >= 3;
+    if (!allow_unary && !allow_binary) {
+        emit_leaf(c);
+        return;
+    }
+    float thresh = c->p->prob_unary;
+    if (!allow_unary) thresh = 0.0f;
+    if (!allow_binary) thresh = 1.0f;
+    int pick_unary = c->r->uniform(c->r, 0.0f, 1.0f) < thresh;
+    int op, arity;
+    if (pick_unary) {
+        op = c->p->unary_ops[c->r->randint(c->r, 0, c->p->n_unary - 1)];
+        arity = 1;
+    } else {
+        op = c->p->binary_ops[c->r->randint(c->r, 0, c->p->n_binary - 1)];
+        arity = 2;
+    }
+    c->tok[c->n] = op;
+    c->val[c->n] = 0.0f;
+    c->n++;
+    c->left -= 1;
+    for (int i = 0; i < arity; i++) {
+        if (c->left <= 0) {
+            c->tok[c->n] = SRO_TOK_VAR; /* budget exhausted: fallback leaf */
+            c->val[c->n] = 0.0f;
+            c->n++;
+            continue;
+        }
+        build_node(c, depth + 1);
+    }
+}
+
+/* generate_with_limit, utils.cpp:1086-1108 */
+int sro_ga_generate(const sro_ga_params *p, int max_tokens, sro_rng *r, int *tok, float *val)
+{
+    if (max_tokens <= 0) return 0;
+    build_ctx c;
+    c.p = p;
+    c.max_tokens = p->max_tokens < max_tokens ? p->max_tokens : max_tokens;
+    c.left = c.max_tokens;
+    c.num_vars = p->num_vars > 1 ? p->num_vars : 1;
+    c.tok = tok;
+    c.val = val;
+    c.n = 0;
+    c.r = r;
+    build_node(&c, 0);
+    if (c.n == 0) {
+        tok[0] = SRO_TOK_CONST;
+        val[0] = 0.0f;
+        c.n = 1;
+    }
+    return c.n;
+}
+
+/* utils.cpp:1110-1129 (op_arity based; terminals are CONST and VAR only). */
+int sro_ga_subtree_size(const int *tok, int len, int start)
+{
+    if (start < 0 || start >= len) return 0;
+    int t = tok[start];
+    int ar = (t == SRO_TOK_CONST || t == SRO_TOK_VAR) ? 0 : sro_op_arity(t);
+    int consumed = 1, cur = start + 1;
+    for (int i = 0; i < ar; i++) {
+        if (cur >= len) break;
+        int ch = sro_ga_subtree_size(tok, len, cur);
+        if (ch <= 0) break;
+        consumed += ch;
+        cur += ch;
+    }
+    return consumed;
+}
+
+/* splice_subtree, utils.cpp:1138-1153 */
+static int splice(const int *st, const float *sv, int slen, int begin, int count,
+                  const int *it, const float *iv, int ilen, int *dt, float *dv)
+{
+    int n = 0;
+    for (int k = 0; k < begin; k++, n++) { dt[n] = st[k]; dv[n] = sv[k]; }
+    for (int k = 0; k < ilen; k++, n++) { dt[n] = it[k]; dv[n] = iv[k]; }
+    for (int k = begin + count; k < slen; k++, n++) { dt[n] = st[k]; dv[n] = sv[k]; }
+    return n;
+}
+
+static int copy_expr(const int *st, const float *sv, int len, int *dt, float *dv)
+{
+    memcpy(dt, st, sizeof(int) * (size_t)len);
+    memcpy(dv, sv, sizeof(float) * (size_t)len);
+    return len;
+}
+
+#define SRO_GA_TMP 4096
+
+/* mutate_expression, utils.cpp:1163-1204 */
+int sro_ga_mutate(const sro_ga_params *p, const int *ptok, const float *pval, int plen,
+                  sro_rng *r, int *ctok, float *cval)
+{
+    if (plen <= 0) return sro_ga_generate(p, p->max_tokens, r, ctok, cval);
+    int idx = r->randint(r, 0, plen - 1);
+    int old = sro_ga_subtree_size(ptok, plen, idx);
+    if (old <= 0) old = 1;
+    int base = plen - old;
+    int max_new = p->max_tokens - base;
+    if (max_new <= 0) return copy_expr(ptok, pval, plen, ctok, cval);
+    int nt[SRO_GA_TMP];
+    float nv[SRO_GA_TMP];
+    int nlen = sro_ga_generate(p, max_new, r, nt, nv);
+    return splice(ptok, pval, plen, idx, old, nt, nv, nlen, ctok, cval);
+}
+
+/* crossover_expressions, utils.cpp:1206-1272 */
+int sro_ga_crossover(const sro_ga_params *p, const int *ltok, const float *lval, int llen,
+                     const int *rtok, const float *rval, int rlen, sro_rng *r, int *ctok,
+                     float *cval)
+{
+    if (llen <= 0 || rlen <= 0) return copy_expr(ltok, lval, llen, ctok, cval);
+    int li = r->randint(r, 0, llen - 1);
+    int lsub = sro_ga_subtree_size(ltok, llen, li);
+    if (lsub <= 0) lsub = 1;
+    int attempts = rlen < 32 ? rlen : 32;
+    for (int a = 0; a < attempts; a++) {
+        int ri = r->randint(r, 0, rlen - 1);
+        int rsub = sro_ga_subtree_size(rtok, rlen, ri);
+        if (rsub <= 0) continue;
+        if (llen - lsub + rsub > p->max_tokens) continue;
+        return splice(ltok, lval, llen, li, lsub, rtok + ri, rval + ri, rsub, ctok, cval);
+    }
+    return copy_expr(ltok, lval, llen, ctok, cval);
+}
+
+static const float *g_fit_for_sort;
+
+static int fit_cmp(const void *a, const void *b)
+{
+    int i = *(const int *)a, j = *(const int *)b;
+    float fi = g_fit_for_sort[i], fj = g_fit_for_sort[j];
+    int ni = isnan(fi), nj = isnan(fj);
+    if (ni != nj) return ni ? 1 : -1;
+    if (!ni && fi != fj) return fi < fj ? -1 : 1;
+    return (i > j) - (i < j);
+}
+
+/* cl_sr_system.cpp:287-291: ascending fitness, NaN last; ties by index. */
+void sro_fitness_order(const float *fit, int n, int *order)
+{
+    for (int i = 0; i < n; i++) order[i] = i;
+    g_fit_for_sort = fit;
+    qsort(order, (size_t)n, sizeof(int), fit_cmp);
+    g_fit_for_sort = NULL;
+}
+
+/* cl_sr_system.cpp:305-313: winner of t draws, strict < keeps the earlier. */
+static int tournament(sro_rng *r, const float *sfit, int pop, int t)
+{
+    int best = r->randint(r, 0, pop - 1);
+    for (int k = 0; k < t - 1; k++) {
+        int cand = r->randint(r, 0, pop - 1);
+        if (sfit[cand] < sfit[best]) best = cand;
+    }
+    return best;
+}
+
+int sro_ga_next_generation(const sro_ga_params *p, const sro_select_params *sp, int pop,
+                           int cap, const int *tok, const float *val, const int *len,
+                           const float *fit, uint64_t seed, uint32_t gen, int *out_tok,
+                           float *out_val, int *out_len)
+{
+    int *order = (int *)malloc(sizeof(int) * (size_t)pop);
+    float *sfit = (float *)malloc(sizeof(float) * (size_t)pop);
+    sro_fitness_order(fit, pop, order);
+    for (int k = 0; k < pop; k++) sfit[k] = fit[order[k]];
+    int elites = (int)(pop * sp->elite_frac);
+    if (elites < 1) elites = 1;
+    if (elites > pop) elites = pop;
+    int rc = 0;
+    for (int c = 0; c < pop; c++) {
+        int *ct = out_tok + (size_t)c * cap;
+        float *cv = out_val + (size_t)c * cap;
+        int n;
+        if (c < elites) {
+            int s = order[c];
+            n = copy_expr(tok + (size_t)s * cap, val + (size_t)s * cap, len[s], ct, cv);
+        } else {
+            sro_philox_stream st;
+            sro_rng r;
+            sro_philox_stream_init(&st, seed, (uint32_t)c, gen, 1u);
+            sro_rng_bind_philox(&r, &st);
+            int b1 = order[tournament(&r, sfit, pop, sp->tournament)];
+            int tmp[SRO_GA_TMP];
+            float tv[SRO_GA_TMP];
+            if (r.uniform(&r, 0.0f, 1.0f) < sp->p_mutation) {
+                n = sro_ga_mutate(p, tok + (size_t)b1 * cap, val + (size_t)b1 * cap, len[b1],
+                                  &r, tmp, tv);
+            } else {
+                int b2 = order[tournament(&r, sfit, pop, sp->tournament)];
+                n = sro_ga_crossover(p, tok + (size_t)b1 * cap, val + (size_t)b1 * cap, len[b1],
+                                     tok + (size_t)b2 * cap, val + (size_t)b2 * cap, len[b2],
+                                     &r, tmp, tv);
+            }
+            if (n > cap) {
+                rc = -1;
+                n = cap;
+            }
+            memcpy(ct, tmp, sizeof(int) * (size_t)n);
+            memcpy(cv, tv, sizeof(float) * (size_t)n);
+        }
+        out_len[c] = n;
+    }
+    free(order);
+    free(sfit);
+    return rc;
+}
+
+int sro_ga_init_population(const sro_ga_params *p, int pop, int cap, uint64_t seed,
+                           int *out_tok, float *out_val, int *out_len)
+{
+    int rc = 0;
+    for (int i = 0; i < pop; i++) {
+        sro_philox_stream st;
+        sro_rng r;
+        sro_philox_stream_init(&st, seed, (uint32_t)i, 0xFFFFFFFFu, 2u);
+        sro_rng_bind_philox(&r, &st);
+        int tmp[SRO_GA_TMP];
+        float tv[SRO_GA_TMP];
+        int n = sro_ga_generate(p, p->max_tokens, &r, tmp, tv);
+        if (n > cap) {
+            rc = -1;
+            n = cap;
+        }
+        memcpy(out_tok + (size_t)i * cap, tmp, sizeof(int) * (size_t)n);
+        memcpy(out_val + (size_t)i * cap, tv, sizeof(float) * (size_t)n);
+        out_len[i] = n;
+    }
+    return rc;
+}

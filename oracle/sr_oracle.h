@@ -1,0 +1,199 @@
+/*
+ * sr_oracle.h -- CPU restatement of the reference symbolic-regression hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it.  Nothing in symbolic_regression_gpu_amd/ links or calls it.
+ *
+ * Every function cites the reference file:line whose behaviour it restates.
+ * Pinning status per function (what checks it independently of this code):
+ *   sro_op32 / sro_eval_point32 / sro_predict32 / sro_mse32
+ *       pinned: golden vectors from the reference evaluator compiled in place
+ *       (oracle/ref_harness.cpp, cl_cpu_eval.cpp:16-168) + closed forms.
+ *   sro_op64 / sro_eval_point64 / sro_predict64
+ *       pinned: reference fp64 op table compiled in place
+ *       (cpu_simple_single.cpp:6-79) for the ops both share; closed forms.
+ *   sro_hash_flat / sro_subtree_merkle
+ *       pinned: golden u64 values from detect.cpp:50-118 compiled in place.
+ *   sro_detect
+ *       pinned: detect_common_subtrees (detect.cpp:120-387) compiled in place,
+ *       compared as hash sets / occurrence sets / hint maps modulo ids.
+ *   sro_mark_cached / sro_skeleton
+ *       pinned by brute force (O(n^2) subtree comparison) and invariants; the
+ *       reference implementation (cl_gpu_eval.cu:294-329,433-459) is CUDA
+ *       source that cannot be compiled here.
+ *   sro_philox4x32_10
+ *       pinned: Random123 published known-answer vectors.
+ *   sro_ga_generate / sro_ga_mutate / sro_ga_crossover
+ *       pinned: with the RNG hooks bound to std::mt19937 + libstdc++
+ *       distributions, outputs equal the reference's
+ *       generate_random_expression / mutate_expression / crossover_expressions
+ *       (utils.cpp:986-1272) compiled in place, bit for bit.
+ *   sro_ga_next_generation (sort + elitism + tournament, cl_sr_system.cpp:285-330)
+ *       pinned by invariants (elites = best-k, tournament winner = min fitness
+ *       of its draws); the reference code sits inside main() and cannot be
+ *       called.
+ *   sro_op32 for SR_OP_EXP_CLAMPED (a new opcode 29, not in the reference
+ *       evaluator) follows the clamped exp of evolution_tests.cpp:40; parity
+ *       for it is closed-form only.
+ */
+#ifndef SR_ORACLE_H
+#define SR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SRO_TOK_CONST 0
+#define SRO_TOK_VAR (-1)
+#define SRO_OP_REF (-2)
+#define SRO_OP_EXP_CLAMPED 29
+#define SRO_MAX_STACK 512
+
+/* ---- op semantics + interpreter (cl_cpu_eval.cpp:16-90) ---- */
+float sro_op32(int op, float a, float b);
+double sro_op64(int op, double a, double b);
+int sro_is_binary(int tok); /* interpreter arity rule: 1..9 binary */
+
+float sro_eval_point32(const int *tok, const double *val, int len, const float *x, int nv);
+double sro_eval_point64(const int *tok, const double *val, int len, const double *x, int nv);
+
+/* Column-major X[v*N + dp], v < nv; y lives in column nv (not read here). */
+void sro_predict32(const int *tok, const double *val, int len, const float *X, int nv,
+                   int64_t N, float *out);
+void sro_predict64(const int *tok, const double *val, int len, const float *X, int nv,
+                   int64_t N, double *out);
+
+/* Packed population: expression e = tok[off[e] .. off[e]+len[e]).
+ * MSE per cl_cpu_eval.cpp:94-168: fp32 pred, fp32 diff*diff, fp64 sum over
+ * finite preds, NaN when none.  threads<=0 -> 1.  Also returns valid count. */
+void sro_mse32(const int *tok, const double *val, const int *off, const int *len, int E,
+               const float *X, int nv, int64_t N, int threads, float *mse_out,
+               int64_t *valid_out);
+/* FP64 evaluator MSE (pred in fp64, diff^2 in fp64) -- error yardstick. */
+void sro_mse64(const int *tok, const double *val, const int *off, const int *len, int E,
+               const float *X, int nv, int64_t N, int threads, double *mse_out);
+
+/* Maximum stack depth the right-to-left interpreter reaches; -1 if malformed
+ * (underflow, or final depth != 1). */
+int sro_stack_depth(const int *tok, int len);
+
+/* ---- subtree hashing / detection / cache marking ---- */
+int sro_subtree_size(const int *tok, int start, int total); /* detect.cpp:23-47 */
+uint64_t sro_hash_flat(const int *tok, const double *val, int len); /* detect.cpp:50-68 */
+/* Merkle hashes + sizes for every position (detect.cpp:97-118,161-203). */
+void sro_subtree_merkle(const int *tok, const double *val, int len, uint64_t *hash_out,
+                        int *size_out);
+
+typedef struct {
+    int num_subs;
+    int *sub_expr;      /* [num_subs] representative occurrence (expr)  */
+    int *sub_start;     /* [num_subs] representative occurrence (start) */
+    int *sub_size;      /* [num_subs] */
+    uint64_t *sub_hash; /* [num_subs] Merkle hash */
+    int *occ_offset;    /* [num_subs+1] into occ_expr/occ_start */
+    int *occ_expr;
+    int *occ_start;
+    int *hints;         /* same packing as the population: -1, -2 or sub id */
+    int64_t covered_tokens;
+    double avg_subtree_size;
+    double coverage;
+} sro_detect_result;
+
+/* detect_common_subtrees (detect.cpp:120-387).  Ties in both sorts are broken
+ * deterministically (hash, expr, start) and (size desc, hash asc). */
+void sro_detect(const int *tok, const double *val, const int *off, const int *len, int E,
+                int min_size, int min_freq, sro_detect_result *res);
+void sro_detect_free(sro_detect_result *res);
+
+/* Persistent cache keyed by flat hash (cl_gpu_eval.cu:384-397).  The caller
+ * owns keys[capacity]; *num_cached is updated.  Returns number of new slots and
+ * writes the detected-subtree index of each new slot into new_sub[]. */
+int sro_cache_update(const sro_detect_result *det, const int *tok, const double *val,
+                     const int *off, uint64_t *keys, int *num_cached, int capacity,
+                     int *new_sub, int *new_slot);
+/* mark_cached_subtrees_from_state (cl_gpu_eval.cu:294-329): left-to-right scan,
+ * first cached subtree of size >= min_size wins.  hints packed like tok. */
+void sro_mark_cached(const int *tok, const double *val, const int *off, const int *len,
+                     int E, const uint64_t *keys, int num_cached, int min_size, int *hints);
+/* Skeleton build (cl_gpu_eval.cu:433-459): cached subtree -> (OP_REF, slot).
+ * Output arrays sized >= total tokens; returns total skeleton tokens. */
+int sro_skeleton(const int *tok, const double *val, const int *off, const int *len, int E,
+                 const int *hints, int *sk_tok, double *sk_val, int *sk_off, int *sk_len);
+/* Evaluate a skeleton point: OP_REF pushes cache[slot*N + dp]. */
+void sro_predict32_cached(const int *tok, const double *val, int len, const float *X, int nv,
+                          int64_t N, const float *cache, float *out);
+
+/* ---- counter-based RNG ---- */
+void sro_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* GA RNG hooks.  Default binding is the Philox stream below; the reference
+ * harness rebinds them to std::mt19937 to pin the algorithm. */
+typedef struct sro_rng {
+    float (*uniform)(struct sro_rng *r, float lo, float hi);
+    int (*randint)(struct sro_rng *r, int lo, int hi); /* inclusive, lo if hi<=lo */
+    void *state;
+} sro_rng;
+
+typedef struct {
+    uint32_t key[2];
+    uint32_t c1, c2, c3; /* counter words 1..3 identify the stream */
+    uint32_t word;       /* next 32-bit word of the stream */
+} sro_philox_stream;
+
+void sro_philox_stream_init(sro_philox_stream *s, uint64_t seed, uint32_t stream,
+                            uint32_t gen, uint32_t domain);
+uint32_t sro_philox_next(sro_philox_stream *s);
+void sro_rng_bind_philox(sro_rng *r, sro_philox_stream *s);
+
+typedef struct {
+    int num_vars;
+    int max_tokens;
+    int max_depth;
+    float const_min, const_max;
+    float prob_leaf, prob_const_leaf, prob_unary;
+    int n_unary, n_binary;
+    int unary_ops[32];
+    int binary_ops[32];
+} sro_ga_params;
+
+/* Reference defaults (utils.h:178-187, utils.cpp:938-947). */
+void sro_ga_params_default(sro_ga_params *p);
+int sro_op_arity(int op); /* opcodes.h:42-78, with opcode 29 unary */
+
+/* generate_random_expression (utils.cpp:1018-1161).  Returns length. */
+int sro_ga_generate(const sro_ga_params *p, int max_tokens, sro_rng *r, int *tok, float *val);
+int sro_ga_subtree_size(const int *tok, int len, int start); /* utils.cpp:1110-1129 */
+/* mutate_expression (utils.cpp:1163-1204). */
+int sro_ga_mutate(const sro_ga_params *p, const int *ptok, const float *pval, int plen,
+                  sro_rng *r, int *ctok, float *cval);
+/* crossover_expressions (utils.cpp:1206-1272). */
+int sro_ga_crossover(const sro_ga_params *p, const int *ltok, const float *lval, int llen,
+                     const int *rtok, const float *rval, int rlen, sro_rng *r, int *ctok,
+                     float *cval);
+
+typedef struct {
+    int tournament;    /* reference: 3 */
+    float p_mutation;  /* reference: 0.2 */
+    double elite_frac; /* reference: 0.05 */
+} sro_select_params;
+
+/* Sort-by-fitness order (NaN last, ties by index): order[rank] = index. */
+void sro_fitness_order(const float *fit, int n, int *order);
+/* One generation of selection + variation (cl_sr_system.cpp:285-330).
+ * Population stored with fixed stride cap: tok[i*cap + k].  Child c uses the
+ * Philox stream (seed, c, gen, domain 1).  Returns 0 or -1 on overflow. */
+int sro_ga_next_generation(const sro_ga_params *p, const sro_select_params *sp, int pop,
+                           int cap, const int *tok, const float *val, const int *len,
+                           const float *fit, uint64_t seed, uint32_t gen, int *out_tok,
+                           float *out_val, int *out_len);
+/* Initial population: individual i uses stream (seed, i, 0xFFFFFFFF, domain 2). */
+int sro_ga_init_population(const sro_ga_params *p, int pop, int cap, uint64_t seed,
+                           int *out_tok, float *out_val, int *out_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

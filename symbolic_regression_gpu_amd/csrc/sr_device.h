@@ -1,0 +1,145 @@
+// sr_device.h -- device-side op semantics and the per-lane interpreter state
+// shared by every evaluation kernel (gfx950, wave64).
+//
+// Op semantics restate cl_cpu_eval.cpp:16-56 (the reference CPU evaluator):
+// fp32 arithmetic, DIV by 0 -> NaN, LOOSE_* guards with DELTA = 1e-9 and
+// MAX_VAL = 1e9, unknown unary/binary op -> 0.  Opcode 29 is the clamped exp of
+// evolution_tests.cpp:40 (a > 10 -> 1e9, else expf(a)).  All code is built
+// with -ffp-contract=off so no product/sum is fused across tokens.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sr_format.h"
+
+#define SR_DEV __device__ __forceinline__
+
+namespace sr {
+
+constexpr float kDelta = 1e-9f;
+constexpr float kMaxVal = 1e9f;
+
+SR_DEV float qnan() { return __builtin_nanf(""); }
+
+// ---------------------------------------------------------------------------
+// Per-datapoint lane state.  Each datapoint a lane evaluates owns its own
+// small arrays so the compiler keeps each one in VGPRs and indexes it with
+// s_set_gpr_idx (the index -- stack pointer or variable id -- is wave-uniform).
+// ---------------------------------------------------------------------------
+template <int S, int XC>
+struct Lane {
+    float (&stk)[S];
+    float (&x)[XC];
+    float &top;
+    float &o;  // second operand / loaded leaf
+};
+
+// Apply a unary op (10..27, 29) to every lane's top.  op is wave-uniform.
+template <typename... L>
+SR_DEV void apply_unary(int op, L &...l)
+{
+    switch (op) {
+    case OP_SIN: ((l.top = sinf(l.top)), ...); break;
+    case OP_COS: ((l.top = cosf(l.top)), ...); break;
+    case OP_TAN: ((l.top = tanf(l.top)), ...); break;
+    case OP_SINH: ((l.top = sinhf(l.top)), ...); break;
+    case OP_COSH: ((l.top = coshf(l.top)), ...); break;
+    case OP_TANH: ((l.top = tanhf(l.top)), ...); break;
+    case OP_EXP: ((l.top = expf(l.top)), ...); break;
+    case OP_LOG: ((l.top = logf(l.top)), ...); break;
+    case OP_INV: ((l.top = 1.0f / l.top), ...); break;
+    case OP_ASIN: ((l.top = asinf(l.top)), ...); break;
+    case OP_ACOS: ((l.top = acosf(l.top)), ...); break;
+    case OP_ATAN: ((l.top = atanf(l.top)), ...); break;
+    case OP_LOOSE_LOG:
+        ((l.top = (l.top == 0.0f) ? -kMaxVal : logf(fabsf(l.top))), ...);
+        break;
+    case OP_LOOSE_INV:
+        ((l.top = 1.0f / (fabsf(l.top) <= kDelta ? (l.top < 0.0f ? -kDelta : kDelta) : l.top)),
+         ...);
+        break;
+    case OP_ABS: ((l.top = fabsf(l.top)), ...); break;
+    case OP_NEG: ((l.top = -l.top), ...); break;
+    case OP_SQRT: ((l.top = sqrtf(l.top)), ...); break;
+    case OP_LOOSE_SQRT: ((l.top = sqrtf(fabsf(l.top))), ...); break;
+    case OP_EXP_CLAMPED: ((l.top = (l.top > 10.0f) ? kMaxVal : expf(l.top)), ...); break;
+    default: ((l.top = 0.0f), ...); break;
+    }
+}
+
+// top = f(top, o) for binary ops 1..9.
+template <typename... L>
+SR_DEV void apply_binary(int op, L &...l)
+{
+    switch (op) {
+    case OP_ADD: ((l.top = l.top + l.o), ...); break;
+    case OP_SUB: ((l.top = l.top - l.o), ...); break;
+    case OP_MUL: ((l.top = l.top * l.o), ...); break;
+    case OP_DIV: ((l.top = (l.o == 0.0f) ? qnan() : l.top / l.o), ...); break;
+    case OP_POW: ((l.top = powf(l.top, l.o)), ...); break;
+    case OP_MIN: ((l.top = fminf(l.top, l.o)), ...); break;
+    case OP_MAX: ((l.top = fmaxf(l.top, l.o)), ...); break;
+    case OP_LOOSE_DIV:
+        ((l.top = l.top / (fabsf(l.o) <= kDelta ? (l.o < 0.0f ? -kDelta : kDelta) : l.o)), ...);
+        break;
+    case OP_LOOSE_POW:
+        ((l.top = (l.top == 0.0f && l.o == 0.0f) ? 0.0f : powf(fabsf(l.top), l.o)), ...);
+        break;
+    default: ((l.top = 0.0f), ...); break;
+    }
+}
+
+// Recursively declare DPT separate Lane objects (one per datapoint a lane
+// owns) and hand them to f as a parameter pack.
+template <int DPT, int S, int XC, typename F, typename... L>
+SR_DEV void with_lanes(F &&f, L &...ls)
+{
+    if constexpr (sizeof...(L) == DPT) {
+        f(ls...);
+    } else {
+        // Separate locals per datapoint: each array is its own alloca, which
+        // the compiler promotes to VGPRs (a struct of arrays would not be).
+        float stk[S];
+        float x[XC];
+        float top = 0.0f, o = 0.0f;
+        Lane<S, XC> l{stk, x, top, o};
+        with_lanes<DPT, S, XC>(f, ls..., l);
+    }
+}
+
+// Fold helper with the lane's ordinal: g(d, lane).
+template <typename G, typename... L>
+SR_DEV void for_each_lane(G &&g, L &...ls)
+{
+    int d = 0;
+    ((g(d++, ls)), ...);
+}
+
+// ---------------------------------------------------------------------------
+// Wave reductions (wave64, deterministic shuffle-xor tree).
+// ---------------------------------------------------------------------------
+SR_DEV double wave_sum(double v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+SR_DEV long long wave_sum(long long v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+SR_DEV float wave_sum(float v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+SR_DEV int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+}  // namespace sr

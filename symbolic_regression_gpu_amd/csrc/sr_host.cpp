@@ -1,0 +1,472 @@
+// sr_host.cpp -- context, data upload, program build and the plain / batched
+// evaluation paths behind the C ABI (include/sr_hip.h).
+//
+// Evaluation flow (the reference's evaluate_gpu_simple, cl_gpu_simple.cu:187-310,
+// without its materialised sq_err[E*N] buffer and separate reduce launch):
+//   host : validate + compile each expression        (O(total tokens))
+//   H2D  : one async copy of the packed program from pinned staging
+//   GPU  : interpreter with fused squared-error partials -> k_finalize -> mse[E]
+//   D2H  : mse[E] (+ valid counts)
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+
+#include "sr_ctx.h"
+#include "sr_kernels.h"
+#include "sr_launch.h"
+
+namespace sr {
+
+void expr_values_f32(const sr_population &pop, int e, std::vector<float> &out)
+{
+    const int32_t o = pop.offsets[e], L = pop.lengths[e];
+    out.resize(L);
+    if (pop.values) {
+        for (int k = 0; k < L; ++k) out[k] = (float)pop.values[o + k];
+    } else {
+        for (int k = 0; k < L; ++k) out[k] = pop.values_f32[o + k];
+    }
+}
+
+int build_program(Ctx &c, const sr_population &pop, bool raw, Program &out, int num_slots)
+{
+    const int E = pop.num_exprs;
+    out.ins.clear();
+    out.off.assign(E, 0);
+    out.len.assign(E, 0);
+    out.need.assign(E, 0);
+    out.max_need = 0;
+    out.tokens = 0;
+    std::vector<float> vals;
+    std::vector<int> size(MAX_EXPR_LEN), need(MAX_EXPR_LEN);
+    std::vector<Ins> tmp(MAX_EXPR_LEN);
+    for (int e = 0; e < E; ++e) {
+        const int32_t L = pop.lengths[e];
+        const int32_t *tok = pop.tokens + pop.offsets[e];
+        if (L <= 0 || L > MAX_EXPR_LEN)
+            return c.fail(SR_ERR_BAD_EXPR, "expression " + std::to_string(e) + ": length " +
+                                               std::to_string(L) + " outside 1.." +
+                                               std::to_string(MAX_EXPR_LEN));
+        int depth = raw_stack_depth(tok, L);
+        if (depth < 0)
+            return c.fail(SR_ERR_BAD_EXPR,
+                          "expression " + std::to_string(e) + ": malformed prefix token stream");
+        expr_values_f32(pop, e, vals);
+        for (int k = 0; k < L; ++k) {
+            if (tok[k] == TOK_VAR && !(vals[k] >= 0.0f))
+                return c.fail(SR_ERR_BAD_EXPR, "expression " + std::to_string(e) +
+                                                   ": negative variable index");
+            // OP_REF only appears in skeletons built by the cache, whose slots
+            // all hold a value column; anywhere else it is an invalid token.
+            if (tok[k] == OP_REF && !(vals[k] >= 0.0f && vals[k] < (float)num_slots))
+                return c.fail(SR_ERR_BAD_EXPR, "expression " + std::to_string(e) +
+                                                   ": cache reference outside the cache");
+        }
+        out.off[e] = (int32_t)out.ins.size();
+        out.tokens += L;
+        if (raw) {
+            for (int k = 0; k < L; ++k) {
+                Ins ins;
+                lower_token(tok[k], vals[k], c.nv, &ins);
+                out.ins.push_back(ins);
+            }
+            out.len[e] = L;
+            out.need[e] = depth;
+        } else {
+            int sn = 0;
+            int n = compile_expr(tok, vals.data(), L, c.nv, size.data(), need.data(), tmp.data(), &sn);
+            if (n < 0)
+                return c.fail(SR_ERR_BAD_EXPR, "expression " + std::to_string(e) + ": compile failed");
+            out.ins.insert(out.ins.end(), tmp.begin(), tmp.begin() + n);
+            out.len[e] = n;
+            out.need[e] = sn;
+        }
+        out.max_need = std::max(out.max_need, out.need[e]);
+    }
+    // Longest first: the persistent queue then approximates LPT scheduling.
+    out.order.resize(E);
+    std::iota(out.order.begin(), out.order.end(), 0);
+    std::stable_sort(out.order.begin(), out.order.end(),
+                     [&](int a, int b) { return out.len[a] > out.len[b]; });
+    return SR_OK;
+}
+
+static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+int run_batched(Ctx &c, const Program &p, int mode, const float *cache_cols, float *out_dev,
+                const int32_t *out_col_host, sr_stats *st)
+{
+    const int E = (int)p.len.size();
+    if (E == 0) return SR_OK;
+    if (p.max_need > kStackSlots)
+        return c.fail(SR_ERR_UNSUPPORTED, "expression needs more than 8 stack slots");
+    hipError_t he;
+    int xc = pick_xc(c.nv);
+    int dpt = c.tune.dpt;
+    if (xc == 32 && dpt > 2) dpt = 2;
+    if (dpt != 1 && dpt != 2 && dpt != 4 && dpt != 8) dpt = 4;
+
+    // Pack program + metadata into pinned staging.
+    const size_t prog_bytes = p.ins.size() * sizeof(Ins);
+    const size_t meta_ints = (size_t)E * 4;
+    if ((he = c.h_prog.ensure(prog_bytes + 16)) != hipSuccess) return c.hip_fail(he, "hipHostMalloc");
+    if ((he = c.h_meta.ensure(meta_ints * 4)) != hipSuccess) return c.hip_fail(he, "hipHostMalloc");
+    std::memcpy(c.h_prog.p, p.ins.data(), prog_bytes);
+    int32_t *meta = c.h_meta.as<int32_t>();
+    std::memcpy(meta, p.off.data(), E * 4);
+    std::memcpy(meta + E, p.len.data(), E * 4);
+    std::memcpy(meta + 2 * E, p.order.data(), E * 4);
+    if (out_col_host) std::memcpy(meta + 3 * E, out_col_host, E * 4);
+    else std::memset(meta + 3 * E, 0, E * 4);
+
+    const int nq = c.tune.n_queues > 1 ? c.tune.n_queues : 1;
+    const int64_t tile = 64 * dpt;
+    const int64_t slice = round_up((c.N + nq - 1) / nq, 512);
+    int64_t chunk = round_up(std::max<int64_t>(c.tune.chunk, tile), tile);
+    if (chunk > slice) chunk = slice;
+    const int cpq = (int)((slice + chunk - 1) / chunk);
+    const int P = nq * cpq;
+
+    if ((he = c.d_prog.ensure(prog_bytes + 16)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    if ((he = c.d_meta.ensure(meta_ints * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    if ((he = c.d_counters.ensure(64 * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    if (mode == MODE_MSE) {
+        if ((he = c.d_psum.ensure((size_t)E * P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+        if ((he = c.d_pcnt.ensure((size_t)E * P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+        if ((he = c.d_mse.ensure((size_t)E * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+        if ((he = c.d_cnt.ensure((size_t)E * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    }
+    hipStream_t s = c.stream;
+    hipEventRecord(c.ev[0], s);
+    if ((he = hipMemcpyAsync(c.d_prog.p, c.h_prog.p, prog_bytes, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return c.hip_fail(he, "H2D program");
+    if ((he = hipMemcpyAsync(c.d_meta.p, c.h_meta.p, meta_ints * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return c.hip_fail(he, "H2D metadata");
+    hipEventRecord(c.ev[1], s);
+
+    BatchArgs a{};
+    a.progs = c.d_prog.as<Ins>();
+    a.prog_off = c.d_meta.as<int32_t>();
+    a.prog_len = a.prog_off + E;
+    a.order = a.prog_off + 2 * E;
+    a.out_col = a.prog_off + 3 * E;
+    a.X = c.X;
+    a.cache = cache_cols;
+    a.out = out_dev;
+    a.psum = c.d_psum.as<double>();
+    a.pcnt = c.d_pcnt.as<long long>();
+    a.counters = c.d_counters.as<uint32_t>();
+    a.N = c.N;
+    a.slice = slice;
+    a.chunk = chunk;
+    a.n_exprs = E;
+    a.nv = c.nv;
+    a.n_queues = nq;
+    a.chunks_per_queue = cpq;
+    a.P = P;
+
+    int bpc = c.tune.blocks_per_cu;
+    int occ = batched_occupancy(mode, dpt, xc);
+    if (occ <= 0) return c.fail(SR_ERR_UNSUPPORTED, "no kernel variant for this dpt / variable count");
+    if (bpc <= 0 || bpc > occ) bpc = occ;
+    int grid = c.num_cu * bpc;
+    grid = (int)round_up(grid, nq);
+
+    if ((he = launch_zero_u32(a.counters, nq, s)) != hipSuccess) return c.hip_fail(he, "zero counters");
+    if ((he = launch_batched(mode, dpt, xc, a, grid, s)) != hipSuccess) return c.hip_fail(he, "batched launch");
+    int launches = 2;
+    if (mode == MODE_MSE) {
+        if ((he = launch_finalize(a.psum, a.pcnt, P, E, c.d_mse.as<float>(), c.d_cnt.as<long long>(), s)) !=
+            hipSuccess)
+            return c.hip_fail(he, "finalize launch");
+        launches++;
+    }
+    hipEventRecord(c.ev[2], s);
+    if (st) {
+        st->num_kernel_launches += launches;
+        st->executed_node_evals += p.tokens * c.N;
+    }
+    return SR_OK;
+}
+
+static int run_plain(Ctx &c, const Program &p, sr_stats *st)
+{
+    const int E = (int)p.len.size();
+    hipError_t he;
+    const int xc = pick_xc(c.nv);
+    const size_t prog_bytes = p.ins.size() * sizeof(Ins);
+    if (p.max_need > 64) return c.fail(SR_ERR_UNSUPPORTED, "plain interpreter: stack depth above 64");
+    if ((he = c.h_prog.ensure(prog_bytes + 16)) != hipSuccess) return c.hip_fail(he, "hipHostMalloc");
+    std::memcpy(c.h_prog.p, p.ins.data(), prog_bytes);
+    const int P = (int)((c.N + kBlock - 1) / kBlock);
+    if ((he = c.d_prog.ensure(prog_bytes + 16)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    if ((he = c.d_psum.ensure((size_t)E * P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    if ((he = c.d_pcnt.ensure((size_t)E * P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    if ((he = c.d_mse.ensure((size_t)E * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    if ((he = c.d_cnt.ensure((size_t)E * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    hipStream_t s = c.stream;
+    hipEventRecord(c.ev[0], s);
+    if ((he = hipMemcpyAsync(c.d_prog.p, c.h_prog.p, prog_bytes, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return c.hip_fail(he, "H2D program");
+    hipEventRecord(c.ev[1], s);
+    const Ins *dprog = c.d_prog.as<Ins>();
+    double *psum = c.d_psum.as<double>();
+    long long *pcnt = c.d_pcnt.as<long long>();
+    for (int e = 0; e < E; ++e) {
+        int slots = p.need[e] <= 16 ? 16 : 64;
+        he = launch_plain(slots, xc, dprog + p.off[e], p.len[e], c.X, c.N, c.nv, psum + (size_t)e * P,
+                          pcnt + (size_t)e * P, s);
+        if (he != hipSuccess) return c.hip_fail(he, "plain launch");
+    }
+    if ((he = launch_finalize(psum, pcnt, P, E, c.d_mse.as<float>(), c.d_cnt.as<long long>(), s)) != hipSuccess)
+        return c.hip_fail(he, "finalize launch");
+    hipEventRecord(c.ev[2], s);
+    if (st) {
+        st->num_kernel_launches += E + 1;
+        st->executed_node_evals += p.tokens * c.N;
+    }
+    return SR_OK;
+}
+
+int eval_cached(Ctx &c, const sr_population &pop, sr_stats *st);  // sr_cache.cpp
+int eval_jit(Ctx &c, const sr_population &pop, float *mse_out, int64_t *valid_out,
+             sr_stats *st);  // sr_jit.cpp
+
+}  // namespace sr
+
+using namespace sr;
+
+static double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+extern "C" {
+
+int sr_ctx_create(int device, sr_ctx **out)
+{
+    if (!out) return SR_ERR_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return SR_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return SR_ERR_INVALID_ARG;
+    if (hipSetDevice(device) != hipSuccess) return SR_ERR_HIP;
+    sr_ctx *x = new sr_ctx();
+    Ctx &c = x->c;
+    c.device = device;
+    hipDeviceGetAttribute(&c.num_cu, hipDeviceAttributeMultiprocessorCount, device);
+    if (hipStreamCreateWithFlags(&c.own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete x;
+        return SR_ERR_HIP;
+    }
+    c.stream = c.own_stream;
+    for (auto &e : c.ev) hipEventCreate(&e);
+    *out = x;
+    return SR_OK;
+}
+
+int sr_ctx_destroy(sr_ctx *x)
+{
+    if (!x) return SR_OK;
+    Ctx &c = x->c;
+    hipSetDevice(c.device);
+    if (c.stream) hipStreamSynchronize(c.stream);
+    for (DevBuf *b : {&c.X_own, &c.d_prog, &c.d_meta, &c.d_psum, &c.d_pcnt, &c.d_mse, &c.d_cnt,
+                      &c.d_counters, &c.d_pred, &c.cache.cols})
+        b->release();
+    for (HostBuf *b : {&c.h_prog, &c.h_meta, &c.h_out}) b->release();
+    for (auto &e : c.ev)
+        if (e) hipEventDestroy(e);
+    if (c.own_stream) hipStreamDestroy(c.own_stream);
+    delete x;
+    return SR_OK;
+}
+
+const char *sr_last_error(const sr_ctx *x) { return x ? x->c.err.c_str() : "null context"; }
+
+int sr_set_stream(sr_ctx *x, void *stream)
+{
+    if (!x) return SR_ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> g(x->c.mu);
+    x->c.stream = stream ? (hipStream_t)stream : x->c.own_stream;
+    return SR_OK;
+}
+
+static int check_shape(Ctx &c, int32_t nv, int64_t N)
+{
+    if (nv < 0 || nv > MAX_VARS) return c.fail(SR_ERR_INVALID_ARG, "nv must be in 0..32");
+    if (N <= 0) return c.fail(SR_ERR_INVALID_ARG, "N must be positive");
+    return SR_OK;
+}
+
+int sr_set_data(sr_ctx *x, const float *X, int32_t nv, int64_t N)
+{
+    if (!x || !X) return SR_ERR_INVALID_ARG;
+    Ctx &c = x->c;
+    std::lock_guard<std::recursive_mutex> g(c.mu);
+    hipSetDevice(c.device);
+    int rc = check_shape(c, nv, N);
+    if (rc) return rc;
+    size_t bytes = (size_t)(nv + 1) * (size_t)N * sizeof(float);
+    hipError_t he = c.X_own.ensure(bytes);
+    if (he != hipSuccess) return c.hip_fail(he, "hipMalloc X");
+    he = hipMemcpyAsync(c.X_own.p, X, bytes, hipMemcpyHostToDevice, c.stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c.stream);
+    if (he != hipSuccess) return c.hip_fail(he, "H2D X");
+    c.X = c.X_own.as<float>();
+    c.nv = nv;
+    c.N = N;
+    c.data_version++;
+    return SR_OK;
+}
+
+int sr_set_data_device(sr_ctx *x, const float *dX, int32_t nv, int64_t N)
+{
+    if (!x || !dX) return SR_ERR_INVALID_ARG;
+    Ctx &c = x->c;
+    std::lock_guard<std::recursive_mutex> g(c.mu);
+    int rc = check_shape(c, nv, N);
+    if (rc) return rc;
+    c.X = dX;
+    c.nv = nv;
+    c.N = N;
+    c.data_version++;
+    return SR_OK;
+}
+
+static int check_pop(Ctx &c, const sr_population *pop)
+{
+    if (!pop || pop->num_exprs < 0) return c.fail(SR_ERR_INVALID_ARG, "null population");
+    if (pop->num_exprs > 0 &&
+        (!pop->tokens || !pop->offsets || !pop->lengths || (!pop->values && !pop->values_f32)))
+        return c.fail(SR_ERR_INVALID_ARG, "population arrays missing");
+    if (!c.X) return c.fail(SR_ERR_NO_DATA, "no data: call sr_set_data first");
+    return SR_OK;
+}
+
+int sr_eval_mse(sr_ctx *x, const sr_population *pop, int mode, float *mse_out, int64_t *valid_out,
+                sr_stats *st)
+{
+    if (!x || !mse_out) return SR_ERR_INVALID_ARG;
+    Ctx &c = x->c;
+    std::lock_guard<std::recursive_mutex> g(c.mu);
+    hipSetDevice(c.device);
+    int rc = check_pop(c, pop);
+    if (rc) return rc;
+    sr_stats local{};
+    sr_stats *S = st ? st : &local;
+    *S = sr_stats{};
+    const double t0 = now_ms();
+    const int E = pop->num_exprs;
+    int64_t logical = 0;
+    for (int e = 0; e < E; ++e) logical += (int64_t)pop->lengths[e] * c.N;
+    S->logical_node_evals = logical;
+    if (E == 0) return SR_OK;
+
+    if (mode == SR_EVAL_JIT) {
+        rc = eval_jit(c, *pop, mse_out, valid_out, S);
+        S->total_eval_time_ms = now_ms() - t0;
+        return rc;
+    }
+    if (mode == SR_EVAL_CACHED) {
+        rc = eval_cached(c, *pop, S);
+    } else if (mode == SR_EVAL_PLAIN || mode == SR_EVAL_BATCHED) {
+        Program p;
+        rc = build_program(c, *pop, mode == SR_EVAL_PLAIN, p);
+        if (rc) return rc;
+        rc = mode == SR_EVAL_PLAIN ? run_plain(c, p, S) : run_batched(c, p, MODE_MSE, nullptr, nullptr, nullptr, S);
+    } else {
+        return c.fail(SR_ERR_INVALID_ARG, "unknown evaluation mode");
+    }
+    if (rc) return rc;
+    hipError_t he;
+    hipStream_t s = c.stream;
+    if ((he = c.h_out.ensure(((size_t)E * 4 + 7) / 8 * 8 + (size_t)E * 8)) != hipSuccess)
+        return c.hip_fail(he, "hipHostMalloc");
+    float *hm = c.h_out.as<float>();
+    long long *hc = (long long *)(c.h_out.as<char>() + (((size_t)E * 4 + 7) / 8) * 8);
+    if ((he = hipMemcpyAsync(hm, c.d_mse.p, (size_t)E * 4, hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return c.hip_fail(he, "D2H mse");
+    if ((he = hipMemcpyAsync(hc, c.d_cnt.p, (size_t)E * 8, hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return c.hip_fail(he, "D2H counts");
+    hipEventRecord(c.ev[3], s);
+    if ((he = hipStreamSynchronize(s)) != hipSuccess) return c.hip_fail(he, "evaluation");
+    std::memcpy(mse_out, hm, (size_t)E * 4);
+    if (valid_out)
+        for (int e = 0; e < E; ++e) valid_out[e] = hc[e];
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c.ev[0], c.ev[1]);
+    S->transfer_time_ms += ms;
+    hipEventElapsedTime(&ms, c.ev[1], c.ev[2]);
+    S->kernel_time_ms += ms;
+    hipEventElapsedTime(&ms, c.ev[2], c.ev[3]);
+    S->transfer_time_ms += ms;
+    S->total_eval_time_ms = now_ms() - t0;
+    return SR_OK;
+}
+
+int sr_eval_predictions(sr_ctx *x, const sr_population *pop, float *pred_out)
+{
+    if (!x || !pred_out) return SR_ERR_INVALID_ARG;
+    Ctx &c = x->c;
+    std::lock_guard<std::recursive_mutex> g(c.mu);
+    hipSetDevice(c.device);
+    int rc = check_pop(c, pop);
+    if (rc) return rc;
+    const int E = pop->num_exprs;
+    if (E == 0) return SR_OK;
+    Program p;
+    rc = build_program(c, *pop, false, p);
+    if (rc) return rc;
+    size_t bytes = (size_t)E * (size_t)c.N * sizeof(float);
+    hipError_t he = c.d_pred.ensure(bytes);
+    if (he != hipSuccess) return c.hip_fail(he, "hipMalloc predictions");
+    rc = run_batched(c, p, MODE_PRED, nullptr, c.d_pred.as<float>(), nullptr, nullptr);
+    if (rc) return rc;
+    he = hipMemcpyAsync(pred_out, c.d_pred.p, bytes, hipMemcpyDeviceToHost, c.stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c.stream);
+    if (he != hipSuccess) return c.hip_fail(he, "D2H predictions");
+    return SR_OK;
+}
+
+int sr_compile_expr(const int32_t *tokens, const float *values, int32_t len, int32_t nv,
+                    int32_t *out_code, int32_t *out_arg, int32_t *stack_need)
+{
+    if (!tokens || !values || !out_code || !out_arg || !stack_need) return SR_ERR_INVALID_ARG;
+    if (len <= 0 || len > MAX_EXPR_LEN || raw_stack_depth(tokens, len) < 0) return SR_ERR_BAD_EXPR;
+    std::vector<int> size(len), need(len);
+    std::vector<Ins> ins(len);
+    int sn = 0;
+    int n = compile_expr(tokens, values, len, nv, size.data(), need.data(), ins.data(), &sn);
+    if (n < 0) return SR_ERR_BAD_EXPR;
+    for (int i = 0; i < n; ++i) {
+        out_code[i] = ins[i].code;
+        out_arg[i] = ins[i].arg;
+    }
+    *stack_need = sn;
+    return n;
+}
+
+int sr_get_tuning(sr_ctx *x, sr_tuning *out)
+{
+    if (!x || !out) return SR_ERR_INVALID_ARG;
+    *out = x->c.tune;
+    return SR_OK;
+}
+
+int sr_set_tuning(sr_ctx *x, const sr_tuning *t)
+{
+    if (!x || !t) return SR_ERR_INVALID_ARG;
+    Ctx &c = x->c;
+    if (t->dpt != 1 && t->dpt != 2 && t->dpt != 4 && t->dpt != 8)
+        return c.fail(SR_ERR_INVALID_ARG, "dpt must be 1, 2, 4 or 8");
+    if (t->chunk < 64 || (t->n_queues != 1 && t->n_queues != 8) || t->blocks_per_cu < 0)
+        return c.fail(SR_ERR_INVALID_ARG, "bad tuning");
+    c.tune = *t;
+    return SR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,231 @@
+// sr_kernels.h -- evaluation kernels (templated; instantiated per output mode
+// in sr_eval_*.hip so the variants compile in parallel).
+//
+// Data layout in HBM:
+//   X      fp32 column-major [nv+1][N], y = column nv (cl_gpu_simple.cu:79-84)
+//   progs  sr::Ins per expression, packed; prog_off/prog_len per expression
+//   cache  fp32 [slots][N] subtree value columns (cl_gpu_common.h:76-83)
+//   psum/pcnt  per-(expression, partial slot) fp64 sum of squared error and
+//              count of finite predictions; reduced by k_finalize_mse
+#pragma once
+
+#include "sr_device.h"
+
+namespace sr {
+
+enum Mode { MODE_MSE = 0, MODE_PRED = 1, MODE_COLUMN = 2 };
+
+constexpr int kBlock = 256;  // 4 waves per workgroup, one per SIMD
+constexpr int kWave = 64;
+
+struct BatchArgs {
+    const Ins *progs;
+    const int32_t *prog_off;
+    const int32_t *prog_len;
+    const int32_t *order;     // expression id for each rank (longest first)
+    const int32_t *out_col;   // MODE_COLUMN: destination cache slot per expression
+    const float *X;
+    const float *cache;       // OP_REF source columns (may be null)
+    float *out;               // MODE_PRED: [E][N]; MODE_COLUMN: cache base
+    double *psum;             // MODE_MSE: [E][P]
+    long long *pcnt;
+    uint32_t *counters;       // one work counter per XCD queue
+    int64_t N;
+    int64_t slice;            // datapoints per XCD queue (multiple of 64)
+    int64_t chunk;            // datapoints per work item
+    int n_exprs;
+    int nv;
+    int n_queues;             // 1 or 8 (one per XCD)
+    int chunks_per_queue;
+    int P;                    // partial slots per expression = n_queues*chunks_per_queue
+};
+
+// ---------------------------------------------------------------------------
+// Batched persistent interpreter: each wave pulls (expression, datapoint chunk)
+// work items from its XCD's queue; items are ordered longest expression first.
+// Lane l of a wave evaluates datapoints tile + d*64 + l for d < DPT, so every
+// X / y / cache access is a coalesced 256-byte wave load.
+// ---------------------------------------------------------------------------
+template <int DPT, int S, int XC, int MODE>
+__global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int q = a.n_queues > 1 ? (int)(blockIdx.x % (unsigned)a.n_queues) : 0;
+    const int items = a.n_exprs * a.chunks_per_queue;
+    const int64_t N = a.N;
+
+    with_lanes<DPT, S, XC>([&](auto &...L) {
+        for (;;) {
+            int item = 0;
+            if (lane == 0) item = (int)atomicAdd(&a.counters[q], 1u);
+            item = __shfl(item, 0, kWave);
+            if (item >= items) break;
+            const int rank = item / a.chunks_per_queue;
+            const int chunk = item - rank * a.chunks_per_queue;
+            const int e = a.order[rank];
+            const Ins *prog = a.progs + a.prog_off[e];
+            const int n = a.prog_len[e];
+            int64_t q0 = (int64_t)q * a.slice;
+            int64_t q1 = q0 + a.slice < N ? q0 + a.slice : N;
+            int64_t c0 = q0 + (int64_t)chunk * a.chunk;
+            int64_t c1 = c0 + a.chunk < q1 ? c0 + a.chunk : q1;
+            double sum = 0.0;
+            long long cnt = 0;
+            float *col_out = nullptr;
+            if constexpr (MODE == MODE_COLUMN) col_out = a.out + (int64_t)a.out_col[e] * N;
+            if constexpr (MODE == MODE_PRED) col_out = a.out + (int64_t)e * N;
+
+            for (int64_t t0 = c0; t0 < c1; t0 += (int64_t)kWave * DPT) {
+                // Per-lane datapoint index, clamped for loads.
+                for_each_lane(
+                    [&](int d, auto &l) {
+                        int64_t dp = t0 + d * kWave + lane;
+                        int64_t dc = dp < N ? dp : N - 1;
+#pragma unroll
+                        for (int k = 0; k < XC; ++k)
+                            if (k < a.nv) l.x[k] = a.X[(int64_t)k * N + dc];
+                    },
+                    L...);
+                int sp = 0;
+                for (int i = 0; i < n; ++i) {
+                    const Ins ins = prog[i];
+                    const int code = ins.code;
+                    if (code & BC_PUSH) {
+                        ((L.stk[sp] = L.top), ...);
+                        ++sp;
+                    }
+                    const int src = bc_src(code);
+                    const int op = bc_op(code);
+                    if (src == SRC_CONST) {
+                        const float c = __int_as_float(ins.arg);
+                        ((L.o = c), ...);
+                    } else if (src == SRC_VAR) {
+                        const int k = ins.arg;
+                        ((L.o = L.x[k]), ...);
+                    } else if (src == SRC_POP) {
+                        --sp;
+                        ((L.o = L.stk[sp]), ...);
+                    } else if (src == SRC_REF) {
+                        const float *colr = a.cache + (int64_t)ins.arg * N;
+                        for_each_lane(
+                            [&](int d, auto &l) {
+                                int64_t dp = t0 + d * kWave + lane;
+                                l.o = colr[dp < N ? dp : N - 1];
+                            },
+                            L...);
+                    }
+                    if (op == 0) {
+                        ((L.top = L.o), ...);
+                    } else if (op < 10) {
+                        if (code & BC_SWAP) {
+                            for_each_lane(
+                                [&](int, auto &l) {
+                                    float t = l.top;
+                                    l.top = l.o;
+                                    l.o = t;
+                                },
+                                L...);
+                        }
+                        apply_binary(op, L...);
+                    } else {
+                        if (src != SRC_NONE) ((L.top = L.o), ...);
+                        apply_unary(op, L...);
+                    }
+                }
+                // Epilogue.
+                for_each_lane(
+                    [&](int d, auto &l) {
+                        int64_t dp = t0 + d * kWave + lane;
+                        if constexpr (MODE == MODE_MSE) {
+                            int64_t dc = dp < N ? dp : N - 1;
+                            float y = a.X[(int64_t)a.nv * N + dc];
+                            bool ok = dp < c1 && isfinite(l.top);
+                            float diff = l.top - y;
+                            float sq = diff * diff;
+                            sum += ok ? (double)sq : 0.0;
+                            cnt += ok ? 1 : 0;
+                        } else {
+                            if (dp < c1) col_out[dp] = l.top;
+                        }
+                    },
+                    L...);
+            }
+            if constexpr (MODE == MODE_MSE) {
+                sum = wave_sum(sum);
+                cnt = wave_sum(cnt);
+                if (lane == 0) {
+                    int slot = q * a.chunks_per_queue + chunk;
+                    a.psum[(int64_t)e * a.P + slot] = sum;
+                    a.pcnt[(int64_t)e * a.P + slot] = cnt;
+                }
+            }
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Plain interpreter: one launch per expression, one datapoint per thread, raw
+// prefix tokens read right to left (cl_gpu_simple.cu:86-119 restated), MSE
+// fused: one (sum, count) partial per workgroup.
+// ---------------------------------------------------------------------------
+template <int S, int XC>
+__global__ __launch_bounds__(kBlock) void k_plain(const Ins *prog, int len, const float *X,
+                                                  int64_t N, int nv, double *psum,
+                                                  long long *pcnt)
+{
+    const int64_t dp = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t dc = dp < N ? dp : N - 1;
+    double sum = 0.0;
+    long long cnt = 0;
+    with_lanes<1, S, XC>([&](auto &l) {
+#pragma unroll
+        for (int k = 0; k < XC; ++k)
+            if (k < nv) l.x[k] = X[(int64_t)k * N + dc];
+        l.top = 0.0f;
+        int sp = 0;
+        for (int i = len - 1; i >= 0; --i) {
+            const Ins ins = prog[i];
+            const int t = ins.code;
+            if (t <= 0) {
+                l.stk[sp] = l.top;  // the first leaf pushes an unused value
+                ++sp;
+                if (t == TOK_CONST) l.top = __int_as_float(ins.arg);
+                else l.top = l.x[ins.arg];
+            } else if (t < 10) {
+                --sp;
+                l.o = l.stk[sp];
+                apply_binary(t, l);
+            } else {
+                apply_unary(t, l);
+            }
+        }
+        float y = X[(int64_t)nv * N + dc];
+        bool ok = dp < N && isfinite(l.top);
+        float diff = l.top - y;
+        float sq = diff * diff;
+        sum = ok ? (double)sq : 0.0;
+        cnt = ok ? 1 : 0;
+    });
+    __shared__ double s_sum[kBlock / kWave];
+    __shared__ long long s_cnt[kBlock / kWave];
+    sum = wave_sum(sum);
+    cnt = wave_sum(cnt);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        s_sum[w] = sum;
+        s_cnt[w] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        long long c = 0;
+        for (int k = 0; k < kBlock / kWave; ++k) {
+            t += s_sum[k];
+            c += s_cnt[k];
+        }
+        psum[blockIdx.x] = t;
+        pcnt[blockIdx.x] = c;
+    }
+}
+
+}  // namespace sr

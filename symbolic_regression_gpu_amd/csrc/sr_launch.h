@@ -1,0 +1,34 @@
+// sr_launch.h -- host-side launchers for the evaluation kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sr_format.h"
+
+namespace sr {
+
+struct BatchArgs;
+
+constexpr int kStackSlots = 8;  // bytecode stack need <= 8 for any tree of <= 512 leaves
+
+// Batched persistent interpreter.  mode: 0 MSE partials, 1 predictions,
+// 2 cache columns.  Returns hipErrorInvalidValue for an unsupported (dpt, xc).
+hipError_t launch_batched(int mode, int dpt, int xc, const BatchArgs &a, int grid,
+                          hipStream_t s);
+// Max resident workgroups per CU for a variant (0 if unsupported).
+int batched_occupancy(int mode, int dpt, int xc);
+
+// Plain interpreter, one expression.  s_slots >= raw stack depth.
+hipError_t launch_plain(int s_slots, int xc, const Ins *prog, int len, const float *X,
+                        int64_t N, int nv, double *psum, long long *pcnt, hipStream_t s);
+
+// mse[e] = sum / count over P partials (NaN when count == 0).
+hipError_t launch_finalize(const double *psum, const long long *pcnt, int P, int E,
+                           float *mse, long long *cnt, hipStream_t s);
+// Zero n words (graph-capturable: a kernel, not a memset node).
+hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t s);
+
+// Smallest register-cached variable count that holds nv variables.
+inline int pick_xc(int nv) { return nv <= 4 ? 4 : (nv <= 16 ? 16 : 32); }
+
+}  // namespace sr
