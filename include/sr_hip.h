@@ -112,6 +112,26 @@ int sr_eval_mse(sr_ctx *ctx, const sr_population *pop, int mode, float *mse_out,
 /* Predictions, host [num_exprs][N]. */
 int sr_eval_predictions(sr_ctx *ctx, const sr_population *pop, float *pred_out);
 
+/* ---- subtree cache (mode SR_EVAL_CACHED) --------------------------------
+ * capacity: value columns kept on the device (reference MAX_CACHED_SUBTREES
+ * 2048, cl_gpu_common.h:13; each column is N floats); min_size / min_freq:
+ * detection thresholds (reference SR system: 3 / 10, cl_gpu_eval.cu:371-373).
+ * No eviction: once full, further subtrees are not cached (the reference's
+ * policy).  The cache persists across calls until sr_cache_clear or new data. */
+int sr_cache_config(sr_ctx *ctx, int32_t capacity, int32_t min_size, int32_t min_freq);
+int sr_cache_clear(sr_ctx *ctx);
+/* Flat hashes of the cached subtrees, by slot. */
+int sr_cache_state(sr_ctx *ctx, uint64_t *keys_out, int32_t max_keys, int32_t *num_cached);
+/* Cache marking of the last CACHED evaluation, packed like its population:
+ * slot at a cached subtree's root, -2 inside it, -1 elsewhere. */
+int sr_cache_last_hints(sr_ctx *ctx, int32_t *hints_out, int64_t max_len, int64_t *len_out);
+/* Stateless common-subtree detection: hints packed like the population
+ * (subtree id at an occurrence root, -2 inside, -1 elsewhere); per detected
+ * subtree (size descending) its flat hash, size and occurrence count. */
+int sr_detect(sr_ctx *ctx, const sr_population *pop, int32_t min_size, int32_t min_freq,
+              int32_t *hints_out, uint64_t *flat_out, int32_t *size_out, int32_t *count_out,
+              int32_t max_subs, int32_t *num_subs);
+
 /* Compile one expression to the batched interpreter's bytecode (host only, no
  * GPU needed).  out_code/out_arg: [len]; returns the instruction count, or a
  * negative sr_status.  *stack_need = per-lane stack slots the program uses.
@@ -121,6 +141,67 @@ int sr_compile_expr(const int32_t *tokens, const float *values, int32_t len, int
 
 int sr_get_tuning(sr_ctx *ctx, sr_tuning *out);
 int sr_set_tuning(sr_ctx *ctx, const sr_tuning *t);
+
+/* ---- genetic algorithm on a device-resident population ------------------
+ * Replaces the host GA of the reference: generate_random_expression /
+ * mutate_expression / crossover_expressions (src/utils/utils.cpp:1156-1272),
+ * the selection loop of src/cleaned/cl_sr_system.cpp:285-330, and the evogp
+ * device kernels generate / mutate / crossover (src/utils/gpu_kernel.h:15-58).
+ * Randomness: Philox4x32-10 streams keyed by (seed, individual, generation),
+ * so offspring depend only on (seed, generation, population, fitness). */
+typedef struct {
+    int32_t num_vars;
+    int32_t max_tokens;  /* token budget of generate / mutate / crossover */
+    int32_t max_depth;
+    float const_min, const_max;
+    float prob_leaf, prob_const_leaf, prob_unary;
+    int32_t n_unary, n_binary;
+    int32_t unary_ops[32];
+    int32_t binary_ops[32];
+} sr_ga_params;
+
+typedef struct {
+    int32_t tournament; /* reference: 3 */
+    float p_mutation;   /* reference: 0.2 (rest crossover) */
+    double elite_frac;  /* reference: 0.05, at least one elite */
+} sr_select_params;
+
+/* Reference defaults (utils.h:178-187, utils.cpp:938-947). */
+int sr_ga_params_default(sr_ga_params *p);
+/* New random population of pop_size on the device (stream domain "init").
+ * Individuals are stored with a fixed stride (max_tokens + max_depth + 2). */
+int sr_ga_init(sr_ctx *ctx, const sr_ga_params *p, int32_t pop_size, uint64_t seed);
+/* Copy the population out: tok/val [pop*cap] (stride cap), len [pop].
+ * Any pointer may be NULL; pop/cap are always written when non-NULL. */
+int sr_ga_get_population(sr_ctx *ctx, int32_t *tok, float *val, int32_t *len, int32_t *pop,
+                         int32_t *cap);
+/* Replace the population (stride cap must equal the context's). */
+int sr_ga_set_population(sr_ctx *ctx, const int32_t *tok, const float *val, const int32_t *len,
+                         int32_t pop, int32_t cap);
+/* Fitness of the current population (any sr_eval_mode); mse_out host [pop]. */
+int sr_ga_evaluate(sr_ctx *ctx, int mode, float *mse_out, sr_stats *stats);
+/* Next generation from a fitness vector (host [pop]; NULL = the last
+ * sr_ga_evaluate result), generation index gen. */
+int sr_ga_next(sr_ctx *ctx, const sr_select_params *sp, const float *fitness, uint64_t seed,
+               uint32_t gen);
+/* gens generations of evaluate -> fitness -> select -> vary, generation
+ * indices gen0 .. gen0+gens-1.  mode BATCHED with use_graph=1 replays one
+ * captured hipGraph per generation.  mse_hist (optional): host [gens*pop]
+ * fitness of each evaluated generation.  ms_out (optional): wall ms. */
+int sr_ga_run(sr_ctx *ctx, const sr_select_params *sp, int mode, int32_t gens, uint32_t gen0,
+              uint64_t seed, int use_graph, float *mse_hist, double *ms_out, sr_stats *stats);
+
+/* Batch subtree replacement on the device (the evogp mutate / crossover ABI,
+ * src/utils/gpu_kernel.h:31-58, on prefix tokens): child k is left tree
+ * lsel[k] with the subtree at node lidx[k] replaced by the subtree of right
+ * tree rsel[k] at node ridx[k].  Trees are stored with stride cap in host
+ * arrays; a child that would exceed cap, or an index outside its tree, is a
+ * copy of the left tree.  Outputs: otok/oval [n*cap], olen [n]. */
+int sr_tree_splice(sr_ctx *ctx, int32_t n, int32_t cap, const int32_t *ltok, const float *lval,
+                   const int32_t *llen, int32_t n_left, const int32_t *lsel, const int32_t *lidx,
+                   const int32_t *rtok, const float *rval, const int32_t *rlen, int32_t n_right,
+                   const int32_t *rsel, const int32_t *ridx, int32_t *otok, float *oval,
+                   int32_t *olen);
 
 #ifdef __cplusplus
 }

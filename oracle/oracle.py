@@ -97,6 +97,7 @@ def lib():
                                              _i32p, _f32p, _i32p]
         L.sro_fitness_order.argtypes = [_f32p, C.c_int, _i32p]
         L.sro_ga_subtree_size.argtypes = [_i32p, C.c_int, C.c_int]
+        _bind_ga_hooks(L)
         _lib = L
     return _lib
 
@@ -123,6 +124,10 @@ def ref():
         R.ref_detect.restype = C.c_int
         R.ref_detect.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, C.c_int, C.c_int, _i32p,
                                  _i32p, _u64p, _i32p, C.c_int]
+        R.ref_ga_op_log.restype = C.c_int
+        R.ref_ga_op_log.argtypes = [C.c_int, C.c_int, C.POINTER(GaParams), _i32p, _f32p, C.c_int,
+                                    _i32p, _f32p, C.c_int, _i32p, _f32p, _i32p, _f32p, _f32p, _i32p,
+                                    C.c_int, C.POINTER(C.c_int)]
         R.ref_ga_op.restype = C.c_int
         R.ref_ga_op.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(GaParams), _i32p, _f32p,
                                 C.c_int, _i32p, _f32p, C.c_int, _i32p, _f32p]
@@ -317,6 +322,43 @@ def ga_next_generation(p: GaParams, tournament, p_mutation, elite_frac, pop, cap
         np.ascontiguousarray(val, np.float32), np.ascontiguousarray(ln, np.int32),
         np.ascontiguousarray(fit, np.float32), seed, gen, otok, oval, oln)
     return rc, otok, oval, oln
+
+
+UNIFORM_FN = C.CFUNCTYPE(C.c_float, C.c_void_p, C.c_float, C.c_float)
+RANDINT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int)
+
+
+class RngHooks(C.Structure):
+    _fields_ = [("uniform", UNIFORM_FN), ("randint", RANDINT_FN), ("state", C.c_void_p)]
+
+
+def ga_op_with_hooks(op, p: GaParams, a, b, uniform_py, randint_py):
+    """Run the oracle's generate (op 0) / mutate (1) / crossover (2) with Python
+    RNG hooks; a, b = (tok int32, val float32) parents."""
+    L = lib()
+    hooks = RngHooks(UNIFORM_FN(lambda _s, lo, hi: uniform_py(lo, hi)),
+                     RANDINT_FN(lambda _s, lo, hi: randint_py(lo, hi)), None)
+    tok = np.zeros(4096, np.int32)
+    val = np.zeros(4096, np.float32)
+    at = np.ascontiguousarray(a[0], np.int32)
+    av = np.ascontiguousarray(a[1], np.float32)
+    bt = np.ascontiguousarray(b[0], np.int32)
+    bv = np.ascontiguousarray(b[1], np.float32)
+    if op == 0:
+        n = L.sro_ga_generate(C.byref(p), p.max_tokens, C.byref(hooks), tok, val)
+    elif op == 1:
+        n = L.sro_ga_mutate(C.byref(p), at, av, len(at), C.byref(hooks), tok, val)
+    else:
+        n = L.sro_ga_crossover(C.byref(p), at, av, len(at), bt, bv, len(bt), C.byref(hooks), tok, val)
+    return tok[:n].copy(), val[:n].copy()
+
+
+def _bind_ga_hooks(L):
+    L.sro_ga_generate.argtypes = [C.POINTER(GaParams), C.c_int, C.POINTER(RngHooks), _i32p, _f32p]
+    L.sro_ga_mutate.argtypes = [C.POINTER(GaParams), _i32p, _f32p, C.c_int, C.POINTER(RngHooks),
+                                _i32p, _f32p]
+    L.sro_ga_crossover.argtypes = [C.POINTER(GaParams), _i32p, _f32p, C.c_int, _i32p, _f32p, C.c_int,
+                                   C.POINTER(RngHooks), _i32p, _f32p]
 
 
 def fitness_order(fit):

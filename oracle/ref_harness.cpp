@@ -135,6 +135,60 @@ static int out(const std::vector<int> &t, const std::vector<float> &v, int *tok,
     return (int)t.size();
 }
 
+// Draw log of the oracle run under the mt19937 twin (kind 0 = uniform(lo, hi)
+// with the float result's bits, kind 1 = randint(lo, hi)).
+static int *g_log_kind, *g_log_res, g_log_n, g_log_max;
+static float *g_log_lo, *g_log_hi;
+
+static void log_draw(int kind, float lo, float hi, int res)
+{
+    if (g_log_n < g_log_max) {
+        g_log_kind[g_log_n] = kind;
+        g_log_lo[g_log_n] = lo;
+        g_log_hi[g_log_n] = hi;
+        g_log_res[g_log_n] = res;
+    }
+    g_log_n++;
+}
+
+static float logged_uniform(sro_rng *r, float lo, float hi)
+{
+    float f = twin_uniform(r, lo, hi);
+    int bits;
+    memcpy(&bits, &f, 4);
+    log_draw(0, lo, hi, bits);
+    return f;
+}
+
+static int logged_randint(sro_rng *r, int lo, int hi)
+{
+    int v = twin_randint(r, lo, hi);
+    log_draw(1, (float)lo, (float)hi, v);
+    return v;
+}
+
+// Oracle GA under the twin RNG, recording every draw.  Returns the output
+// length; *n_log = number of draws (may exceed max_log: then truncated).
+int ref_ga_op_log(int op, int seed, const sro_ga_params *p, const int *at, const float *av, int alen,
+                  const int *bt, const float *bv, int blen, int *tok, float *val, int *kind,
+                  float *lo, float *hi, int *res, int max_log, int *n_log)
+{
+    g_twin.seed(seed);
+    g_log_kind = kind;
+    g_log_lo = lo;
+    g_log_hi = hi;
+    g_log_res = res;
+    g_log_n = 0;
+    g_log_max = max_log;
+    sro_rng r{logged_uniform, logged_randint, nullptr};
+    int n;
+    if (op == 0) n = sro_ga_generate(p, p->max_tokens, &r, tok, val);
+    else if (op == 1) n = sro_ga_mutate(p, at, av, alen, &r, tok, val);
+    else n = sro_ga_crossover(p, at, av, alen, bt, bv, blen, &r, tok, val);
+    *n_log = g_log_n;
+    return n;
+}
+
 // op: 0 generate, 1 mutate(a), 2 crossover(a, b).  which: 0 reference, 1 oracle.
 int ref_ga_op(int op, int which, int seed, const sro_ga_params *p, const int *at,
               const float *av, int alen, const int *bt, const float *bv, int blen, int *tok,

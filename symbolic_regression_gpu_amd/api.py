@@ -3,7 +3,9 @@
 Mirrors the reference's evaluate entry points (cl_sr_system.cpp:15-28):
 ``Context.eval_mse(pop, mode)`` is evaluate_gpu_simple_wrapper (mode PLAIN /
 BATCHED), evaluate_gpu_mse_wrapper (mode CACHED) and evaluate_gpu_nvrtc_wrapper
-(mode JIT); it returns the per-expression MSE vector and a RunStats-like dict.
+(mode JIT); it returns the per-expression MSE vector, the finite-prediction
+counts and a RunStats-like dict.  The GA methods replace the host GA of
+utils.cpp:1156-1272 and the selection loop of cl_sr_system.cpp:285-330.
 Errors raise SrError instead of exiting the process (cl_gpu_common.h:15-23).
 """
 from __future__ import annotations
@@ -21,8 +23,10 @@ LIB_PATH = os.path.join(_PKG, "libsrhip.so")
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
 
 
 class SrError(RuntimeError):
@@ -33,8 +37,8 @@ class SrError(RuntimeError):
 
 class _Pop(C.Structure):
     _fields_ = [
-        ("num_exprs", C.c_int32), ("tokens", C.c_void_p), ("values", C.c_void_p),
-        ("values_f32", C.c_void_p), ("offsets", C.c_void_p), ("lengths", C.c_void_p),
+        ("num_exprs", C.c_int32), ("tokens", _vp), ("values", _vp),
+        ("values_f32", _vp), ("offsets", _vp), ("lengths", _vp),
     ]
 
 
@@ -58,6 +62,66 @@ class _Tuning(C.Structure):
                 ("blocks_per_cu", C.c_int32)]
 
 
+class GaParams(C.Structure):
+    """sr_ga_params: generate / mutate / crossover settings (utils.h:178-187)."""
+    _fields_ = [
+        ("num_vars", C.c_int32), ("max_tokens", C.c_int32), ("max_depth", C.c_int32),
+        ("const_min", C.c_float), ("const_max", C.c_float),
+        ("prob_leaf", C.c_float), ("prob_const_leaf", C.c_float), ("prob_unary", C.c_float),
+        ("n_unary", C.c_int32), ("n_binary", C.c_int32),
+        ("unary_ops", C.c_int32 * 32), ("binary_ops", C.c_int32 * 32),
+    ]
+
+    @classmethod
+    def make(cls, **kw):
+        p = cls()
+        lib().sr_ga_params_default(C.byref(p))
+        for k, v in kw.items():
+            if k in ("unary_ops", "binary_ops"):
+                arr = getattr(p, k)
+                for i, o in enumerate(v):
+                    arr[i] = int(o)
+                setattr(p, "n_unary" if k == "unary_ops" else "n_binary", len(v))
+            else:
+                setattr(p, k, v)
+        return p
+
+
+class SelectParams(C.Structure):
+    """sr_select_params: tournament size, mutation share, elite fraction."""
+    _fields_ = [("tournament", C.c_int32), ("p_mutation", C.c_float), ("elite_frac", C.c_double)]
+
+
+_SIGNATURES = {
+    "sr_ctx_create": [C.c_int, C.POINTER(_vp)],
+    "sr_ctx_destroy": [_vp],
+    "sr_last_error": [_vp],
+    "sr_set_stream": [_vp, _vp],
+    "sr_set_data": [_vp, _f32p, C.c_int32, C.c_int64],
+    "sr_set_data_device": [_vp, _vp, C.c_int32, C.c_int64],
+    "sr_eval_mse": [_vp, C.POINTER(_Pop), C.c_int, _f32p, _i64p, C.POINTER(_Stats)],
+    "sr_eval_predictions": [_vp, C.POINTER(_Pop), _f32p],
+    "sr_get_tuning": [_vp, C.POINTER(_Tuning)],
+    "sr_set_tuning": [_vp, C.POINTER(_Tuning)],
+    "sr_compile_expr": [_i32p, _f32p, C.c_int32, C.c_int32, _i32p, _i32p, C.POINTER(C.c_int32)],
+    "sr_cache_config": [_vp, C.c_int32, C.c_int32, C.c_int32],
+    "sr_cache_clear": [_vp],
+    "sr_cache_state": [_vp, _vp, C.c_int32, C.POINTER(C.c_int32)],
+    "sr_cache_last_hints": [_vp, _vp, C.c_int64, C.POINTER(C.c_int64)],
+    "sr_detect": [_vp, C.POINTER(_Pop), C.c_int32, C.c_int32, _i32p, _u64p, _i32p, _i32p,
+                  C.c_int32, C.POINTER(C.c_int32)],
+    "sr_ga_params_default": [C.POINTER(GaParams)],
+    "sr_ga_init": [_vp, C.POINTER(GaParams), C.c_int32, C.c_uint64],
+    "sr_ga_get_population": [_vp, _vp, _vp, _vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+    "sr_ga_set_population": [_vp, _i32p, _f32p, _i32p, C.c_int32, C.c_int32],
+    "sr_ga_evaluate": [_vp, C.c_int, _f32p, C.POINTER(_Stats)],
+    "sr_ga_next": [_vp, C.POINTER(SelectParams), _vp, C.c_uint64, C.c_uint32],
+    "sr_ga_run": [_vp, C.POINTER(SelectParams), C.c_int, C.c_int32, C.c_uint32, C.c_uint64,
+                  C.c_int, _vp, C.POINTER(C.c_double), C.POINTER(_Stats)],
+    "sr_tree_splice": [_vp, C.c_int32, C.c_int32, _i32p, _f32p, _i32p, C.c_int32, _i32p, _i32p,
+                       _i32p, _f32p, _i32p, C.c_int32, _i32p, _i32p, _i32p, _f32p, _i32p],
+}
+
 _lib = None
 
 
@@ -69,20 +133,12 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise SrError(-8, f"{LIB_PATH} is missing: run python -m symbolic_regression_gpu_amd.build")
     L = C.CDLL(LIB_PATH)
-    vp = C.c_void_p
-    L.sr_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
-    L.sr_ctx_destroy.argtypes = [vp]
-    L.sr_last_error.argtypes = [vp]
+    for name, args in _SIGNATURES.items():
+        f = getattr(L, name, None)
+        if f is None:
+            continue
+        f.argtypes = args
     L.sr_last_error.restype = C.c_char_p
-    L.sr_set_stream.argtypes = [vp, vp]
-    L.sr_set_data.argtypes = [vp, _f32p, C.c_int32, C.c_int64]
-    L.sr_set_data_device.argtypes = [vp, vp, C.c_int32, C.c_int64]
-    L.sr_eval_mse.argtypes = [vp, C.POINTER(_Pop), C.c_int, _f32p, _i64p, C.POINTER(_Stats)]
-    L.sr_eval_predictions.argtypes = [vp, C.POINTER(_Pop), _f32p]
-    L.sr_get_tuning.argtypes = [vp, C.POINTER(_Tuning)]
-    L.sr_set_tuning.argtypes = [vp, C.POINTER(_Tuning)]
-    L.sr_compile_expr.argtypes = [_i32p, _f32p, C.c_int32, C.c_int32, _i32p, _i32p,
-                                  C.POINTER(C.c_int32)]
     _lib = L
     return L
 
@@ -129,8 +185,8 @@ class Population:
     def from_strided(cls, tok, val, lengths, cap):
         """From a fixed-stride population (individual i at i*cap)."""
         lengths = np.asarray(lengths, dtype=np.int32)
-        idx = np.concatenate([np.arange(i * cap, i * cap + l) for i, l in enumerate(lengths)]) \
-            if len(lengths) else np.zeros(0, np.int64)
+        idx = (np.concatenate([np.arange(i * cap, i * cap + n) for i, n in enumerate(lengths)])
+               if len(lengths) else np.zeros(0, np.int64))
         offs = np.zeros(len(lengths), np.int32)
         if len(lengths) > 1:
             offs[1:] = np.cumsum(lengths[:-1])
@@ -139,6 +195,10 @@ class Population:
     @property
     def num_exprs(self):
         return int(self.lengths.shape[0])
+
+    @property
+    def total_tokens(self):
+        return int(self.lengths.sum())
 
     def expr(self, e):
         a = int(self.offsets[e])
@@ -193,11 +253,12 @@ class Context:
         if rc != 0:
             raise SrError(rc, self._L.sr_last_error(self._h).decode())
 
+    # ---- data -----------------------------------------------------------
     def set_stream(self, stream_ptr: int | None):
         self._check(self._L.sr_set_stream(self._h, stream_ptr))
 
     def set_data(self, X, y=None):
-        """X: [nv, N] (column-major per variable) or [nv+1, N] with y last."""
+        """X: [nv, N] (one row per variable) or [nv+1, N] with y last."""
         X = np.asarray(X, dtype=np.float32)
         if y is not None:
             X = np.vstack([X, np.asarray(y, np.float32)[None, :]])
@@ -210,6 +271,7 @@ class Context:
         self._check(self._L.sr_set_data_device(self._h, ptr, nv, N))
         self.nv, self.N, self._keep = nv, N, keepalive
 
+    # ---- evaluation -----------------------------------------------------
     def eval_mse(self, pop: Population, mode: int = EVAL_BATCHED):
         E = pop.num_exprs
         mse = np.empty(max(E, 1), np.float32)
@@ -233,3 +295,105 @@ class Context:
     def tuning(self, v: Tuning):
         t = _Tuning(v.dpt, v.chunk, v.n_queues, v.blocks_per_cu)
         self._check(self._L.sr_set_tuning(self._h, C.byref(t)))
+
+    # ---- subtree cache ----------------------------------------------------
+    def cache_config(self, capacity=2048, min_size=3, min_freq=10):
+        self._check(self._L.sr_cache_config(self._h, capacity, min_size, min_freq))
+
+    def cache_clear(self):
+        self._check(self._L.sr_cache_clear(self._h))
+
+    def cache_keys(self):
+        n = C.c_int32()
+        self._check(self._L.sr_cache_state(self._h, None, 0, C.byref(n)))
+        keys = np.zeros(max(n.value, 1), np.uint64)
+        self._check(self._L.sr_cache_state(self._h, keys.ctypes.data, n.value, C.byref(n)))
+        return keys[:n.value]
+
+    def cache_last_hints(self):
+        n = C.c_int64()
+        self._check(self._L.sr_cache_last_hints(self._h, None, 0, C.byref(n)))
+        h = np.zeros(max(n.value, 1), np.int32)
+        self._check(self._L.sr_cache_last_hints(self._h, h.ctypes.data, n.value, C.byref(n)))
+        return h[:n.value]
+
+    def detect(self, pop: Population, min_size=3, min_freq=2, max_subs=1 << 16):
+        hints = np.zeros(max(int(pop.offsets[-1] + pop.lengths[-1]) if pop.num_exprs else 1, 1),
+                         np.int32)
+        flat = np.zeros(max_subs, np.uint64)
+        size = np.zeros(max_subs, np.int32)
+        count = np.zeros(max_subs, np.int32)
+        n = C.c_int32()
+        self._check(self._L.sr_detect(self._h, C.byref(pop._c()), min_size, min_freq, hints, flat,
+                                      size, count, max_subs, C.byref(n)))
+        k = min(n.value, max_subs)
+        return {"num_subs": n.value, "flat": flat[:k], "size": size[:k], "count": count[:k],
+                "hints": hints}
+
+    # ---- genetic algorithm ------------------------------------------------
+    def ga_init(self, params: GaParams, pop: int, seed: int):
+        self._check(self._L.sr_ga_init(self._h, C.byref(params), pop, seed))
+
+    def _ga_shape(self):
+        pop, cap = C.c_int32(), C.c_int32()
+        self._check(self._L.sr_ga_get_population(self._h, None, None, None, C.byref(pop),
+                                                 C.byref(cap)))
+        return pop.value, cap.value
+
+    def ga_population(self):
+        """(tok int32[pop*cap], val float32[pop*cap], len int32[pop], cap)."""
+        pop, cap = self._ga_shape()
+        tok = np.empty(pop * cap, np.int32)
+        val = np.empty(pop * cap, np.float32)
+        ln = np.empty(pop, np.int32)
+        p, c = C.c_int32(), C.c_int32()
+        self._check(self._L.sr_ga_get_population(self._h, tok.ctypes.data, val.ctypes.data,
+                                                 ln.ctypes.data, C.byref(p), C.byref(c)))
+        return tok, val, ln, cap
+
+    def ga_set_population(self, tok, val, ln, cap):
+        self._check(self._L.sr_ga_set_population(
+            self._h, np.ascontiguousarray(tok, np.int32), np.ascontiguousarray(val, np.float32),
+            np.ascontiguousarray(ln, np.int32), len(ln), cap))
+
+    def ga_evaluate(self, mode=EVAL_BATCHED):
+        pop, _ = self._ga_shape()
+        m = np.empty(pop, np.float32)
+        st = _Stats()
+        self._check(self._L.sr_ga_evaluate(self._h, mode, m, C.byref(st)))
+        return m, st.as_dict()
+
+    def ga_next(self, sel: SelectParams, fitness, seed: int, gen: int):
+        f = None if fitness is None else np.ascontiguousarray(fitness, np.float32)
+        self._check(self._L.sr_ga_next(self._h, C.byref(sel), None if f is None else f.ctypes.data,
+                                       seed, gen))
+
+    def ga_run(self, sel: SelectParams, mode: int, gens: int, gen0: int, seed: int,
+               use_graph: bool = True):
+        """gens generations; returns (fitness history [gens, pop], ms, stats)."""
+        pop, _ = self._ga_shape()
+        hist = np.empty(max(gens, 1) * pop, np.float32)
+        ms = C.c_double()
+        st = _Stats()
+        self._check(self._L.sr_ga_run(self._h, C.byref(sel), mode, gens, gen0, seed,
+                                      int(use_graph), hist.ctypes.data, C.byref(ms), C.byref(st)))
+        return hist[:gens * pop].reshape(gens, pop), ms.value, st.as_dict()
+
+    def tree_splice(self, cap, left, lsel, lidx, right, rsel, ridx):
+        """Batch subtree replacement; left/right = (tok, val, len) with stride cap."""
+        lt, lv, ll = (np.ascontiguousarray(a, d)
+                      for a, d in zip(left, (np.int32, np.float32, np.int32)))
+        rt, rv, rl = (np.ascontiguousarray(a, d)
+                      for a, d in zip(right, (np.int32, np.float32, np.int32)))
+        n = len(lsel)
+        ot = np.zeros(max(n, 1) * cap, np.int32)
+        ov = np.zeros(max(n, 1) * cap, np.float32)
+        ol = np.zeros(max(n, 1), np.int32)
+
+        def i32(a):
+            return np.ascontiguousarray(a, np.int32)
+
+        self._check(self._L.sr_tree_splice(self._h, n, cap, lt, lv, ll, len(ll), i32(lsel),
+                                           i32(lidx), rt, rv, rl, len(rl), i32(rsel), i32(ridx),
+                                           ot, ov, ol))
+        return ot[:n * cap], ov[:n * cap], ol[:n]

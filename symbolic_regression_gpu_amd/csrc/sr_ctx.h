@@ -12,6 +12,7 @@
 
 #include "../../include/sr_hip.h"
 #include "sr_format.h"
+#include "sr_ga.h"
 
 namespace sr {
 
@@ -78,6 +79,9 @@ struct SubtreeCache {
     std::unordered_map<uint64_t, int> slot_of;
     std::vector<uint64_t> keys;  // keys[slot]
     int capacity = 2048;         // cl_gpu_common.h:13 MAX_CACHED_SUBTREES
+    int min_size = 3;            // detection parameters of cl_gpu_eval.cu:371-373
+    int min_freq = 10;
+    std::vector<int32_t> last_hints;  // cache marking of the last evaluation
     int64_t N = 0;               // columns valid for this N / X upload
     uint64_t data_version = 0;
     DevBuf cols;
@@ -85,6 +89,27 @@ struct SubtreeCache {
     {
         slot_of.clear();
         keys.clear();
+    }
+};
+
+// Device-resident GA population (double-buffered, fixed stride cap) and the
+// buffers of its device-side evaluation.
+struct GaState {
+    bool ready = false;
+    GaParams p{};
+    int pop = 0, cap = 0, cur = 0;
+    DevBuf tok[2], val[2], len[2];
+    DevBuf progs, prog_len, size_s, need_s, frames, order_len, order_fit, status, max_need, gen,
+        hist, hist_len, fit;
+    std::vector<float> last_mse;  // host copy of the last evaluation
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    uint64_t graph_key = 0;  // parameters the captured graphs were built for
+    void release_graphs()
+    {
+        for (auto &g : graph)
+            if (g) (void)hipGraphExecDestroy(g);
+        graph[0] = graph[1] = nullptr;
+        graph_key = 0;
     }
 };
 
@@ -110,6 +135,7 @@ struct Ctx {
     DevBuf d_prog, d_meta, d_psum, d_pcnt, d_mse, d_cnt, d_counters, d_pred;
 
     SubtreeCache cache;
+    GaState ga;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
     int fail(int code, const std::string &msg)
@@ -138,6 +164,19 @@ struct Program {
 int build_program(Ctx &c, const sr_population &pop, bool raw, Program &out, int num_slots = 0);
 // Values of expression e as float (values_f32 or (float)values).
 void expr_values_f32(const sr_population &pop, int e, std::vector<float> &out);
+
+// Launch shape of one batched evaluation (see plan_batched).
+struct LaunchPlan {
+    int dpt = 4, xc = 4, nq = 8, cpq = 1, P = 8, grid = 0;
+    int64_t slice = 0, chunk = 0;
+};
+struct BatchArgs;
+// Pick the variant + work decomposition for E expressions and make sure the
+// partial / result buffers exist.
+int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp);
+// Enqueue counter reset + batched kernel (+ finalize for MODE_MSE).  a must
+// carry the program / order / output pointers; the rest is filled in here.
+int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, hipStream_t s);
 
 // Batched evaluation over a compiled program already described by host
 // vectors; uploads, launches, finalizes.  mode: MODE_MSE/MODE_PRED/MODE_COLUMN.

@@ -146,13 +146,18 @@ SR_HD void leaf_src(int t, float v, int nv, int &src, int32_t &arg)
     }
 }
 
+// Emission frame of compile_expr (caller-provided scratch, len + 1 frames).
+struct CompileFrame {
+    int node, stage, live, second, swap;
+};
+
 // Compile one validated prefix expression into bytecode.
 //   tok/val : token stream (len tokens, already checked by raw_stack_depth)
-//   size/need : scratch arrays of len ints
+//   size/need : scratch arrays of len ints; fr: scratch of len + 1 frames
 // Returns the number of instructions (<= len) and sets *stack_need; -1 on a
 // malformed stream.
 SR_HD int compile_expr(const int32_t *tok, const float *val, int len, int nv, int *size,
-                       int *need, Ins *out, int *stack_need)
+                       int *need, CompileFrame *fr, Ins *out, int *stack_need)
 {
     if (len <= 0 || len > MAX_EXPR_LEN) return -1;
     // Right-to-left pass: subtree sizes and stack need (Sethi-Ullman).
@@ -186,14 +191,10 @@ SR_HD int compile_expr(const int32_t *tok, const float *val, int len, int nv, in
     // Iterative emission.  Frame stages: 0 = enter, 1 = emit after the single
     // evaluated child, 2 = after first of two subtrees (then evaluate second),
     // 3 = emit after the second subtree.
-    struct Frame {
-        int node, stage, live, second, swap;
-    };
-    Frame fr[MAX_EXPR_LEN + 1];
     int fsp = 0, n = 0;
     fr[fsp++] = {0, 0, 0, 0, 0};
     while (fsp > 0) {
-        Frame &f = fr[fsp - 1];
+        CompileFrame &f = fr[fsp - 1];
         int t = tok[f.node];
         int push = f.live ? BC_PUSH : 0;
         if (f.stage == 0) {

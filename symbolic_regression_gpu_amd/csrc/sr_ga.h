@@ -1,0 +1,298 @@
+// sr_ga.h -- genetic-algorithm operators on the prefix-token format, one code
+// path for host and device, driven by a counter-based RNG (Philox4x32-10).
+//
+// Semantics restate the reference's host GA (utils.cpp:986-1272):
+//   generate  = grow generator (prob_leaf, prob_const_leaf, prob_unary, consts
+//               U[const_min, const_max)) bounded by max_depth / a token budget,
+//               with the reference's fallback leaf (VAR 0) when the budget runs
+//               out under a pending child, so a tree may exceed the budget by
+//               up to max_depth tokens;
+//   mutate    = replace the subtree at a uniformly drawn node by a fresh tree
+//               within max_tokens - (len - old subtree);
+//   crossover = splice a uniformly drawn subtree of the right parent into a
+//               uniformly drawn node of the left one, <= 32 tries to fit
+//               max_tokens, else copy the left parent;
+// and the selection of cl_sr_system.cpp:285-330: sort by fitness (NaN last),
+// copy the elite fraction, tournament winners by strict '<' on fitness.
+//
+// Random draws: individual c of generation g draws a private stream
+// Philox(key = seed, counter = {block, c, g, domain}); the draw order equals
+// the recursion order of the reference generator, so CPU and GPU runs of the
+// same (seed, g, fitness vector) produce identical offspring.
+#pragma once
+
+#include <stdint.h>
+
+#include "sr_format.h"
+
+namespace sr {
+
+constexpr int GA_MAX_OPS = 32;
+constexpr uint32_t GA_DOMAIN_NEXT = 1u;   // offspring of a generation
+constexpr uint32_t GA_DOMAIN_INIT = 2u;   // initial population
+constexpr uint32_t GA_GEN_INIT = 0xFFFFFFFFu;
+
+struct GaParams {
+    int32_t num_vars;
+    int32_t max_tokens;
+    int32_t max_depth;
+    float const_min, const_max;
+    float prob_leaf, prob_const_leaf, prob_unary;
+    int32_t n_unary, n_binary;
+    int32_t unary_ops[GA_MAX_OPS];
+    int32_t binary_ops[GA_MAX_OPS];
+};
+
+struct SelectParams {
+    int32_t tournament;
+    float p_mutation;
+    int32_t elites;  // number of elite copies (>= 1)
+};
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+SR_HD void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                         uint32_t k1, uint32_t out[4])
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+struct Rng {
+    uint32_t k0, k1, c1, c2, c3;
+    uint32_t word;
+    uint32_t buf[4];
+
+    SR_HD void init(uint64_t seed, uint32_t stream, uint32_t gen, uint32_t domain)
+    {
+        k0 = (uint32_t)seed;
+        k1 = (uint32_t)(seed >> 32);
+        c1 = stream;
+        c2 = gen;
+        c3 = domain;
+        word = 0;
+    }
+    SR_HD uint32_t next()
+    {
+        if ((word & 3u) == 0u) philox4x32_10(word >> 2, c1, c2, c3, k0, k1, buf);
+        return buf[word++ & 3u];
+    }
+    // [lo, hi): lo + (hi - lo) * u with u = top 24 bits / 2^24; each operation
+    // rounds once (the build disables contraction).
+    SR_HD float uniform(float lo, float hi)
+    {
+        const float u = (float)(next() >> 8) * (1.0f / 16777216.0f);
+        const float span = hi - lo;
+        const float t = span * u;
+        return lo + t;
+    }
+    // [lo, hi] by multiply-shift; no draw when hi <= lo (utils.cpp:973-979).
+    SR_HD int randint(int lo, int hi)
+    {
+        if (hi <= lo) return lo;
+        const uint32_t range = (uint32_t)(hi - lo) + 1u;
+        return lo + (int)(((uint64_t)next() * range) >> 32);
+    }
+};
+
+// opcodes.h:42-78 arity: unary 10..27 (and the clamped exp 29), everything
+// else binary; leaves have none.
+SR_HD int ga_arity(int t)
+{
+    if (t == TOK_CONST || t == TOK_VAR) return 0;
+    if ((t >= 10 && t <= 27) || t == OP_EXP_CLAMPED) return 1;
+    return 2;
+}
+
+// utils.cpp:1110-1129 subtree extent, iteratively.
+SR_HD int ga_subtree_size(const int32_t *tok, int len, int start)
+{
+    if (start < 0 || start >= len) return 0;
+    int need = 1, pos = start;
+    while (need > 0 && pos < len) {
+        need += ga_arity(tok[pos]) - 1;
+        ++pos;
+    }
+    return pos - start;
+}
+
+// Grow generator (utils.cpp:995-1108) writing at most cap tokens into
+// tok/val.  Returns the length, or -1 if cap would be exceeded.
+SR_HD int ga_generate(const GaParams &p, int budget, Rng &r, int32_t *tok, float *val, int cap)
+{
+    if (budget <= 0) return 0;
+    int left = p.max_tokens < budget ? p.max_tokens : budget;
+    const int nvars = p.num_vars > 1 ? p.num_vars : 1;
+    int n = 0;
+    // Pending child slots: depth, or -1 for the top-level call.
+    int16_t stack[2 * 64 + 4];
+    int sp = 0;
+    stack[sp++] = -1;
+    while (sp > 0) {
+        int d = stack[--sp];
+        const bool top = d < 0;
+        if (top) d = 0;
+        if (left <= 0) {
+            if (!top) {  // budget exhausted under a pending child: fallback leaf
+                if (n >= cap) return -1;
+                tok[n] = TOK_VAR;
+                val[n] = 0.0f;
+                ++n;
+            }
+            continue;
+        }
+        bool leaf = d >= p.max_depth || left == 1;
+        if (!leaf && r.uniform(0.0f, 1.0f) < p.prob_leaf) leaf = true;
+        if (!leaf && !(left >= 2)) leaf = true;
+        if (leaf) {
+            if (n >= cap) return -1;
+            bool is_const = r.uniform(0.0f, 1.0f) < p.prob_const_leaf;
+            if (is_const) {
+                tok[n] = TOK_CONST;
+                val[n] = r.uniform(p.const_min, p.const_max);
+            } else {
+                tok[n] = TOK_VAR;
+                val[n] = (float)r.randint(0, nvars - 1);
+            }
+            ++n;
+            --left;
+            continue;
+        }
+        float thresh = p.prob_unary;
+        if (!(left >= 3)) thresh = 1.0f;
+        const bool unary = r.uniform(0.0f, 1.0f) < thresh;
+        int op, arity;
+        if (unary) {
+            op = p.unary_ops[r.randint(0, p.n_unary - 1)];
+            arity = 1;
+        } else {
+            op = p.binary_ops[r.randint(0, p.n_binary - 1)];
+            arity = 2;
+        }
+        if (n >= cap) return -1;
+        tok[n] = op;
+        val[n] = 0.0f;
+        ++n;
+        --left;
+        if (sp + arity > (int)(sizeof(stack) / sizeof(stack[0]))) return -1;
+        for (int k = 0; k < arity; ++k) stack[sp++] = (int16_t)(d + 1);
+    }
+    if (n == 0) {
+        tok[0] = TOK_CONST;
+        val[0] = 0.0f;
+        n = 1;
+    }
+    return n;
+}
+
+SR_HD int ga_copy(const int32_t *st, const float *sv, int len, int32_t *dt, float *dv)
+{
+    for (int k = 0; k < len; ++k) {
+        dt[k] = st[k];
+        dv[k] = sv[k];
+    }
+    return len;
+}
+
+// mutate_expression (utils.cpp:1163-1204); child written in place of a
+// splice: prefix, fresh subtree generated straight into the child, suffix.
+SR_HD int ga_mutate(const GaParams &p, const int32_t *pt, const float *pv, int plen, Rng &r,
+                    int32_t *ct, float *cv, int cap)
+{
+    if (plen <= 0) return ga_generate(p, p.max_tokens, r, ct, cv, cap);
+    const int idx = r.randint(0, plen - 1);
+    int old = ga_subtree_size(pt, plen, idx);
+    if (old <= 0) old = 1;
+    const int base = plen - old;
+    const int max_new = p.max_tokens - base;
+    if (max_new <= 0) return ga_copy(pt, pv, plen, ct, cv);
+    ga_copy(pt, pv, idx, ct, cv);
+    const int tail = plen - (idx + old);
+    const int room = cap - idx - tail;
+    const int nl = ga_generate(p, max_new, r, ct + idx, cv + idx, room);
+    if (nl < 0) return -1;
+    ga_copy(pt + idx + old, pv + idx + old, tail, ct + idx + nl, cv + idx + nl);
+    return idx + nl + tail;
+}
+
+// crossover_expressions (utils.cpp:1206-1272).
+SR_HD int ga_crossover(const GaParams &p, const int32_t *lt, const float *lv, int llen,
+                       const int32_t *rt, const float *rv, int rlen, Rng &r, int32_t *ct,
+                       float *cv, int cap)
+{
+    if (llen <= 0 || rlen <= 0) return ga_copy(lt, lv, llen, ct, cv);
+    const int li = r.randint(0, llen - 1);
+    int lsub = ga_subtree_size(lt, llen, li);
+    if (lsub <= 0) lsub = 1;
+    const int attempts = rlen < 32 ? rlen : 32;
+    for (int a = 0; a < attempts; ++a) {
+        const int ri = r.randint(0, rlen - 1);
+        const int rsub = ga_subtree_size(rt, rlen, ri);
+        if (rsub <= 0) continue;
+        const int total = llen - lsub + rsub;
+        if (total > p.max_tokens) continue;
+        if (total > cap) return -1;
+        ga_copy(lt, lv, li, ct, cv);
+        ga_copy(rt + ri, rv + ri, rsub, ct + li, cv + li);
+        ga_copy(lt + li + lsub, lv + li + lsub, llen - li - lsub, ct + li + rsub, cv + li + rsub);
+        return total;
+    }
+    return ga_copy(lt, lv, llen, ct, cv);
+}
+
+// Strict weak order of cl_sr_system.cpp:287-291 made total: ascending
+// fitness, NaN last, ties by index.
+SR_HD bool fitness_before(float fa, int a, float fb, int b)
+{
+    const bool na = fa != fa, nb = fb != fb;
+    if (na != nb) return nb;
+    if (!na && fa != fb) return fa < fb;
+    return a < b;
+}
+
+// Tournament over the sorted population (cl_sr_system.cpp:305-313).
+SR_HD int ga_tournament(Rng &r, const float *fit, const int32_t *order, int pop, int t)
+{
+    int best = r.randint(0, pop - 1);
+    for (int k = 0; k < t - 1; ++k) {
+        const int cand = r.randint(0, pop - 1);
+        if (fit[order[cand]] < fit[order[best]]) best = cand;
+    }
+    return best;
+}
+
+// One offspring of generation gen: elite copy or tournament + variation.
+// Population layout: individual i at tok/val + i*cap, length len[i].
+SR_HD int ga_child(const GaParams &p, const SelectParams &s, int pop, int cap, const int32_t *tok,
+                   const float *val, const int32_t *len, const float *fit, const int32_t *order,
+                   uint64_t seed, uint32_t gen, int c, int32_t *ct, float *cv)
+{
+    if (c < s.elites) {
+        const int e = order[c];
+        return ga_copy(tok + (size_t)e * cap, val + (size_t)e * cap, len[e], ct, cv);
+    }
+    Rng r;
+    r.init(seed, (uint32_t)c, gen, GA_DOMAIN_NEXT);
+    const int b1 = order[ga_tournament(r, fit, order, pop, s.tournament)];
+    if (r.uniform(0.0f, 1.0f) < s.p_mutation)
+        return ga_mutate(p, tok + (size_t)b1 * cap, val + (size_t)b1 * cap, len[b1], r, ct, cv, cap);
+    const int b2 = order[ga_tournament(r, fit, order, pop, s.tournament)];
+    return ga_crossover(p, tok + (size_t)b1 * cap, val + (size_t)b1 * cap, len[b1],
+                        tok + (size_t)b2 * cap, val + (size_t)b2 * cap, len[b2], r, ct, cv, cap);
+}
+
+}  // namespace sr

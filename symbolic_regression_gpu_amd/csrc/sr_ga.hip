@@ -1,0 +1,265 @@
+// sr_ga.hip -- device kernels of the GA loop and the device-side program
+// builder, so one generation (compile -> evaluate -> fitness -> select ->
+// vary) runs without a host round trip and can be captured in a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include "sr_ga.h"
+#include "sr_ga_launch.h"
+
+namespace sr {
+
+// One thread per individual: initial population (stream domain INIT).
+__global__ void k_ga_init(GaParams p, int pop, int cap, uint64_t seed, int32_t *tok, float *val,
+                          int32_t *len, int32_t *status)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pop) return;
+    Rng r;
+    r.init(seed, (uint32_t)i, GA_GEN_INIT, GA_DOMAIN_INIT);
+    int n = ga_generate(p, p.max_tokens, r, tok + (size_t)i * cap, val + (size_t)i * cap, cap);
+    if (n < 0) {
+        atomicOr((unsigned *)status, 1u);
+        tok[(size_t)i * cap] = TOK_CONST;
+        val[(size_t)i * cap] = 0.0f;
+        n = 1;
+    }
+    len[i] = n;
+}
+
+// order[rank] = i under fitness_before (ascending fitness, NaN last, ties by
+// index).  O(n^2) comparisons; fitness staged through LDS in tiles.
+__global__ __launch_bounds__(256) void k_ga_rank(const float *fit, int n, int32_t *order)
+{
+    __shared__ float tile[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const float fi = i < n ? fit[i] : 0.0f;
+    int rank = 0;
+    for (int base = 0; base < n; base += 256) {
+        const int j = base + threadIdx.x;
+        tile[threadIdx.x] = j < n ? fit[j] : 0.0f;
+        __syncthreads();
+        const int m = n - base < 256 ? n - base : 256;
+        if (i < n)
+            for (int k = 0; k < m; ++k) rank += fitness_before(tile[k], base + k, fi, i) ? 1 : 0;
+        __syncthreads();
+    }
+    if (i < n) order[rank] = i;
+}
+
+// One thread per child of generation *gen.
+__global__ void k_ga_next(GaParams p, SelectParams s, int pop, int cap, const int32_t *tok,
+                          const float *val, const int32_t *len, const float *fit,
+                          const int32_t *order, uint64_t seed, const uint32_t *gen,
+                          int32_t *otok, float *oval, int32_t *olen, int32_t *status)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= pop) return;
+    int n = ga_child(p, s, pop, cap, tok, val, len, fit, order, seed, *gen, c,
+                     otok + (size_t)c * cap, oval + (size_t)c * cap);
+    if (n < 0) {
+        atomicOr((unsigned *)status, 2u);
+        otok[(size_t)c * cap] = TOK_CONST;
+        oval[(size_t)c * cap] = 0.0f;
+        n = 1;
+    }
+    olen[c] = n;
+}
+
+// One thread per expression: validate and compile into the batched
+// interpreter's bytecode (program e at progs + e*cap).  A malformed
+// expression sets status bit 4 and gets the program "LOAD NaN".
+__global__ void k_compile_pop(const int32_t *tok, const float *val, const int32_t *len, int pop,
+                              int cap, int nv, Ins *progs, int32_t *prog_len, int32_t *size_scratch,
+                              int32_t *need_scratch, CompileFrame *frames, int32_t *max_need,
+                              int32_t *status)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= pop) return;
+    const int32_t *t = tok + (size_t)e * cap;
+    const float *v = val + (size_t)e * cap;
+    const int L = len[e];
+    Ins *out = progs + (size_t)e * cap;
+    int sn = 0, n = -1;
+    bool ok = L > 0 && L <= cap && raw_stack_depth(t, L) >= 0;
+    for (int k = 0; ok && k < L; ++k)
+        if ((t[k] == TOK_VAR && !(v[k] >= 0.0f)) || t[k] == OP_REF) ok = false;
+    if (ok)
+        n = compile_expr(t, v, L, nv, size_scratch + (size_t)e * cap, need_scratch + (size_t)e * cap,
+                         frames + (size_t)e * (cap + 1), out, &sn);
+    if (n < 0 || sn > 8) {
+        atomicOr((unsigned *)status, 4u);
+        out[0].code = 0 | (SRC_CONST << 8);
+        out[0].arg = f32_bits(__builtin_nanf(""));
+        n = 1;
+        sn = 0;
+    }
+    prog_len[e] = n;
+    atomicMax(max_need, sn);
+}
+
+// Longest program first (single workgroup counting sort, lengths < 4096).
+__global__ __launch_bounds__(1024) void k_order_by_len(const int32_t *plen, int n, int32_t *order)
+{
+    constexpr int B = 4096;
+    __shared__ int hist[B];
+    for (int k = threadIdx.x; k < B; k += blockDim.x) hist[k] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int l = plen[i];
+        l = l < 0 ? 0 : (l >= B ? B - 1 : l);
+        atomicAdd(&hist[B - 1 - l], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < B; ++k) {
+            int h = hist[k];
+            hist[k] = acc;
+            acc += h;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int l = plen[i];
+        l = l < 0 ? 0 : (l >= B ? B - 1 : l);
+        order[atomicAdd(&hist[B - 1 - l], 1)] = i;
+    }
+}
+
+// hist[(*gen - gen0) * pop + i] = src[i] for 4-byte words (generations
+// outside [gen0, gen0 + max_gens) are dropped).
+__global__ void k_record(const uint32_t *src, int pop, uint32_t *hist, const uint32_t *gen,
+                         uint32_t gen0, int max_gens)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t g = *gen - gen0;
+    if (i < pop && g < (uint32_t)max_gens) hist[(size_t)g * pop + i] = src[i];
+}
+
+__global__ void k_u32_add(uint32_t *p, uint32_t v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p += v;
+}
+
+__global__ void k_u32_set(uint32_t *p, uint32_t v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+static unsigned blocks(int n, int b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t launch_ga_init(const GaParams &p, int pop, int cap, uint64_t seed, int32_t *tok,
+                          float *val, int32_t *len, int32_t *status, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ga_init, dim3(blocks(pop, 64)), dim3(64), 0, s, p, pop, cap, seed, tok, val,
+                       len, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_ga_rank(const float *fit, int n, int32_t *order, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ga_rank, dim3(blocks(n, 256)), dim3(256), 0, s, fit, n, order);
+    return hipGetLastError();
+}
+
+hipError_t launch_ga_next(const GaParams &p, const SelectParams &sp, int pop, int cap,
+                          const int32_t *tok, const float *val, const int32_t *len, const float *fit,
+                          const int32_t *order, uint64_t seed, const uint32_t *gen, int32_t *otok,
+                          float *oval, int32_t *olen, int32_t *status, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ga_next, dim3(blocks(pop, 64)), dim3(64), 0, s, p, sp, pop, cap, tok, val,
+                       len, fit, order, seed, gen, otok, oval, olen, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_compile_pop(const int32_t *tok, const float *val, const int32_t *len, int pop,
+                              int cap, int nv, Ins *progs, int32_t *prog_len, int32_t *size_scratch,
+                              int32_t *need_scratch, CompileFrame *frames, int32_t *max_need,
+                              int32_t *status, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_compile_pop, dim3(blocks(pop, 64)), dim3(64), 0, s, tok, val, len, pop, cap,
+                       nv, progs, prog_len, size_scratch, need_scratch, frames, max_need, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_order_by_len(const int32_t *plen, int n, int32_t *order, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_order_by_len, dim3(1), dim3(1024), 0, s, plen, n, order);
+    return hipGetLastError();
+}
+
+hipError_t launch_record(const void *src, int pop, void *hist, const uint32_t *gen, uint32_t gen0,
+                         int max_gens, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_record, dim3(blocks(pop, 256)), dim3(256), 0, s, (const uint32_t *)src, pop,
+                       (uint32_t *)hist, gen, gen0, max_gens);
+    return hipGetLastError();
+}
+
+hipError_t launch_u32_add(uint32_t *p, uint32_t v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_u32_add, dim3(1), dim3(64), 0, s, p, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_u32_set(uint32_t *p, uint32_t v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_u32_set, dim3(1), dim3(64), 0, s, p, v);
+    return hipGetLastError();
+}
+
+}  // namespace sr
+
+namespace sr {
+
+// Batch subtree replacement (the evogp mutate / crossover kernels of
+// mutation.cu:75-125 and crossover.cu:42-69 restated on prefix tokens):
+// child n = left[ln] with the subtree at node lidx[n] replaced by the subtree
+// of right[rn] at node ridx[n].  A child that would exceed cap, or an index
+// outside its tree, copies the left tree unchanged.
+__global__ void k_splice(int n, int cap, const int32_t *ltok, const float *lval, const int32_t *llen,
+                         const int32_t *lsel, const int32_t *lidx, const int32_t *rtok,
+                         const float *rval, const int32_t *rlen, const int32_t *rsel,
+                         const int32_t *ridx, int32_t *otok, float *oval, int32_t *olen)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int32_t *lt = ltok + (size_t)lsel[k] * cap;
+    const float *lv = lval + (size_t)lsel[k] * cap;
+    const int L = llen[lsel[k]];
+    const int32_t *rt = rtok + (size_t)rsel[k] * cap;
+    const float *rv = rval + (size_t)rsel[k] * cap;
+    const int R = rlen[rsel[k]];
+    int32_t *ct = otok + (size_t)k * cap;
+    float *cv = oval + (size_t)k * cap;
+    const int li = lidx[k], ri = ridx[k];
+    int out;
+    if (li < 0 || li >= L || ri < 0 || ri >= R) {
+        out = ga_copy(lt, lv, L, ct, cv);
+    } else {
+        const int ls = ga_subtree_size(lt, L, li);
+        const int rs = ga_subtree_size(rt, R, ri);
+        const int total = L - ls + rs;
+        if (total > cap) {
+            out = ga_copy(lt, lv, L, ct, cv);
+        } else {
+            ga_copy(lt, lv, li, ct, cv);
+            ga_copy(rt + ri, rv + ri, rs, ct + li, cv + li);
+            ga_copy(lt + li + ls, lv + li + ls, L - li - ls, ct + li + rs, cv + li + rs);
+            out = total;
+        }
+    }
+    olen[k] = out;
+}
+
+hipError_t launch_splice(int n, int cap, const int32_t *ltok, const float *lval, const int32_t *llen,
+                         const int32_t *lsel, const int32_t *lidx, const int32_t *rtok,
+                         const float *rval, const int32_t *rlen, const int32_t *rsel,
+                         const int32_t *ridx, int32_t *otok, float *oval, int32_t *olen, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_splice, dim3(blocks(n, 64)), dim3(64), 0, s, n, cap, ltok, lval, llen, lsel,
+                       lidx, rtok, rval, rlen, rsel, ridx, otok, oval, olen);
+    return hipGetLastError();
+}
+
+}  // namespace sr

@@ -40,6 +40,7 @@ int build_program(Ctx &c, const sr_population &pop, bool raw, Program &out, int 
     out.tokens = 0;
     std::vector<float> vals;
     std::vector<int> size(MAX_EXPR_LEN), need(MAX_EXPR_LEN);
+    std::vector<CompileFrame> frames(MAX_EXPR_LEN + 1);
     std::vector<Ins> tmp(MAX_EXPR_LEN);
     for (int e = 0; e < E; ++e) {
         const int32_t L = pop.lengths[e];
@@ -75,7 +76,8 @@ int build_program(Ctx &c, const sr_population &pop, bool raw, Program &out, int 
             out.need[e] = depth;
         } else {
             int sn = 0;
-            int n = compile_expr(tok, vals.data(), L, c.nv, size.data(), need.data(), tmp.data(), &sn);
+            int n = compile_expr(tok, vals.data(), L, c.nv, size.data(), need.data(), frames.data(),
+                                 tmp.data(), &sn);
             if (n < 0)
                 return c.fail(SR_ERR_BAD_EXPR, "expression " + std::to_string(e) + ": compile failed");
             out.ins.insert(out.ins.end(), tmp.begin(), tmp.begin() + n);
@@ -94,6 +96,61 @@ int build_program(Ctx &c, const sr_population &pop, bool raw, Program &out, int 
 
 static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
+int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp)
+{
+    hipError_t he;
+    lp.xc = pick_xc(c.nv);
+    lp.dpt = c.tune.dpt;
+    if (lp.dpt != 1 && lp.dpt != 2 && lp.dpt != 4 && lp.dpt != 8) lp.dpt = 4;
+    if (lp.xc == 32 && lp.dpt > 2) lp.dpt = 2;
+    lp.nq = c.tune.n_queues > 1 ? c.tune.n_queues : 1;
+    const int64_t tile = 64 * lp.dpt;
+    lp.slice = round_up((c.N + lp.nq - 1) / lp.nq, 512);
+    lp.chunk = round_up(std::max<int64_t>(c.tune.chunk, tile), tile);
+    if (lp.chunk > lp.slice) lp.chunk = lp.slice;
+    lp.cpq = (int)((lp.slice + lp.chunk - 1) / lp.chunk);
+    lp.P = lp.nq * lp.cpq;
+    int occ = batched_occupancy(mode, lp.dpt, lp.xc);
+    if (occ <= 0) return c.fail(SR_ERR_UNSUPPORTED, "no kernel variant for this dpt / variable count");
+    int bpc = c.tune.blocks_per_cu;
+    if (bpc <= 0 || bpc > occ) bpc = occ;
+    lp.grid = (int)round_up((int64_t)c.num_cu * bpc, lp.nq);
+    if ((he = c.d_counters.ensure(64 * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    if (mode == MODE_MSE) {
+        if ((he = c.d_psum.ensure((size_t)E * lp.P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+        if ((he = c.d_pcnt.ensure((size_t)E * lp.P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+        if ((he = c.d_mse.ensure((size_t)E * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+        if ((he = c.d_cnt.ensure((size_t)E * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    }
+    return SR_OK;
+}
+
+int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, hipStream_t s)
+{
+    hipError_t he;
+    a.X = c.X;
+    a.psum = c.d_psum.as<double>();
+    a.pcnt = c.d_pcnt.as<long long>();
+    a.counters = c.d_counters.as<uint32_t>();
+    a.N = c.N;
+    a.slice = lp.slice;
+    a.chunk = lp.chunk;
+    a.n_exprs = E;
+    a.nv = c.nv;
+    a.n_queues = lp.nq;
+    a.chunks_per_queue = lp.cpq;
+    a.P = lp.P;
+    if ((he = launch_zero_u32(a.counters, lp.nq, s)) != hipSuccess) return c.hip_fail(he, "zero counters");
+    if ((he = launch_batched(mode, lp.dpt, lp.xc, a, lp.grid, s)) != hipSuccess)
+        return c.hip_fail(he, "batched launch");
+    if (mode == MODE_MSE) {
+        if ((he = launch_finalize(a.psum, a.pcnt, lp.P, E, c.d_mse.as<float>(), c.d_cnt.as<long long>(),
+                                  s)) != hipSuccess)
+            return c.hip_fail(he, "finalize launch");
+    }
+    return SR_OK;
+}
+
 int run_batched(Ctx &c, const Program &p, int mode, const float *cache_cols, float *out_dev,
                 const int32_t *out_col_host, sr_stats *st)
 {
@@ -102,12 +159,11 @@ int run_batched(Ctx &c, const Program &p, int mode, const float *cache_cols, flo
     if (p.max_need > kStackSlots)
         return c.fail(SR_ERR_UNSUPPORTED, "expression needs more than 8 stack slots");
     hipError_t he;
-    int xc = pick_xc(c.nv);
-    int dpt = c.tune.dpt;
-    if (xc == 32 && dpt > 2) dpt = 2;
-    if (dpt != 1 && dpt != 2 && dpt != 4 && dpt != 8) dpt = 4;
+    LaunchPlan lp;
+    int rc = plan_batched(c, mode, E, lp);
+    if (rc) return rc;
 
-    // Pack program + metadata into pinned staging.
+    // Pack program + metadata into pinned staging, one async copy each.
     const size_t prog_bytes = p.ins.size() * sizeof(Ins);
     const size_t meta_ints = (size_t)E * 4;
     if ((he = c.h_prog.ensure(prog_bytes + 16)) != hipSuccess) return c.hip_fail(he, "hipHostMalloc");
@@ -119,24 +175,8 @@ int run_batched(Ctx &c, const Program &p, int mode, const float *cache_cols, flo
     std::memcpy(meta + 2 * E, p.order.data(), E * 4);
     if (out_col_host) std::memcpy(meta + 3 * E, out_col_host, E * 4);
     else std::memset(meta + 3 * E, 0, E * 4);
-
-    const int nq = c.tune.n_queues > 1 ? c.tune.n_queues : 1;
-    const int64_t tile = 64 * dpt;
-    const int64_t slice = round_up((c.N + nq - 1) / nq, 512);
-    int64_t chunk = round_up(std::max<int64_t>(c.tune.chunk, tile), tile);
-    if (chunk > slice) chunk = slice;
-    const int cpq = (int)((slice + chunk - 1) / chunk);
-    const int P = nq * cpq;
-
     if ((he = c.d_prog.ensure(prog_bytes + 16)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
     if ((he = c.d_meta.ensure(meta_ints * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
-    if ((he = c.d_counters.ensure(64 * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
-    if (mode == MODE_MSE) {
-        if ((he = c.d_psum.ensure((size_t)E * P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
-        if ((he = c.d_pcnt.ensure((size_t)E * P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
-        if ((he = c.d_mse.ensure((size_t)E * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
-        if ((he = c.d_cnt.ensure((size_t)E * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
-    }
     hipStream_t s = c.stream;
     hipEventRecord(c.ev[0], s);
     if ((he = hipMemcpyAsync(c.d_prog.p, c.h_prog.p, prog_bytes, hipMemcpyHostToDevice, s)) != hipSuccess)
@@ -151,40 +191,13 @@ int run_batched(Ctx &c, const Program &p, int mode, const float *cache_cols, flo
     a.prog_len = a.prog_off + E;
     a.order = a.prog_off + 2 * E;
     a.out_col = a.prog_off + 3 * E;
-    a.X = c.X;
     a.cache = cache_cols;
     a.out = out_dev;
-    a.psum = c.d_psum.as<double>();
-    a.pcnt = c.d_pcnt.as<long long>();
-    a.counters = c.d_counters.as<uint32_t>();
-    a.N = c.N;
-    a.slice = slice;
-    a.chunk = chunk;
-    a.n_exprs = E;
-    a.nv = c.nv;
-    a.n_queues = nq;
-    a.chunks_per_queue = cpq;
-    a.P = P;
-
-    int bpc = c.tune.blocks_per_cu;
-    int occ = batched_occupancy(mode, dpt, xc);
-    if (occ <= 0) return c.fail(SR_ERR_UNSUPPORTED, "no kernel variant for this dpt / variable count");
-    if (bpc <= 0 || bpc > occ) bpc = occ;
-    int grid = c.num_cu * bpc;
-    grid = (int)round_up(grid, nq);
-
-    if ((he = launch_zero_u32(a.counters, nq, s)) != hipSuccess) return c.hip_fail(he, "zero counters");
-    if ((he = launch_batched(mode, dpt, xc, a, grid, s)) != hipSuccess) return c.hip_fail(he, "batched launch");
-    int launches = 2;
-    if (mode == MODE_MSE) {
-        if ((he = launch_finalize(a.psum, a.pcnt, P, E, c.d_mse.as<float>(), c.d_cnt.as<long long>(), s)) !=
-            hipSuccess)
-            return c.hip_fail(he, "finalize launch");
-        launches++;
-    }
+    rc = enqueue_batched(c, mode, lp, a, E, s);
+    if (rc) return rc;
     hipEventRecord(c.ev[2], s);
     if (st) {
-        st->num_kernel_launches += launches;
+        st->num_kernel_launches += mode == MODE_MSE ? 3 : 2;
         st->executed_node_evals += p.tokens * c.N;
     }
     return SR_OK;
@@ -438,9 +451,11 @@ int sr_compile_expr(const int32_t *tokens, const float *values, int32_t len, int
     if (!tokens || !values || !out_code || !out_arg || !stack_need) return SR_ERR_INVALID_ARG;
     if (len <= 0 || len > MAX_EXPR_LEN || raw_stack_depth(tokens, len) < 0) return SR_ERR_BAD_EXPR;
     std::vector<int> size(len), need(len);
+    std::vector<CompileFrame> frames(len + 1);
     std::vector<Ins> ins(len);
     int sn = 0;
-    int n = compile_expr(tokens, values, len, nv, size.data(), need.data(), ins.data(), &sn);
+    int n = compile_expr(tokens, values, len, nv, size.data(), need.data(), frames.data(), ins.data(),
+                         &sn);
     if (n < 0) return SR_ERR_BAD_EXPR;
     for (int i = 0; i < n; ++i) {
         out_code[i] = ins[i].code;

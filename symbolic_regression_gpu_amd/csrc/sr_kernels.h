@@ -20,7 +20,7 @@ constexpr int kWave = 64;
 
 struct BatchArgs {
     const Ins *progs;
-    const int32_t *prog_off;
+    const int32_t *prog_off;  // null: fixed stride (device-resident GA population)
     const int32_t *prog_len;
     const int32_t *order;     // expression id for each rank (longest first)
     const int32_t *out_col;   // MODE_COLUMN: destination cache slot per expression
@@ -38,6 +38,7 @@ struct BatchArgs {
     int n_queues;             // 1 or 8 (one per XCD)
     int chunks_per_queue;
     int P;                    // partial slots per expression = n_queues*chunks_per_queue
+    int prog_stride;          // programs at e*prog_stride when prog_off is null
 };
 
 // ---------------------------------------------------------------------------
@@ -63,7 +64,8 @@ __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
             const int rank = item / a.chunks_per_queue;
             const int chunk = item - rank * a.chunks_per_queue;
             const int e = a.order[rank];
-            const Ins *prog = a.progs + a.prog_off[e];
+            const Ins *prog =
+                a.progs + (a.prog_off ? (int64_t)a.prog_off[e] : (int64_t)e * a.prog_stride);
             const int n = a.prog_len[e];
             int64_t q0 = (int64_t)q * a.slice;
             int64_t q1 = q0 + a.slice < N ? q0 + a.slice : N;

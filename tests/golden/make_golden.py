@@ -1,0 +1,162 @@
+"""Writes tests/golden/*.npz from the reference's own CPU code, compiled in
+place by oracle/Makefile into oracle/_ref/libref.so (run in a container that
+holds the reference tree):
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Each fixture holds inputs and the reference's outputs on seeded data:
+  ops.npz      eval_op_cpu (cl_cpu_eval.cpp:16-56) on special and random operands
+  mse.npz      evaluate_cpu_mse (cl_cpu_eval.cpp:94-168) on a seeded population
+  points.npz   evaluate_single_expr (cl_cpu_eval.cpp:59-90) per datapoint
+  hashes.npz   compute_hash (detect.cpp:50-68) of seeded subtrees
+  detect.npz   detect_common_subtrees (detect.cpp:120-387): sizes, counts,
+               flat hashes of detected subtrees, hint map
+  ga.npz       generate_random_expression / mutate_expression /
+               crossover_expressions (utils.cpp:1156-1272) under set_seed(k)
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as O  # noqa: E402
+
+OPS = list(range(1, 28))
+
+
+def population(n, seed, nv, max_tokens, max_depth):
+    p = O.ga_params(num_vars=nv, max_tokens=max_tokens, max_depth=max_depth,
+                    const_min=-3.0, const_max=3.0)
+    cap = max_tokens + max_depth + 2
+    rc, tok, val, ln = O.ga_init_population(p, n, cap, seed)
+    assert rc == 0
+    return O.Population([(tok[i * cap:i * cap + ln[i]], val[i * cap:i * cap + ln[i]].astype(np.float64))
+                         for i in range(n)])
+
+
+def main():
+    R = O.ref()
+    rng = np.random.default_rng(2024)
+    # ---- op table -------------------------------------------------------
+    special = np.array([0.0, -0.0, 1.0, -1.0, 0.5, -0.5, 2.0, 1e-9, -1e-9, 5e-10, -5e-10, 1e-10,
+                        10.0, 10.5, 88.0, 89.0, -100.0, 1e30, -1e30, np.inf, -np.inf, np.nan],
+                       np.float32)
+    rnd = rng.uniform(-6, 6, size=12).astype(np.float32)
+    vals = np.concatenate([special, rnd])
+    a_list, b_list, op_list = [], [], []
+    for op in OPS:
+        for a in vals:
+            for b in (vals if op < 10 else vals[:1]):
+                op_list.append(op)
+                a_list.append(a)
+                b_list.append(b)
+    ops = np.array(op_list, np.int32)
+    a = np.array(a_list, np.float32)
+    b = np.array(b_list, np.float32)
+    out = np.array([R.ref_op32(int(o), float(x), float(y)) for o, x, y in zip(ops, a, b)], np.float32)
+    np.savez(os.path.join(HERE, "ops.npz"), op=ops, a=a, b=b, out=out)
+
+    # ---- MSE + per-point values ------------------------------------------
+    pop = population(40, 77, nv=3, max_tokens=40, max_depth=6)
+    N = 512
+    X = rng.uniform(1, 5, size=(4, N)).astype(np.float32)
+    X[3] = rng.normal(0, 2, size=N).astype(np.float32)
+    mse = np.empty(pop.E, np.float32)
+    R.ref_mse32(pop.tok, pop.val, pop.off, pop.len, pop.E, X.ravel(), 3, N, mse)
+    np.savez(os.path.join(HERE, "mse.npz"), tok=pop.tok, val=pop.val, off=pop.off, len=pop.len,
+             X=X, mse=mse)
+    pts = np.empty((pop.E, 64), np.float32)
+    for e in range(pop.E):
+        t, v = pop.expr(e)
+        for k in range(64):
+            pts[e, k] = R.ref_eval_point32(np.ascontiguousarray(t), np.ascontiguousarray(v), len(t),
+                                           np.ascontiguousarray(X[:3, k]), 3)
+    np.savez(os.path.join(HERE, "points.npz"), tok=pop.tok, val=pop.val, off=pop.off, len=pop.len,
+             X=X[:, :64].copy(), pred=pts)
+
+    # ---- hashes ------------------------------------------------------------
+    hp = population(30, 5, nv=4, max_tokens=30, max_depth=5)
+    h = np.array([R.ref_hash_flat(*[np.ascontiguousarray(z) for z in hp.expr(e)], int(hp.len[e]))
+                  for e in range(hp.E)], np.uint64)
+    np.savez(os.path.join(HERE, "hashes.npz"), tok=hp.tok, val=hp.val, off=hp.off, len=hp.len,
+             flat=h)
+
+    # ---- detection: a population with planted shared subtrees ------------
+    base = population(12, 9, nv=2, max_tokens=24, max_depth=5)
+    exprs = [base.expr(e) for e in range(base.E)]
+    shared = [([1, 3, -1, 0], [0, 0, 1, 2.5]), ([10, 2, -1, -1], [0, 0, 0, 1])]
+    for k in range(24):
+        t, v = shared[k % 2]
+        bt, bv = exprs[k % base.E]
+        exprs.append(([3] + list(t) + list(bt), [0.0] + list(v) + list(bv)))
+    dpop = O.Population(exprs)
+    maxs = 4096
+    ss = np.zeros(maxs, np.int32)
+    so = np.zeros(maxs, np.int32)
+    sf = np.zeros(maxs, np.uint64)
+    hints = np.zeros(int(dpop.len.sum()), np.int32)
+    res = {}
+    for (mn, mf) in ((3, 2), (3, 10), (2, 2)):
+        n = R.ref_detect(dpop.tok, dpop.val, dpop.off, dpop.len, dpop.E, mn, mf, ss, so, sf, hints, maxs)
+        res[f"n_{mn}_{mf}"] = np.int32(n)
+        res[f"size_{mn}_{mf}"] = ss[:n].copy()
+        res[f"occ_{mn}_{mf}"] = so[:n].copy()
+        res[f"flat_{mn}_{mf}"] = sf[:n].copy()
+        res[f"hints_{mn}_{mf}"] = hints.copy()
+    np.savez(os.path.join(HERE, "detect.npz"), tok=dpop.tok, val=dpop.val, off=dpop.off,
+             len=dpop.len, **res)
+
+    # ---- GA operators under the reference's own mt19937 -------------------
+    gp = O.ga_params(num_vars=3, max_tokens=48, max_depth=6, const_min=-2.0, const_max=2.0)
+    parents = population(20, 31, nv=3, max_tokens=48, max_depth=6)
+    # For each case: the reference's output, and the draw sequence (kind, lo,
+    # hi, result) of its mt19937 + libstdc++ distributions, recorded through
+    # the oracle run under the same engine (whose output is checked equal to
+    # the reference's here).  A test replays the draws into the oracle.
+    rows = []
+    for seed in range(60):
+        for op in range(3):
+            a = parents.expr(seed % 20)
+            bexp = parents.expr((seed * 7 + 3) % 20)
+            at = np.ascontiguousarray(a[0], np.int32)
+            av = np.ascontiguousarray(a[1], np.float32)
+            bt = np.ascontiguousarray(bexp[0], np.int32)
+            bv = np.ascontiguousarray(bexp[1], np.float32)
+            t = np.zeros(4096, np.int32)
+            v = np.zeros(4096, np.float32)
+            n = R.ref_ga_op(op, 0, seed, C.byref(gp), at, av, len(at), bt, bv, len(bt), t, v)
+            t2 = np.zeros(4096, np.int32)
+            v2 = np.zeros(4096, np.float32)
+            kind = np.zeros(8192, np.int32)
+            lo = np.zeros(8192, np.float32)
+            hi = np.zeros(8192, np.float32)
+            res = np.zeros(8192, np.int32)
+            nl = C.c_int()
+            n2 = R.ref_ga_op_log(op, seed, C.byref(gp), at, av, len(at), bt, bv, len(bt), t2, v2,
+                                 kind, lo, hi, res, 8192, C.byref(nl))
+            assert n2 == n and np.array_equal(t[:n], t2[:n]) and np.array_equal(v[:n], v2[:n])
+            assert nl.value <= 8192
+            k = nl.value
+            rows.append((seed, op, seed % 20, (seed * 7 + 3) % 20, t[:n].copy(), v[:n].copy(),
+                         kind[:k].copy(), lo[:k].copy(), hi[:k].copy(), res[:k].copy()))
+    np.savez(os.path.join(HERE, "ga.npz"),
+             seed=np.array([r[0] for r in rows], np.int32), op=np.array([r[1] for r in rows], np.int32),
+             a=np.array([r[2] for r in rows], np.int32), b=np.array([r[3] for r in rows], np.int32),
+             out_len=np.array([len(r[4]) for r in rows], np.int32),
+             out_tok=np.concatenate([r[4] for r in rows]), out_val=np.concatenate([r[5] for r in rows]),
+             log_len=np.array([len(r[6]) for r in rows], np.int32),
+             log_kind=np.concatenate([r[6] for r in rows]), log_lo=np.concatenate([r[7] for r in rows]),
+             log_hi=np.concatenate([r[8] for r in rows]), log_res=np.concatenate([r[9] for r in rows]),
+             p_tok=parents.tok, p_val=parents.val, p_off=parents.off, p_len=parents.len,
+             params=np.array([3, 48, 6], np.int32), consts=np.array([-2.0, 2.0], np.float32))
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    O.build(ref=True)
+    main()

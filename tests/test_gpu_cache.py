@@ -1,0 +1,151 @@
+"""Subtree-cached evaluation (mode CACHED) against the oracle.
+
+* detection output (flat hashes, sizes, counts, hint map) and the cache's
+  slot keys / per-position hit map equal the oracle's restatement of
+  detect.cpp:120-387 and cl_gpu_eval.cu:294-329,384-397 exactly;
+* cached MSEs equal uncached batched MSEs bit for bit (a cached column holds
+  the value the full evaluation computes at that node), and the oracle's
+  MSEs within 1e-5.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from symbolic_regression_gpu_amd import EVAL_BATCHED, EVAL_CACHED
+from tests.helpers import (GRAMMAR_BINARY, GRAMMAR_UNARY, mse_close, random_population, to_api,
+                           uniform_data)
+
+pytestmark = pytest.mark.gpu
+
+
+def mutants(base, n, seed, nv):
+    """C3 shape: n copies of base, each with one subtree of depth 1..3
+    replaced by a fresh random subtree of depth 1..3."""
+    rng = np.random.default_rng(seed)
+    bt, bv = base
+    fresh = random_population(4 * n, seed + 1000, nv=nv, max_tokens=15, max_depth=3,
+                              unary=GRAMMAR_UNARY, binary=GRAMMAR_BINARY)
+    hs, sz = O.merkle(bt, bv)
+    # depth of each subtree
+    depth = np.zeros(len(bt), int)
+    for i in range(len(bt) - 1, -1, -1):
+        t = bt[i]
+        d, pos = 1, i + 1
+        for _ in range(0 if t <= 0 else (2 if t < 10 else 1)):
+            d = max(d, 1 + depth[pos])
+            pos += sz[pos]
+        depth[i] = d
+    cand = np.flatnonzero((depth >= 1) & (depth <= 3) & (np.arange(len(bt)) > 0))
+    out = []
+    for k in range(n):
+        i = int(rng.choice(cand))
+        ft, fv = fresh.expr(k)
+        t = np.concatenate([bt[:i], ft, bt[i + sz[i]:]])
+        v = np.concatenate([bv[:i], fv, bv[i + sz[i]:]])
+        out.append((t, v))
+    return out
+
+
+def c3_population(seed=3):
+    nv = 5
+    base = random_population(1, seed, nv=nv, max_tokens=45, max_depth=7, unary=GRAMMAR_UNARY,
+                             binary=GRAMMAR_BINARY, min_len=36)
+    b = base.expr(0)
+    return O.Population([b] + mutants(b, 100, seed, nv)), nv
+
+
+def oracle_cache_replay(pops, capacity, min_size=3, min_freq=10):
+    keys = np.zeros(capacity, np.uint64)
+    n = np.zeros(1, np.int32)
+    out = []
+    import ctypes as C
+    L = O.lib()
+    for pop in pops:
+        det = O.DetectResult()
+        L.sro_detect(pop.tok, pop.val, pop.off, pop.len, pop.E, min_size, min_freq, C.byref(det))
+        new_sub = np.zeros(max(det.num_subs, 1), np.int32)
+        new_slot = np.zeros(max(det.num_subs, 1), np.int32)
+        nc = C.c_int(int(n[0]))
+        L.sro_cache_update.argtypes = [C.POINTER(O.DetectResult), np.ctypeslib.ndpointer(np.int32),
+                                       np.ctypeslib.ndpointer(np.float64),
+                                       np.ctypeslib.ndpointer(np.int32),
+                                       np.ctypeslib.ndpointer(np.uint64), C.POINTER(C.c_int), C.c_int,
+                                       np.ctypeslib.ndpointer(np.int32),
+                                       np.ctypeslib.ndpointer(np.int32)]
+        L.sro_cache_update(C.byref(det), pop.tok, pop.val, pop.off, keys, C.byref(nc), capacity,
+                           new_sub, new_slot)
+        L.sro_detect_free(C.byref(det))
+        n[0] = nc.value
+        hints = O.mark_cached(pop, keys[:nc.value], min_size)
+        out.append((keys[:nc.value].copy(), hints))
+    return out
+
+
+def test_detect_matches_oracle(ctx):
+    pop, nv = c3_population()
+    X = uniform_data(nv, 1000, 1)
+    ctx.set_data(X)
+    for mn, mf in [(3, 10), (3, 2), (5, 20)]:
+        got = ctx.detect(to_api(pop), mn, mf)
+        ref = O.detect(pop, mn, mf)
+        assert got["num_subs"] == ref["num_subs"]
+        assert np.array_equal(got["flat"], ref["sub_flat"])
+        assert np.array_equal(got["size"], ref["sub_size"])
+        assert np.array_equal(got["count"], [len(o) for o in ref["occurrences"]])
+        assert np.array_equal(got["hints"], ref["hints"])
+
+
+@pytest.mark.parametrize("capacity", [2048, 7])
+def test_cached_c3_matches_batched_and_oracle(ctx, capacity):
+    pop, nv = c3_population()
+    N = 200_001
+    X = uniform_data(nv, N, 2)
+    X[nv] = np.cos(X[0]) + X[1]
+    ctx.set_data(X)
+    ctx.cache_config(capacity, 3, 10)
+    ctx.cache_clear()
+    api = to_api(pop)
+    m_b, v_b, _ = ctx.eval_mse(api, EVAL_BATCHED)
+    m_c, v_c, st = ctx.eval_mse(api, EVAL_CACHED)
+    assert np.array_equal(m_c, m_b, equal_nan=True)
+    assert np.array_equal(v_c, v_b)
+    m_ref, v_ref = O.mse32(pop, X, nv, N, threads=8)
+    assert mse_close(m_c, m_ref).all()
+    keys, hints = oracle_cache_replay([pop], capacity)[0]
+    assert np.array_equal(ctx.cache_keys(), keys)
+    assert np.array_equal(ctx.cache_last_hints(), hints)
+    assert st["num_cached"] == len(keys)
+    assert st["executed_node_evals"] < st["logical_node_evals"]
+    saved = 1 - st["executed_node_evals"] / st["logical_node_evals"]
+    print(f"C3 capacity={capacity}: {st['num_subtrees']} detected, {st['num_cached']} cached, "
+          f"coverage {st['coverage']:.3f}, node-evals saved {saved:.3f}")
+    # second call: everything already cached, no new columns
+    m_c2, _, st2 = ctx.eval_mse(api, EVAL_CACHED)
+    assert np.array_equal(m_c2, m_c, equal_nan=True)
+    assert st2["num_new_subtrees"] == 0
+
+
+def test_cache_persists_across_generations(ctx):
+    nv = 3
+    X = uniform_data(nv, 30_000, 5)
+    ctx.set_data(X)
+    ctx.cache_config(64, 3, 10)
+    ctx.cache_clear()
+    base = random_population(12, 41, nv=nv, max_tokens=40, max_depth=6, unary=GRAMMAR_UNARY,
+                             binary=GRAMMAR_BINARY, min_len=20)
+    pops = []
+    for g in range(3):
+        exprs = []
+        for k in range(60):
+            b = base.expr((k + g) % 12)
+            exprs += mutants(b, 1, 100 * g + k, nv)
+        pops.append(O.Population(exprs))
+    replay = oracle_cache_replay(pops, 64)
+    for g, pop in enumerate(pops):
+        api = to_api(pop)
+        m_c, _, _ = ctx.eval_mse(api, EVAL_CACHED)
+        m_b, _, _ = ctx.eval_mse(api, EVAL_BATCHED)
+        assert np.array_equal(m_c, m_b, equal_nan=True), g
+        keys, hints = replay[g]
+        assert np.array_equal(ctx.cache_keys(), keys), g
+        assert np.array_equal(ctx.cache_last_hints(), hints), g
