@@ -132,6 +132,26 @@ int sr_detect(sr_ctx *ctx, const sr_population *pop, int32_t min_size, int32_t m
               int32_t *hints_out, uint64_t *flat_out, int32_t *size_out, int32_t *count_out,
               int32_t max_subs, int32_t *num_subs);
 
+/* ---- constant fitting ------------------------------------------------------
+ * No counterpart in the reference (it has no gradient code); these serve the
+ * differentiable-fitting callers (the torch.autograd.Function shim).
+ * const_slot: per token of pop, -1 (fixed) or the free-constant slot 0..7 of
+ * its expression (CONST tokens only; tokens sharing a slot share a value).
+ * init_vals (optional, host [E*8]): the population's values of each slot.
+ * The eval / grad calls take DEVICE pointers and only enqueue work on the
+ * context stream (sr_set_stream); sr_sync waits for it.
+ *   cvals [E*8] constants, mse [E], grad [E*8] = dMSE/dconstant. */
+int sr_param_prepare(sr_ctx *ctx, const sr_population *pop, const int32_t *const_slot,
+                     float *init_vals);
+int sr_param_info(sr_ctx *ctx, int32_t *num_exprs, int32_t *nslots_out, int64_t *tokens);
+/* Forward MSE with the given constants. */
+int sr_param_eval(sr_ctx *ctx, const float *d_cvals, float *d_mse);
+/* Reverse-mode adjoint: MSE and dMSE/dconstants in one launch. */
+int sr_param_grad(sr_ctx *ctx, const float *d_cvals, float *d_mse, float *d_grad);
+/* Central differences (step h) on the GPU: 2 forward passes per used slot. */
+int sr_param_fd_grad(sr_ctx *ctx, const float *d_cvals, float h, float *d_grad);
+int sr_sync(sr_ctx *ctx);
+
 /* Compile one expression to the batched interpreter's bytecode (host only, no
  * GPU needed).  out_code/out_arg: [len]; returns the instruction count, or a
  * negative sr_status.  *stack_need = per-lane stack slots the program uses.

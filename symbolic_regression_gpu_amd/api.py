@@ -118,6 +118,12 @@ _SIGNATURES = {
     "sr_ga_next": [_vp, C.POINTER(SelectParams), _vp, C.c_uint64, C.c_uint32],
     "sr_ga_run": [_vp, C.POINTER(SelectParams), C.c_int, C.c_int32, C.c_uint32, C.c_uint64,
                   C.c_int, _vp, C.POINTER(C.c_double), C.POINTER(_Stats)],
+    "sr_param_prepare": [_vp, C.POINTER(_Pop), _i32p, _vp],
+    "sr_param_info": [_vp, C.POINTER(C.c_int32), _vp, C.POINTER(C.c_int64)],
+    "sr_param_eval": [_vp, _vp, _vp],
+    "sr_param_grad": [_vp, _vp, _vp, _vp],
+    "sr_param_fd_grad": [_vp, _vp, C.c_float, _vp],
+    "sr_sync": [_vp],
     "sr_tree_splice": [_vp, C.c_int32, C.c_int32, _i32p, _f32p, _i32p, C.c_int32, _i32p, _i32p,
                        _i32p, _f32p, _i32p, C.c_int32, _i32p, _i32p, _i32p, _f32p, _i32p],
 }
@@ -378,6 +384,36 @@ class Context:
         self._check(self._L.sr_ga_run(self._h, C.byref(sel), mode, gens, gen0, seed,
                                       int(use_graph), hist.ctypes.data, C.byref(ms), C.byref(st)))
         return hist[:gens * pop].reshape(gens, pop), ms.value, st.as_dict()
+
+    # ---- constant fitting (device pointers) ---------------------------------
+    def param_prepare(self, pop: Population, const_slot):
+        """Free constants: const_slot[t] = slot 0..7 of token t (CONST tokens) or
+        -1.  Returns the initial constant table float32 [E, 8]."""
+        init = np.zeros(max(pop.num_exprs, 1) * 8, np.float32)
+        self._check(self._L.sr_param_prepare(self._h, C.byref(pop._c()),
+                                             np.ascontiguousarray(const_slot, np.int32),
+                                             init.ctypes.data))
+        return init[:pop.num_exprs * 8].reshape(pop.num_exprs, 8)
+
+    def param_nslots(self):
+        n = C.c_int32()
+        t = C.c_int64()
+        self._check(self._L.sr_param_info(self._h, C.byref(n), None, C.byref(t)))
+        out = np.zeros(max(n.value, 1), np.int32)
+        self._check(self._L.sr_param_info(self._h, C.byref(n), out.ctypes.data, C.byref(t)))
+        return out[:n.value], t.value
+
+    def param_eval(self, d_cvals: int, d_mse: int):
+        self._check(self._L.sr_param_eval(self._h, d_cvals, d_mse))
+
+    def param_grad(self, d_cvals: int, d_mse: int, d_grad: int):
+        self._check(self._L.sr_param_grad(self._h, d_cvals, d_mse, d_grad))
+
+    def param_fd_grad(self, d_cvals: int, h: float, d_grad: int):
+        self._check(self._L.sr_param_fd_grad(self._h, d_cvals, h, d_grad))
+
+    def sync(self):
+        self._check(self._L.sr_sync(self._h))
 
     def tree_splice(self, cap, left, lsel, lidx, right, rsel, ridx):
         """Batch subtree replacement; left/right = (tok, val, len) with stride cap."""

@@ -16,10 +16,14 @@
 
 namespace sr {
 
-// Grow-only device buffer.
+// Grow-only device buffer (freed with its owner).
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
     hipError_t ensure(size_t need)
     {
         if (need <= bytes) return hipSuccess;
@@ -49,6 +53,10 @@ struct DevBuf {
 struct HostBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf &) = delete;
+    HostBuf &operator=(const HostBuf &) = delete;
+    ~HostBuf() { release(); }
     hipError_t ensure(size_t need)
     {
         if (need <= bytes) return hipSuccess;
@@ -104,6 +112,7 @@ struct GaState {
     std::vector<float> last_mse;  // host copy of the last evaluation
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     uint64_t graph_key = 0;  // parameters the captured graphs were built for
+    ~GaState() { release_graphs(); }
     void release_graphs()
     {
         for (auto &g : graph)
@@ -111,6 +120,16 @@ struct GaState {
         graph[0] = graph[1] = nullptr;
         graph_key = 0;
     }
+};
+
+// Parametric population for constant fitting (free constants in a device
+// array [E][8]).
+struct ParamState {
+    bool ready = false;
+    int E = 0, max_len = 0;
+    int64_t tokens = 0;
+    std::vector<int32_t> nslots;
+    DevBuf progs, meta, psum, pcnt, pgrad, tape, fd_mse;
 };
 
 struct Ctx {
@@ -136,6 +155,7 @@ struct Ctx {
 
     SubtreeCache cache;
     GaState ga;
+    ParamState par;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
     int fail(int code, const std::string &msg)

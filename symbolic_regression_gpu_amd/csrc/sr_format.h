@@ -154,10 +154,13 @@ struct CompileFrame {
 // Compile one validated prefix expression into bytecode.
 //   tok/val : token stream (len tokens, already checked by raw_stack_depth)
 //   size/need : scratch arrays of len ints; fr: scratch of len + 1 frames
+//   leaf_pos : optional [len] output, token position of the leaf each
+//              instruction loads (-1 for none / a popped operand)
 // Returns the number of instructions (<= len) and sets *stack_need; -1 on a
 // malformed stream.
 SR_HD int compile_expr(const int32_t *tok, const float *val, int len, int nv, int *size,
-                       int *need, CompileFrame *fr, Ins *out, int *stack_need)
+                       int *need, CompileFrame *fr, Ins *out, int *stack_need,
+                       int *leaf_pos = nullptr)
 {
     if (len <= 0 || len > MAX_EXPR_LEN) return -1;
     // Right-to-left pass: subtree sizes and stack need (Sethi-Ullman).
@@ -202,6 +205,7 @@ SR_HD int compile_expr(const int32_t *tok, const float *val, int len, int nv, in
                 int src;
                 int32_t arg;
                 leaf_src(t, val[f.node], nv, src, arg);
+                if (leaf_pos) leaf_pos[n] = f.node;
                 out[n++] = {0 | (src << 8) | push, arg};
                 --fsp;
                 continue;
@@ -212,6 +216,7 @@ SR_HD int compile_expr(const int32_t *tok, const float *val, int len, int nv, in
                     int src;
                     int32_t arg;
                     leaf_src(tok[c], val[c], nv, src, arg);
+                    if (leaf_pos) leaf_pos[n] = c;
                     out[n++] = {t | (src << 8) | push, arg};
                     --fsp;
                 } else {
@@ -229,8 +234,10 @@ SR_HD int compile_expr(const int32_t *tok, const float *val, int len, int nv, in
                 int src;
                 int32_t arg;
                 leaf_src(tok[a], val[a], nv, src, arg);
+                if (leaf_pos) leaf_pos[n] = a;
                 out[n++] = {0 | (src << 8) | push, arg};
                 leaf_src(tok[b], val[b], nv, src, arg);
+                if (leaf_pos) leaf_pos[n] = b;
                 out[n++] = {t | (src << 8), arg};
                 --fsp;
             } else if (la) {
@@ -255,11 +262,13 @@ SR_HD int compile_expr(const int32_t *tok, const float *val, int len, int nv, in
         }
         if (f.stage == 1) {
             if (f.second < 0) {
+                if (leaf_pos) leaf_pos[n] = -1;
                 out[n++] = {t | (SRC_NONE << 8), 0};
             } else {
                 int src;
                 int32_t arg;
                 leaf_src(tok[f.second], val[f.second], nv, src, arg);
+                if (leaf_pos) leaf_pos[n] = f.second;
                 out[n++] = {t | (src << 8) | f.swap, arg};
             }
             --fsp;
@@ -271,10 +280,40 @@ SR_HD int compile_expr(const int32_t *tok, const float *val, int len, int nv, in
             continue;
         }
         // stage 3: o = pop (first subtree's value), top = second subtree
+        if (leaf_pos) leaf_pos[n] = -1;
         out[n++] = {t | (SRC_POP << 8) | f.swap, 0};
         --fsp;
     }
     return n;
+}
+
+// ---- parametric programs (constant fitting) ---------------------------------
+// Bits 16..19 of a SRC_CONST instruction hold 1 + the free-constant slot it
+// reads (0 = a fixed constant); a SRC_POP instruction's arg holds the index of
+// the instruction whose output it pops (the one before the matching push),
+// which is where the adjoint pass finds that operand on its tape.
+constexpr int MAX_SLOTS = 8;
+SR_HD int bc_slot(int code) { return ((code >> 16) & 15) - 1; }
+
+// const_slot: per token of the expression, -1 or its free-constant slot.
+// Returns 0, or -1 if a slot is out of range.
+SR_HD int param_postpass(Ins *ins, int n, const int *leaf_pos, const int32_t *const_slot,
+                         int *pop_stack)
+{
+    int sp = 0;
+    for (int i = 0; i < n; ++i) {
+        int code = ins[i].code;
+        if (code & BC_PUSH) pop_stack[sp++] = i - 1;
+        const int src = bc_src(code);
+        if (src == SRC_POP) {
+            ins[i].arg = pop_stack[--sp];
+        } else if (src == SRC_CONST && leaf_pos && const_slot && leaf_pos[i] >= 0) {
+            const int s = const_slot[leaf_pos[i]];
+            if (s >= MAX_SLOTS) return -1;
+            if (s >= 0) ins[i].code = code | ((s + 1) << 16);
+        }
+    }
+    return 0;
 }
 
 }  // namespace sr

@@ -287,10 +287,6 @@ int sr_ctx_destroy(sr_ctx *x)
     Ctx &c = x->c;
     hipSetDevice(c.device);
     if (c.stream) hipStreamSynchronize(c.stream);
-    for (DevBuf *b : {&c.X_own, &c.d_prog, &c.d_meta, &c.d_psum, &c.d_pcnt, &c.d_mse, &c.d_cnt,
-                      &c.d_counters, &c.d_pred, &c.cache.cols})
-        b->release();
-    for (HostBuf *b : {&c.h_prog, &c.h_meta, &c.h_out}) b->release();
     for (auto &e : c.ev)
         if (e) hipEventDestroy(e);
     if (c.own_stream) hipStreamDestroy(c.own_stream);
