@@ -96,8 +96,11 @@ typedef struct {
 int sr_ctx_create(int device, sr_ctx **out);
 int sr_ctx_destroy(sr_ctx *ctx);
 const char *sr_last_error(const sr_ctx *ctx);
-/* Run on a caller-owned hipStream_t (NULL = the context's own stream). */
+/* Run on a caller-owned hipStream_t; NULL is the device's null (default)
+ * stream, which is what torch's default stream reports as its handle. */
 int sr_set_stream(sr_ctx *ctx, void *hip_stream);
+/* Go back to the context's own non-blocking stream (the state after create). */
+int sr_use_own_stream(sr_ctx *ctx);
 
 /* Upload X|y, column-major [nv+1][N] fp32, y = column nv (host pointer). */
 int sr_set_data(sr_ctx *ctx, const float *X_colmajor, int32_t nv, int64_t N);

@@ -69,6 +69,8 @@ def lib():
         L.sro_op64.argtypes = [C.c_int, C.c_double, C.c_double]
         L.sro_predict32.argtypes = [_i32p, _f64p, C.c_int, _f32p, C.c_int, C.c_int64, _f32p]
         L.sro_predict64.argtypes = [_i32p, _f64p, C.c_int, _f32p, C.c_int, C.c_int64, _f64p]
+        L.sro_predict64_bound.argtypes = [_i32p, _f64p, C.c_int, _f32p, C.c_int, C.c_int64, _f64p,
+                                          _f64p]
         L.sro_mse32.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, _f32p, C.c_int, C.c_int64,
                                 C.c_int, _f32p, _i64p]
         L.sro_mse64.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, _f32p, C.c_int, C.c_int64,
@@ -185,6 +187,16 @@ def predict64(tok, val, X, nv, N):
     lib().sro_predict64(np.ascontiguousarray(tok, np.int32), np.ascontiguousarray(val, np.float64),
                         len(tok), np.ascontiguousarray(X, np.float32), nv, N, out)
     return out
+
+
+def predict64_bound(tok, val, X, nv, N):
+    """FP64 values and first-order error bounds of an fp32 evaluation."""
+    out = np.empty(N, dtype=np.float64)
+    bnd = np.empty(N, dtype=np.float64)
+    lib().sro_predict64_bound(np.ascontiguousarray(tok, np.int32),
+                              np.ascontiguousarray(val, np.float64), len(tok),
+                              np.ascontiguousarray(X, np.float32), nv, N, out, bnd)
+    return out, bnd
 
 
 def mse32(pop: Population, X, nv, N, threads=1):

@@ -156,6 +156,160 @@ double sro_eval_point64(const int *tok, const double *val, int len, const double
     return stk[0];
 }
 
+/* ---- FP64 value + running error bound ------------------------------------ */
+
+#define U32 (1.0 / 16777216.0) /* fp32 unit roundoff 2^-24 */
+#define FLT_MAXD 3.4028234663852886e38
+#define TINY 1.5e-45
+
+/* rounding added by one fp32 operation producing y */
+static double round_err(double y, double ulps)
+{
+    return ulps * 2.0 * U32 * fabs(y) + TINY;
+}
+
+/* Clamp to fp32 range semantics; returns 1 if ambiguous (within bound). */
+static int to_fp32_range(double *y, double *b)
+{
+    if (isnan(*y) || isinf(*y)) {
+        *b = 0.0;
+        return 0;
+    }
+    double ay = fabs(*y);
+    if (ay - *b > FLT_MAXD) {
+        *y = *y > 0 ? INFINITY : -INFINITY;
+        *b = 0.0;
+        return 0;
+    }
+    if (ay + *b > FLT_MAXD) return 1;
+    return 0;
+}
+
+static void eval_bound_point(const int *tok, const double *val, int len, const double *x, int nv,
+                             double *out, double *bout)
+{
+    double sv[SRO_MAX_STACK] = {0}, sb[SRO_MAX_STACK] = {0};
+    int sp = 0, unstable = 0;
+    const double D = 1e-9;
+    for (int i = len - 1; i >= 0; i--) {
+        int t = tok[i];
+        if (t == SRO_TOK_CONST) {
+            sv[sp] = (double)(float)val[i];
+            sb[sp++] = 0.0;
+            continue;
+        }
+        if (t == SRO_TOK_VAR) {
+            int v = (int)val[i];
+            sv[sp] = v < nv ? x[v] : 0.0;
+            sb[sp++] = 0.0;
+            continue;
+        }
+        double a = sv[--sp], ea = sb[sp];
+        double b = 0.0, eb = 0.0;
+        if (sro_is_binary(t)) {
+            b = sv[--sp];
+            eb = sb[sp];
+        }
+        /* fp32 semantics in double: MIN/MAX as fmin/fmax (NaN-ignoring) */
+        double y = t == 6 ? fmin(a, b) : (t == 7 ? fmax(a, b) : sro_op64(t, a, b)), e = 0.0;
+        switch (t) {
+        case 1: case 2: e = ea + eb + round_err(y, 0.5); break;
+        case 3: e = fabs(b) * ea + fabs(a) * eb + ea * eb + round_err(y, 0.5); break;
+        case 4:
+            if (fabs(b) <= eb && !(b == 0.0 && eb == 0.0)) unstable = 1;
+            e = (b != 0.0) ? (ea + fabs(y) * eb) / fabs(b) + round_err(y, 0.5) : 0.0;
+            break;
+        case 5:
+        case 9: {
+            double aa = t == 9 ? fabs(a) : a;
+            /* sign / zero of the base uncertain, or a negative base with an
+             * inexact exponent (NaN vs finite can flip) */
+            if (fabs(a) <= ea && !(a == 0.0 && ea == 0.0)) unstable = 1;
+            if (t == 5 && a < 0.0 && eb > 0.0) unstable = 1;
+            double da = aa != 0.0 ? fabs(b * y / aa) : 0.0;
+            double db = aa > 0.0 ? fabs(y * log(aa)) : 0.0;
+            e = da * ea + db * eb + round_err(y, 2.0);
+            break;
+        }
+        case 6: case 7: e = ea > eb ? ea : eb; break;
+        case 8: {
+            if (fabs(fabs(b) - D) <= eb + 1e-300) unstable = 1;
+            double d = fabs(b) <= D ? D : fabs(b);
+            e = (ea + (fabs(b) <= D ? 0.0 : fabs(y) * eb)) / d + round_err(y, 0.5);
+            break;
+        }
+        case 10: case 11: e = ea + round_err(y, 2.0); break;
+        case 12: e = (1.0 + y * y) * ea + round_err(y, 2.0); break;
+        case 13: case 14: e = cosh(a) * ea + round_err(y, 2.0); break;
+        case 15: e = (1.0 - y * y) * ea + round_err(y, 2.0); break;
+        case 16: e = fabs(y) * ea + round_err(y, 2.0); break;
+        case 17:
+            if (fabs(a) <= ea && !(a == 0.0 && ea == 0.0)) unstable = 1;
+            e = ea / fabs(a) + round_err(y, 2.0);
+            break;
+        case 18: e = y * y * ea + round_err(y, 0.5); break;
+        case 19: case 20:
+            if (fabs(fabs(a) - 1.0) <= ea) unstable = 1;
+            e = ea / sqrt(fabs(1.0 - a * a)) + round_err(y, 2.0);
+            break;
+        case 21: e = ea / (1.0 + a * a) + round_err(y, 2.0); break;
+        case 22:
+            if (fabs(a) <= ea && !(a == 0.0 && ea == 0.0)) unstable = 1;
+            e = a != 0.0 ? ea / fabs(a) + round_err(y, 2.0) : 0.0;
+            break;
+        case 23:
+            if (fabs(fabs(a) - D) <= ea + 1e-300) unstable = 1;
+            e = (fabs(a) <= D ? 0.0 : y * y * ea) + round_err(y, 0.5);
+            break;
+        case 24: case 25: e = ea; break;
+        case 26:
+            if (fabs(a) <= ea && !(a == 0.0 && ea == 0.0)) unstable = 1;
+            e = (a > 0.0 ? ea / (2.0 * y) : 0.0) + round_err(y, 0.5);
+            break;
+        case 27:
+            if (fabs(a) <= ea && !(a == 0.0 && ea == 0.0)) unstable = 1;
+            e = (a != 0.0 ? ea / (2.0 * y) : 0.0) + round_err(y, 0.5);
+            break;
+        case SRO_OP_EXP_CLAMPED:
+            if (fabs(a - 10.0) <= ea) unstable = 1;
+            e = a > 10.0 ? 0.0 : fabs(y) * ea + round_err(y, 2.0);
+            break;
+        default: e = 0.0; break;
+        }
+        /* first-order propagation only holds while the operand's error is
+         * small against the scale on which the op's derivative changes */
+        double scale = INFINITY;
+        switch (t) {
+        case 4: case 8: case 5: case 9: scale = t >= 8 || t == 4 ? fabs(b) : fabs(a); break;
+        case 12: scale = fabs(cos(a)); break;
+        case 13: case 14: case 16: case SRO_OP_EXP_CLAMPED: scale = 1.0; break;
+        case 17: case 18: case 22: case 23: case 26: case 27: scale = fabs(a); break;
+        case 19: case 20: scale = 1.0 - fabs(a); break;
+        default: break;
+        }
+        if (t == 5 || t == 9) {
+            if (eb > 1e-2 * (1.0 + fabs(b))) unstable = 1;
+        }
+        if (isfinite(scale) && (t == 4 || t == 8 ? eb : ea) > 1e-2 * scale) unstable = 1;
+        if (isnan(e)) e = INFINITY;
+        if (to_fp32_range(&y, &e)) unstable = 1;
+        sv[sp] = y;
+        sb[sp++] = e;
+    }
+    *out = sv[0];
+    *bout = unstable ? INFINITY : sb[0];
+}
+
+void sro_predict64_bound(const int *tok, const double *val, int len, const float *X, int nv,
+                         int64_t N, double *out, double *bound_out)
+{
+    double x[64];
+    for (int64_t dp = 0; dp < N; dp++) {
+        for (int v = 0; v < nv && v < 64; v++) x[v] = (double)X[(int64_t)v * N + dp];
+        eval_bound_point(tok, val, len, x, nv, &out[dp], &bound_out[dp]);
+    }
+}
+
 int sro_stack_depth(const int *tok, int len)
 {
     int sp = 0, mx = 0;

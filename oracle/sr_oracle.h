@@ -76,6 +76,18 @@ void sro_mse32(const int *tok, const double *val, const int *off, const int *len
 void sro_mse64(const int *tok, const double *val, const int *off, const int *len, int E,
                const float *X, int nv, int64_t N, int threads, double *mse_out);
 
+/* FP64 evaluation with a first-order running error bound (forward error
+ * analysis of an fp32 evaluation): every fp32 operation adds its rounding
+ * (0.5 ulp for correctly rounded ops, 2 ulp for libm transcendentals) and
+ * propagates its operands' bounds through |df/da|, |df/db|.  bound_out is
+ * +inf where a protected branch (DIV/LOOSE_* guards, clamped exp, log/sqrt
+ * domain, asin/acos domain) could flip within the bound, or where an
+ * intermediate leaves the fp32 range ambiguously.  Values beyond the fp32
+ * range become +-inf as in fp32.  Not pinned by the reference: it is the
+ * yardstick for the parity tolerance; checked against fp32 evaluations. */
+void sro_predict64_bound(const int *tok, const double *val, int len, const float *X, int nv,
+                         int64_t N, double *out, double *bound_out);
+
 /* Maximum stack depth the right-to-left interpreter reaches; -1 if malformed
  * (underflow, or final depth != 1). */
 int sro_stack_depth(const int *tok, int len);

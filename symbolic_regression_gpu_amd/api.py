@@ -97,6 +97,7 @@ _SIGNATURES = {
     "sr_ctx_destroy": [_vp],
     "sr_last_error": [_vp],
     "sr_set_stream": [_vp, _vp],
+    "sr_use_own_stream": [_vp],
     "sr_set_data": [_vp, _f32p, C.c_int32, C.c_int64],
     "sr_set_data_device": [_vp, _vp, C.c_int32, C.c_int64],
     "sr_eval_mse": [_vp, C.POINTER(_Pop), C.c_int, _f32p, _i64p, C.POINTER(_Stats)],
@@ -261,7 +262,12 @@ class Context:
 
     # ---- data -----------------------------------------------------------
     def set_stream(self, stream_ptr: int | None):
+        """Run on a caller's hipStream_t handle (0 / None = the null stream,
+        torch's default stream)."""
         self._check(self._L.sr_set_stream(self._h, stream_ptr))
+
+    def use_own_stream(self):
+        self._check(self._L.sr_use_own_stream(self._h))
 
     def set_data(self, X, y=None):
         """X: [nv, N] (one row per variable) or [nv+1, N] with y last."""

@@ -370,7 +370,13 @@ int sr_ga_run(sr_ctx *x, const sr_select_params *spp, int mode, int32_t gens, ui
     if ((he = g.hist.ensure((size_t)std::max(gens, 1) * g.pop * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
     if ((he = g.hist_len.ensure((size_t)std::max(gens, 1) * g.pop * 4)) != hipSuccess)
         return c.hip_fail(he, "hipMalloc");
+    // The null stream cannot be captured: a graph run then goes on the
+    // context's own stream, after the caller's null-stream work has drained.
     hipStream_t s = c.stream;
+    if (use_graph && !s) {
+        if ((he = hipStreamSynchronize(nullptr)) != hipSuccess) return c.hip_fail(he, "sync");
+        s = c.own_stream;
+    }
     auto enqueue_gen = [&](int b) -> int {
         int r = ga_enqueue_eval(c, b, lp, s);
         if (r) return r;

@@ -286,7 +286,7 @@ int sr_ctx_destroy(sr_ctx *x)
     if (!x) return SR_OK;
     Ctx &c = x->c;
     hipSetDevice(c.device);
-    if (c.stream) hipStreamSynchronize(c.stream);
+    hipStreamSynchronize(c.stream);
     for (auto &e : c.ev)
         if (e) hipEventDestroy(e);
     if (c.own_stream) hipStreamDestroy(c.own_stream);
@@ -300,7 +300,15 @@ int sr_set_stream(sr_ctx *x, void *stream)
 {
     if (!x) return SR_ERR_INVALID_ARG;
     std::lock_guard<std::recursive_mutex> g(x->c.mu);
-    x->c.stream = stream ? (hipStream_t)stream : x->c.own_stream;
+    x->c.stream = (hipStream_t)stream;
+    return SR_OK;
+}
+
+int sr_use_own_stream(sr_ctx *x)
+{
+    if (!x) return SR_ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> g(x->c.mu);
+    x->c.stream = x->c.own_stream;
     return SR_OK;
 }
 

@@ -53,40 +53,69 @@ def ulp(x):
     return np.spacing(x).astype(np.float64)
 
 
-ILL_REL = 1e-3  # fp32 oracle off the FP64 evaluator by > ~8000 ulp
-
-
-def ill_conditioned(cpu32, cpu64, rel=ILL_REL):
-    """Points where the fp32 oracle itself is not a faithful answer: its class
-    (finite / non-finite) differs from the FP64 evaluator's, or its relative
-    error against FP64 exceeds rel.  There two correct fp32 evaluators with
-    different libm rounding may legitimately disagree."""
-    c32 = np.asarray(cpu32, np.float64)
-    c64 = np.asarray(cpu64, np.float64)
-    cls = np.isfinite(c32) != np.isfinite(c64)
-    both = np.isfinite(c32) & np.isfinite(c64)
-    big = np.zeros(c32.shape, bool)
+def point_parity(gpu, cpu32, v64, b64, k=2.0):
+    """Per-point fp32 parity against the FP64 running-error bound
+    (oracle sro_predict64_bound): a point whose bound is infinite (a protected
+    branch or the fp32 range can flip within rounding) is accepted; otherwise
+    the finite / NaN / Inf class must equal the CPU oracle's and
+        |gpu - v64| <= k * max(bound, |cpu32 - v64|) + 2 ulp(v64).
+    Returns a bool mask."""
+    g = np.asarray(gpu, np.float64)
+    c = np.asarray(cpu32, np.float64)
+    v = np.asarray(v64, np.float64)
+    b = np.asarray(b64, np.float64)
+    stable = np.isfinite(b)
+    same_nan = np.isnan(g) & np.isnan(c)
+    same_inf = np.isinf(g) & np.isinf(c) & (np.sign(g) == np.sign(c))
     with np.errstate(invalid="ignore", over="ignore"):
-        big[both] = rel_err(c32[both], c64[both]) > rel
-    return cls | big
+        err = np.where(np.isfinite(c) & np.isfinite(v), np.abs(c - v), 0.0)
+        tol = k * np.maximum(b, err) + 2.0 * ulp(np.where(np.isfinite(v), v, 0.0))
+        close = np.isfinite(g) & np.isfinite(c) & (np.abs(g - v) <= tol)
+    return ~stable | same_nan | same_inf | close
 
 
-def point_tolerance_ok(gpu, cpu32, cpu64, k=4.0):
-    """Per-point parity: both non-finite in the same way, or
-    |gpu - cpu32| <= k*ulp(cpu32) + 2*|cpu32 - cpu64| (error the fp32 oracle
-    itself carries against the FP64 evaluator), or the point is ill-conditioned
-    (see ill_conditioned).  Returns a bool mask."""
-    gpu = np.asarray(gpu, np.float64)
-    c32 = np.asarray(cpu32, np.float64)
-    c64 = np.asarray(cpu64, np.float64)
-    both_nan = np.isnan(gpu) & np.isnan(c32)
-    same_inf = np.isinf(gpu) & np.isinf(c32) & (np.sign(gpu) == np.sign(c32))
-    fin = np.isfinite(gpu) & np.isfinite(c32)
-    with np.errstate(invalid="ignore", over="ignore"):
-        cond = np.where(np.isfinite(c64), np.abs(c32 - c64), 0.0)
-        tol = k * ulp(c32) + 2.0 * cond
-        ok = fin & (np.abs(gpu - c32) <= tol)
-    return both_nan | same_inf | ok | ill_conditioned(c32, c64)
+def mse_error_bound(v64, b64, c32, y):
+    """Bound on |MSE_fp32 - MSE_exact| over the finite points, or inf when a
+    point is unstable.  Relative to the FP64 MSE."""
+    v = np.asarray(v64, np.float64)
+    b = np.asarray(b64, np.float64)
+    fin = np.isfinite(np.asarray(c32, np.float64))
+    if not np.isfinite(b[fin]).all() or fin.sum() == 0:
+        return np.inf
+    e = np.maximum(b, np.abs(np.asarray(c32, np.float64) - v))[fin]
+    d = np.abs(v[fin] - np.asarray(y, np.float64)[fin])
+    mse = np.mean(d * d)
+    bound = np.mean(2 * d * e + e * e) + 2 * EPS32 * mse
+    return bound / max(mse, 1e-300)
+
+
+def check_population(pop, X, nv, N, m_gpu, v_gpu, pred_gpu=None, rtol=1e-5):
+    """Conditioning-aware parity of a GPU evaluation (MSE, valid counts and
+    optionally predictions [E, N]) with the oracle.  Expressions whose relative
+    MSE error bound is <= 1e-6 must match the oracle's MSE within rtol and its
+    valid count exactly; every expression's MSE must lie within 2x its bound
+    of the FP64 MSE; predictions must pass point_parity.  Returns the number
+    of strictly checked expressions."""
+    m_ref, v_ref = O.mse32(pop, X, nv, N, threads=8)
+    y = X[nv]
+    strict = 0
+    for e in range(pop.E):
+        tok, val = pop.expr(e)
+        c32 = O.predict32(tok, val, X, nv, N)
+        v64, b64 = O.predict64_bound(tok, val, X, nv, N)
+        rb = mse_error_bound(v64, b64, c32, y)
+        if pred_gpu is not None:
+            ok = point_parity(pred_gpu[e], c32, v64, b64)
+            assert ok.all(), (e, np.flatnonzero(~ok)[:5])
+        if rb <= 1e-6:
+            strict += 1
+            assert v_gpu[e] == v_ref[e], (e, v_gpu[e], v_ref[e])
+            assert mse_close([m_gpu[e]], [m_ref[e]], rtol).all(), (e, m_gpu[e], m_ref[e])
+        if np.isfinite(rb) and np.isfinite(m_gpu[e]):
+            fin = np.isfinite(c32)
+            m64 = np.mean((v64[fin] - y[fin]) ** 2)
+            assert abs(m_gpu[e] - m64) <= 2 * rb * m64 + 4 * EPS32 * m64, (e, m_gpu[e], m64, rb)
+    return strict
 
 
 def rel_err(a, b):

@@ -1,11 +1,14 @@
 """GPU parity of the plain and batched interpreters against the CPU oracle
 (restatement of cl_cpu_eval.cpp:16-168, pinned to the compiled reference).
 
-Tolerances (stated per result):
-  * per-point predictions: |gpu - cpu32| <= 4 ulp(cpu32) + 2 |cpu32 - cpu64|
-    (the oracle's own error against the FP64 evaluator bounds the
-    conditioning of each point), non-finite classes equal;
-  * MSE: relative error <= 1e-5, NaN/Inf classes equal.
+Tolerances (stated per result, see tests/helpers.py):
+  * per point: |gpu - v64| <= 2 max(bound, |cpu32 - v64|) + 2 ulp, where v64 is
+    the FP64 value and bound the first-order error bound of an fp32
+    evaluation (oracle sro_predict64_bound), classes equal to the oracle's;
+    points whose bound is infinite (a protected branch may flip) are exempt;
+  * MSE: relative error <= 1e-5 against the oracle and equal valid counts
+    for every expression whose MSE error bound is <= 1e-6; every MSE within
+    twice its bound of the FP64 MSE.
 """
 import numpy as np
 import pytest
@@ -13,8 +16,9 @@ import pytest
 from oracle import oracle as O
 from symbolic_regression_gpu_amd import (EVAL_BATCHED, EVAL_PLAIN, Population, SrError)
 from symbolic_regression_gpu_amd.api import Tuning
-from tests.helpers import (ALL_BINARY, ALL_UNARY, GRAMMAR_BINARY, GRAMMAR_UNARY, ill_conditioned, mse_close,
-                           point_tolerance_ok, random_population, rel_err, to_api, uniform_data)
+from tests.helpers import (ALL_BINARY, ALL_UNARY, GRAMMAR_BINARY, GRAMMAR_UNARY, check_population,
+                           mse_close, point_parity, random_population, rel_err, to_api,
+                           uniform_data)
 
 pytestmark = pytest.mark.gpu
 
@@ -42,15 +46,13 @@ def test_single_expression_predictions_c1_shape(ctx):
     pred = ctx.eval_predictions(api)[0]
     tok, val = pop.expr(0)
     c32 = O.predict32(tok, val, X, 3, N)
-    c64 = O.predict64(tok, val, X, 3, N)
-    ok = point_tolerance_ok(pred, c32, c64)
+    c64, b64 = O.predict64_bound(tok, val, X, 3, N)
+    ok = point_parity(pred, c32, c64, b64)
     _report("C1 predictions", pred.astype(np.float64), c32.astype(np.float64), c64)
     assert ok.all(), np.flatnonzero(~ok)[:10]
-    m_ref, v_ref = O.mse32(pop, X, 3, N)
     for mode in (EVAL_PLAIN, EVAL_BATCHED):
         m, v, st = ctx.eval_mse(api, mode)
-        assert mse_close(m, m_ref).all(), (mode, m, m_ref)
-        assert (v == v_ref).all()
+        check_population(pop, X, 3, N, m, v)
         assert st["logical_node_evals"] == int(pop.len.sum()) * N
 
 
@@ -62,29 +64,12 @@ def test_population_mse_all_ops(ctx, nv):
     X = uniform_data(nv, N, nv)
     X[nv] = np.sin(X[0]) * 2 + 1
     ctx.set_data(X)
-    m_ref, v_ref = O.mse32(pop, X, nv, N, threads=8)
-    m64 = O.mse64(pop, X, nv, N, threads=8)
-    # Classify expressions: well-conditioned ones (the fp32 oracle is a
-    # faithful answer at every point) get the strict MSE bound; the others are
-    # checked point by point with the conditioning-aware rule.
-    c32 = [O.predict32(*pop.expr(e), X, nv, N) for e in range(pop.E)]
-    c64 = [O.predict64(*pop.expr(e), X, nv, N) for e in range(pop.E)]
-    ill = np.array([ill_conditioned(a, b).sum() for a, b in zip(c32, c64)])
-    well = ill == 0
-    print(f"nv={nv}: {int((~well).sum())} of {pop.E} expressions have ill-conditioned points")
     api = to_api(pop)
+    pred = ctx.eval_predictions(api)
     for mode in (EVAL_PLAIN, EVAL_BATCHED):
         m, v, _ = ctx.eval_mse(api, mode)
-        close = mse_close(m, m_ref)
-        bad = np.flatnonzero(well & ~(close & (v == v_ref)))
-        assert len(bad) == 0, (mode, bad, m[bad], m_ref[bad])
-        fin = well & np.isfinite(m) & np.isfinite(m64)
-        print(f"nv={nv} mode={mode}: max rel err vs FP64 MSE {np.max(rel_err(m[fin], m64[fin])):.3e}"
-              f" (oracle {np.max(rel_err(m_ref[fin], m64[fin])):.3e})")
-    pred = ctx.eval_predictions(api)
-    for e in range(pop.E):
-        ok = point_tolerance_ok(pred[e], c32[e], c64[e])
-        assert ok.all(), (e, np.flatnonzero(~ok)[:5])
+        strict = check_population(pop, X, nv, N, m, v, pred_gpu=pred if mode == EVAL_BATCHED else None)
+        print(f"nv={nv} mode={mode}: {strict} of {pop.E} expressions within the strict MSE bound")
 
 
 def test_batched_tunings_agree(ctx):
@@ -94,15 +79,18 @@ def test_batched_tunings_agree(ctx):
     X = uniform_data(3, N, 9)
     ctx.set_data(X)
     api = to_api(pop)
-    m_ref, _ = O.mse32(pop, X, 3, N, threads=8)
+    base, vbase, _ = ctx.eval_mse(api, EVAL_BATCHED)
+    check_population(pop, X, 3, N, base, vbase)
     saved = ctx.tuning
     try:
         for dpt in (1, 2, 4, 8):
             for nq in (1, 8):
                 for chunk in (512, 16384):
                     ctx.tuning = Tuning(dpt=dpt, chunk=chunk, n_queues=nq, blocks_per_cu=0)
-                    m, _, _ = ctx.eval_mse(api, EVAL_BATCHED)
-                    assert mse_close(m, m_ref).all(), (dpt, nq, chunk)
+                    m, v, _ = ctx.eval_mse(api, EVAL_BATCHED)
+                    # same per-point values; only the fp64 partial-sum order moves
+                    assert np.array_equal(v, vbase), (dpt, nq, chunk)
+                    assert mse_close(m, base, 1e-12).all(), (dpt, nq, chunk)
     finally:
         ctx.tuning = saved
 
@@ -113,17 +101,10 @@ def test_ragged_datapoint_counts(ctx, N):
     X = uniform_data(2, N, N)
     ctx.set_data(X)
     api = to_api(pop)
-    m_ref, v_ref = O.mse32(pop, X, 2, N)
+    pred = ctx.eval_predictions(api)
     for mode in (EVAL_PLAIN, EVAL_BATCHED):
         m, v, _ = ctx.eval_mse(api, mode)
-        assert mse_close(m, m_ref).all()
-        assert (v == v_ref).all()
-    pred = ctx.eval_predictions(api)
-    for e in range(pop.E):
-        tok, val = pop.expr(e)
-        c32 = O.predict32(tok, val, X, 2, N)
-        c64 = O.predict64(tok, val, X, 2, N)
-        assert point_tolerance_ok(pred[e], c32, c64).all()
+        check_population(pop, X, 2, N, m, v, pred_gpu=pred)
 
 
 def test_edge_expressions(ctx):
