@@ -161,6 +161,12 @@ int sr_sync(sr_ctx *ctx);
  * A test and inspection hook; no counterpart in the reference. */
 int sr_compile_expr(const int32_t *tokens, const float *values, int32_t len, int32_t nv,
                     int32_t *out_code, int32_t *out_arg, int32_t *stack_need);
+/* Same, as the parametric program the adjoint pass runs: const_slot [len] is
+ * -1 or the free-constant slot (0..7) of each token; SRC_POP instructions get
+ * the index of the instruction whose output they pop. */
+int sr_compile_param_expr(const int32_t *tokens, const float *values, const int32_t *const_slot,
+                          int32_t len, int32_t nv, int32_t *out_code, int32_t *out_arg,
+                          int32_t *stack_need);
 
 int sr_get_tuning(sr_ctx *ctx, sr_tuning *out);
 int sr_set_tuning(sr_ctx *ctx, const sr_tuning *t);

@@ -71,6 +71,9 @@ def lib():
         L.sro_predict64.argtypes = [_i32p, _f64p, C.c_int, _f32p, C.c_int, C.c_int64, _f64p]
         L.sro_predict64_bound.argtypes = [_i32p, _f64p, C.c_int, _f32p, C.c_int, C.c_int64, _f64p,
                                           _f64p]
+        L.sro_mse_grad64_bound.argtypes = [_i32p, _f64p, _i32p, C.c_int, _f32p, C.c_int, C.c_int64,
+                                           _f32p, C.c_int, C.POINTER(C.c_double), _f64p, _f64p,
+                                           C.POINTER(C.c_int64)]
         L.sro_mse32.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, _f32p, C.c_int, C.c_int64,
                                 C.c_int, _f32p, _i64p]
         L.sro_mse64.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, _f32p, C.c_int, C.c_int64,
@@ -197,6 +200,21 @@ def predict64_bound(tok, val, X, nv, N):
                               np.ascontiguousarray(val, np.float64), len(tok),
                               np.ascontiguousarray(X, np.float32), nv, N, out, bnd)
     return out, bnd
+
+
+def mse_grad64_bound(tok, val, slot, X, nv, N, cvals, lane_terms):
+    """FP64 MSE, dMSE/dconstants [8] and the error bound [8] of an fp32
+    adjoint evaluation (inf where a branch can flip), and the valid count."""
+    m = C.c_double()
+    cnt = C.c_int64()
+    g = np.empty(8, dtype=np.float64)
+    b = np.empty(8, dtype=np.float64)
+    lib().sro_mse_grad64_bound(np.ascontiguousarray(tok, np.int32), np.ascontiguousarray(val, np.float64),
+                               np.ascontiguousarray(slot, np.int32), len(tok),
+                               np.ascontiguousarray(X, np.float32), nv, N,
+                               np.ascontiguousarray(cvals, np.float32), lane_terms, C.byref(m), g, b,
+                               C.byref(cnt))
+    return m.value, g, b, cnt.value
 
 
 def mse32(pop: Population, X, nv, N, threads=1):

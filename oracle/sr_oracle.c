@@ -185,12 +185,121 @@ static int to_fp32_range(double *y, double *b)
     return 0;
 }
 
+/* One node of the bound evaluation: FP64 value *y of op t on (a, b) and the
+ * error bound *e of its fp32 evaluation from operand bounds ea, eb; sets
+ * *unstable when rounding can flip a branch of the op. */
+static void bound_step(int t, double a, double ea, double b, double eb, double *yo, double *eo,
+                       int *unst)
+{
+    const double D = 1e-9;
+    int unstable = 0;
+    /* fp32 semantics in double: MIN/MAX as fmin/fmax (NaN-ignoring) */
+    double y = t == 6 ? fmin(a, b) : (t == 7 ? fmax(a, b) : sro_op64(t, a, b)), e = 0.0;
+    switch (t) {
+    /* fp32 addition never underflows: a zero sum of exact operands is exact */
+    case 1: case 2: e = ea + eb + 0.5 * 2.0 * U32 * fabs(y); break;
+    case 3: e = fabs(b) * ea + fabs(a) * eb + ea * eb + round_err(y, 0.5); break;
+    case 4:
+        if (fabs(b) <= eb && !(b == 0.0 && eb == 0.0)) unstable = 1;
+        e = (b != 0.0) ? (ea + fabs(y) * eb) / fabs(b) + round_err(y, 0.5) : 0.0;
+        break;
+    case 5:
+    case 9: {
+        double aa = t == 9 ? fabs(a) : a;
+        /* sign / zero of the base uncertain, or a negative base with an
+         * inexact exponent (NaN vs finite can flip) */
+        if (fabs(a) <= ea && !(a == 0.0 && ea == 0.0)) unstable = 1;
+        if (t == 5 && a < 0.0 && eb > 0.0) unstable = 1;
+        /* an exponent that may round to zero turns any base into 1, and
+         * so does a base that may round to 1 under a NaN/inf exponent */
+        if (fabs(b) <= eb && !(b == 0.0 && eb == 0.0)) unstable = 1;
+        if (!isfinite(b) && ea > 0.0 && fabs(fabs(a) - 1.0) <= ea) unstable = 1;
+        double da = aa != 0.0 ? fabs(b * y / aa) : 0.0;
+        double db = aa > 0.0 ? fabs(y * log(aa)) : 0.0;
+        e = da * ea + db * eb + round_err(y, 2.0);
+        break;
+    }
+    case 6: case 7: e = ea > eb ? ea : eb; break;
+    case 8: {
+        if (fabs(fabs(b) - D) <= eb + 1e-300) unstable = 1;
+        double d = fabs(b) <= D ? D : fabs(b);
+        e = (ea + (fabs(b) <= D ? 0.0 : fabs(y) * eb)) / d + round_err(y, 0.5);
+        break;
+    }
+    case 10: case 11: e = ea + round_err(y, 2.0); break;
+    case 12: e = (1.0 + y * y) * ea + round_err(y, 2.0); break;
+    case 13: case 14: e = cosh(a) * ea + round_err(y, 2.0); break;
+    case 15: e = (1.0 - y * y) * ea + round_err(y, 2.0); break;
+    case 16: e = fabs(y) * ea + round_err(y, 2.0); break;
+    case 17:
+        if (fabs(a) <= ea && !(a == 0.0 && ea == 0.0)) unstable = 1;
+        e = ea / fabs(a) + round_err(y, 2.0);
+        break;
+    case 18: e = y * y * ea + round_err(y, 0.5); break;
+    case 19: case 20:
+        if (ea > 0.0 && fabs(fabs(a) - 1.0) <= ea) unstable = 1;
+        e = ea / sqrt(fabs(1.0 - a * a)) + round_err(y, 2.0);
+        break;
+    case 21: e = ea / (1.0 + a * a) + round_err(y, 2.0); break;
+    case 22:
+        if (fabs(a) <= ea && !(a == 0.0 && ea == 0.0)) unstable = 1;
+        e = a != 0.0 ? ea / fabs(a) + round_err(y, 2.0) : 0.0;
+        break;
+    case 23:
+        if (fabs(fabs(a) - D) <= ea + 1e-300) unstable = 1;
+        e = (fabs(a) <= D ? 0.0 : y * y * ea) + round_err(y, 0.5);
+        break;
+    case 24: case 25: e = ea; break;
+    case 26:
+        if (fabs(a) <= ea && !(a == 0.0 && ea == 0.0)) unstable = 1;
+        e = (a > 0.0 ? ea / (2.0 * y) : 0.0) + round_err(y, 0.5);
+        break;
+    case 27:
+        if (fabs(a) <= ea && !(a == 0.0 && ea == 0.0)) unstable = 1;
+        e = (a != 0.0 ? ea / (2.0 * y) : 0.0) + round_err(y, 0.5);
+        break;
+    case SRO_OP_EXP_CLAMPED:
+        if (ea > 0.0 && fabs(a - 10.0) <= ea) unstable = 1;
+        e = a > 10.0 ? 0.0 : fabs(y) * ea + round_err(y, 2.0);
+        break;
+    default: e = 0.0; break;
+    }
+    /* correctly rounded fp32 ops on exact operands whose exact result is
+     * an fp32 number return it exactly */
+    if (ea == 0.0 && eb == 0.0 && isfinite(y) && (double)(float)y == y &&
+        (t == 1 || t == 2 || t == 3 || t == 4 || t == 6 || t == 7 || t == 18 || t == 24 ||
+         t == 25 || t == 26 || t == 27 || ((t == 8 || t == 23) && fabs(t == 8 ? b : a) > D)))
+        e = 0.0;
+    /* first-order propagation only holds while the operand's error is
+     * small against the scale on which the op's derivative changes */
+    double scale = INFINITY;
+    switch (t) {
+    case 4: case 8: case 5: case 9: scale = t >= 8 || t == 4 ? fabs(b) : fabs(a); break;
+    case 12: scale = fabs(cos(a)); break;
+    case 13: case 14: case 16: case SRO_OP_EXP_CLAMPED: scale = 1.0; break;
+    case 17: case 18: case 22: case 23: case 26: case 27: scale = fabs(a); break;
+    case 19: case 20: scale = 1.0 - fabs(a); break;
+    default: break;
+    }
+    if (t == 5 || t == 9) {
+        if (eb > 1e-2 * (1.0 + fabs(b))) unstable = 1;
+    }
+    if (isfinite(scale) && (t == 4 || t == 8 ? eb : ea) > 1e-2 * scale) unstable = 1;
+    /* an infinite FP64 value from inexact operands (pow(1 - d, huge),
+     * exp of a huge argument) need not be infinite in fp32 */
+    if (isinf(y) && (ea > 0.0 || eb > 0.0)) unstable = 1;
+    if (isnan(e)) e = INFINITY;
+    if (to_fp32_range(&y, &e)) unstable = 1;
+    *yo = y;
+    *eo = e;
+    if (unstable) *unst = 1;
+}
+
 static void eval_bound_point(const int *tok, const double *val, int len, const double *x, int nv,
                              double *out, double *bout)
 {
     double sv[SRO_MAX_STACK] = {0}, sb[SRO_MAX_STACK] = {0};
     int sp = 0, unstable = 0;
-    const double D = 1e-9;
     for (int i = len - 1; i >= 0; i--) {
         int t = tok[i];
         if (t == SRO_TOK_CONST) {
@@ -210,94 +319,215 @@ static void eval_bound_point(const int *tok, const double *val, int len, const d
             b = sv[--sp];
             eb = sb[sp];
         }
-        /* fp32 semantics in double: MIN/MAX as fmin/fmax (NaN-ignoring) */
-        double y = t == 6 ? fmin(a, b) : (t == 7 ? fmax(a, b) : sro_op64(t, a, b)), e = 0.0;
-        switch (t) {
-        case 1: case 2: e = ea + eb + round_err(y, 0.5); break;
-        case 3: e = fabs(b) * ea + fabs(a) * eb + ea * eb + round_err(y, 0.5); break;
-        case 4:
-            if (fabs(b) <= eb && !(b == 0.0 && eb == 0.0)) unstable = 1;
-            e = (b != 0.0) ? (ea + fabs(y) * eb) / fabs(b) + round_err(y, 0.5) : 0.0;
-            break;
-        case 5:
-        case 9: {
-            double aa = t == 9 ? fabs(a) : a;
-            /* sign / zero of the base uncertain, or a negative base with an
-             * inexact exponent (NaN vs finite can flip) */
-            if (fabs(a) <= ea && !(a == 0.0 && ea == 0.0)) unstable = 1;
-            if (t == 5 && a < 0.0 && eb > 0.0) unstable = 1;
-            double da = aa != 0.0 ? fabs(b * y / aa) : 0.0;
-            double db = aa > 0.0 ? fabs(y * log(aa)) : 0.0;
-            e = da * ea + db * eb + round_err(y, 2.0);
-            break;
-        }
-        case 6: case 7: e = ea > eb ? ea : eb; break;
-        case 8: {
-            if (fabs(fabs(b) - D) <= eb + 1e-300) unstable = 1;
-            double d = fabs(b) <= D ? D : fabs(b);
-            e = (ea + (fabs(b) <= D ? 0.0 : fabs(y) * eb)) / d + round_err(y, 0.5);
-            break;
-        }
-        case 10: case 11: e = ea + round_err(y, 2.0); break;
-        case 12: e = (1.0 + y * y) * ea + round_err(y, 2.0); break;
-        case 13: case 14: e = cosh(a) * ea + round_err(y, 2.0); break;
-        case 15: e = (1.0 - y * y) * ea + round_err(y, 2.0); break;
-        case 16: e = fabs(y) * ea + round_err(y, 2.0); break;
-        case 17:
-            if (fabs(a) <= ea && !(a == 0.0 && ea == 0.0)) unstable = 1;
-            e = ea / fabs(a) + round_err(y, 2.0);
-            break;
-        case 18: e = y * y * ea + round_err(y, 0.5); break;
-        case 19: case 20:
-            if (fabs(fabs(a) - 1.0) <= ea) unstable = 1;
-            e = ea / sqrt(fabs(1.0 - a * a)) + round_err(y, 2.0);
-            break;
-        case 21: e = ea / (1.0 + a * a) + round_err(y, 2.0); break;
-        case 22:
-            if (fabs(a) <= ea && !(a == 0.0 && ea == 0.0)) unstable = 1;
-            e = a != 0.0 ? ea / fabs(a) + round_err(y, 2.0) : 0.0;
-            break;
-        case 23:
-            if (fabs(fabs(a) - D) <= ea + 1e-300) unstable = 1;
-            e = (fabs(a) <= D ? 0.0 : y * y * ea) + round_err(y, 0.5);
-            break;
-        case 24: case 25: e = ea; break;
-        case 26:
-            if (fabs(a) <= ea && !(a == 0.0 && ea == 0.0)) unstable = 1;
-            e = (a > 0.0 ? ea / (2.0 * y) : 0.0) + round_err(y, 0.5);
-            break;
-        case 27:
-            if (fabs(a) <= ea && !(a == 0.0 && ea == 0.0)) unstable = 1;
-            e = (a != 0.0 ? ea / (2.0 * y) : 0.0) + round_err(y, 0.5);
-            break;
-        case SRO_OP_EXP_CLAMPED:
-            if (fabs(a - 10.0) <= ea) unstable = 1;
-            e = a > 10.0 ? 0.0 : fabs(y) * ea + round_err(y, 2.0);
-            break;
-        default: e = 0.0; break;
-        }
-        /* first-order propagation only holds while the operand's error is
-         * small against the scale on which the op's derivative changes */
-        double scale = INFINITY;
-        switch (t) {
-        case 4: case 8: case 5: case 9: scale = t >= 8 || t == 4 ? fabs(b) : fabs(a); break;
-        case 12: scale = fabs(cos(a)); break;
-        case 13: case 14: case 16: case SRO_OP_EXP_CLAMPED: scale = 1.0; break;
-        case 17: case 18: case 22: case 23: case 26: case 27: scale = fabs(a); break;
-        case 19: case 20: scale = 1.0 - fabs(a); break;
-        default: break;
-        }
-        if (t == 5 || t == 9) {
-            if (eb > 1e-2 * (1.0 + fabs(b))) unstable = 1;
-        }
-        if (isfinite(scale) && (t == 4 || t == 8 ? eb : ea) > 1e-2 * scale) unstable = 1;
-        if (isnan(e)) e = INFINITY;
-        if (to_fp32_range(&y, &e)) unstable = 1;
+        double y, e;
+        bound_step(t, a, ea, b, eb, &y, &e, &unstable);
         sv[sp] = y;
         sb[sp++] = e;
     }
     *out = sv[0];
     *bout = unstable ? INFINITY : sb[0];
+}
+
+/* ---- FP64 gradient of the MSE + error bound of an fp32 adjoint ------------ */
+
+#define RND(p, ulps) ((ulps) * 2.0 * U32 * fabs(p) + TINY)
+
+/* Partials pa = dy/da, pb = dy/db of op t at (a, b) (y its value) as the fp32
+ * adjoint pass defines them, and bounds epa, epb on their fp32 evaluation from
+ * the operand bounds (second derivative x operand error + rounding).  Returns
+ * 1 when rounding can flip which branch of a partial applies. */
+static int partials(int t, double a, double ea, double b, double eb, double y, double *pa, double *epa,
+                    double *pb, double *epb)
+{
+    const double D = 1e-9;
+    double p = 0.0, q = 0.0, ep = 0.0, eq = 0.0;
+    int unst = 0;
+    switch (t) {
+    case 1: p = 1.0; q = 1.0; break;
+    case 2: p = 1.0; q = -1.0; break;
+    case 3: p = b; ep = eb; q = a; eq = ea; break;
+    case 4:
+    case 8:
+        if (t == 8 && fabs(b) <= D) {
+            p = b < 0.0 ? -1.0 / D : 1.0 / D;
+            q = 0.0;
+            break;
+        }
+        p = 1.0 / b;
+        ep = eb / (b * b) + RND(p, 0.5);
+        q = -a / (b * b);
+        eq = ea / (b * b) + 2.0 * fabs(a) * eb / fabs(b * b * b) + RND(q, 1.5);
+        break;
+    case 5:
+    case 9: {
+        double aa = t == 9 ? fabs(a) : a, sg = (t == 9 && a < 0.0) ? -1.0 : 1.0;
+        if (a == 0.0) {
+            /* the kernel's convention at a zero base */
+            p = (t == 5 && b == 1.0) ? 1.0 : 0.0;
+            q = 0.0;
+            break;
+        }
+        p = b * y / a;
+        double lg = aa > 0.0 ? log(aa) : 0.0;
+        double dpa_a = b * (b - 1.0) * y / (a * a), dpa_b = (y / a) * (1.0 + b * lg);
+        ep = fabs(dpa_a) * ea + fabs(dpa_b) * eb + RND(p, 3.0);
+        if (aa > 0.0) {
+            q = y * lg;
+            double dq_a = sg * (y * b * lg / aa + y / aa), dq_b = y * lg * lg;
+            eq = fabs(dq_a) * ea + fabs(dq_b) * eb + RND(q, 3.0);
+        }
+        break;
+    }
+    case 6:
+    case 7: {
+        int sel_a = isnan(b) || (!isnan(a) && (t == 6 ? a <= b : a >= b));
+        if (fabs(a - b) <= ea + eb && (ea + eb) > 0.0) unst = 1;
+        p = sel_a ? 1.0 : 0.0;
+        q = sel_a ? 0.0 : 1.0;
+        break;
+    }
+    case 10: p = cos(a); ep = fmin(ea, 2.0) + RND(p, 2.0); break;
+    case 11: p = -sin(a); ep = fmin(ea, 2.0) + RND(p, 2.0); break;
+    case 12: p = 1.0 + y * y; ep = fabs(2.0 * y * p) * ea + RND(p, 3.0); break;
+    case 13: p = cosh(a); ep = fabs(sinh(a)) * ea + RND(p, 2.0); break;
+    case 14: p = sinh(a); ep = cosh(a) * ea + RND(p, 2.0); break;
+    case 15: p = 1.0 - y * y; ep = fabs(2.0 * y * p) * ea + RND(p, 3.0) + 4.0 * U32 * y * y; break;
+    case 16: p = y; ep = fabs(y) * ea + RND(p, 2.0); break;
+    case 17:
+    case 22:
+        if (a == 0.0) break;
+        p = 1.0 / a;
+        ep = ea / (a * a) + RND(p, 0.5);
+        break;
+    case 18: p = -y * y; ep = 2.0 * fabs(y * y * y) * ea + RND(p, 1.5); break;
+    case 23:
+        if (fabs(a) <= D) break;
+        p = -y * y;
+        ep = 2.0 * fabs(y * y * y) * ea + RND(p, 1.5);
+        break;
+    case 19:
+    case 20: {
+        double w = 1.0 - a * a;
+        p = (t == 19 ? 1.0 : -1.0) / sqrt(w);
+        ep = fabs(a) / (w * sqrt(w)) * ea + RND(p, 3.0);
+        break;
+    }
+    case 21: {
+        double w = 1.0 + a * a;
+        p = 1.0 / w;
+        ep = 2.0 * fabs(a) / (w * w) * ea + RND(p, 2.0);
+        break;
+    }
+    case 24:
+        if (fabs(a) <= ea && ea > 0.0) unst = 1;
+        p = a > 0.0 ? 1.0 : (a < 0.0 ? -1.0 : 0.0);
+        break;
+    case 25: p = -1.0; break;
+    case 26:
+        if (a <= 0.0) break;
+        p = 0.5 / y;
+        ep = 0.25 / (a * y) * ea + RND(p, 1.5);
+        break;
+    case 27:
+        if (a == 0.0) break;
+        p = (a > 0.0 ? 0.5 : -0.5) / y;
+        ep = 0.25 / (fabs(a) * y) * ea + RND(p, 1.5);
+        break;
+    case SRO_OP_EXP_CLAMPED:
+        if (a > 10.0) break;
+        p = y;
+        ep = fabs(y) * ea + RND(p, 2.0);
+        break;
+    default: break;
+    }
+    *pa = p;
+    *epa = ep;
+    *pb = q;
+    *epb = eq;
+    return unst;
+}
+
+void sro_mse_grad64_bound(const int *tok, const double *val, const int *slot, int len, const float *X,
+                          int nv, int64_t N, const float *cvals, int lane_terms, double *mse_out,
+                          double *grad_out, double *bound_out, int64_t *count_out)
+{
+    enum { K = 8 };
+    double *v32 = (double *)malloc(sizeof(double) * (size_t)len);
+    for (int i = 0; i < len; i++)
+        v32[i] = (slot[i] >= 0 && slot[i] < K) ? (double)cvals[slot[i]] : val[i];
+    double sv[SRO_MAX_STACK], sb[SRO_MAX_STACK], sd[SRO_MAX_STACK][K], se[SRO_MAX_STACK][K];
+    double gsum[K] = {0}, gerr[K] = {0}, gabs[K] = {0}, msum = 0.0;
+    int64_t cnt = 0;
+    int unstable_any = 0;
+    float xf[64];
+    double xd[64];
+    for (int64_t dp = 0; dp < N; dp++) {
+        for (int v = 0; v < nv && v < 64; v++) {
+            xf[v] = X[(int64_t)v * N + dp];
+            xd[v] = (double)xf[v];
+        }
+        float p32 = sro_eval_point32(tok, v32, len, xf, nv);
+        if (!isfinite(p32)) continue;
+        int sp = 0, unst = 0;
+        for (int i = len - 1; i >= 0; i--) {
+            int t = tok[i];
+            if (t == SRO_TOK_CONST || t == SRO_TOK_VAR) {
+                int vv = (int)v32[i];
+                sv[sp] = t == SRO_TOK_CONST ? (double)(float)v32[i] : (vv < nv ? xd[vv] : 0.0);
+                sb[sp] = 0.0;
+                for (int k = 0; k < K; k++) {
+                    sd[sp][k] = (t == SRO_TOK_CONST && slot[i] == k) ? 1.0 : 0.0;
+                    se[sp][k] = 0.0;
+                }
+                sp++;
+                continue;
+            }
+            int ia = --sp, ib = -1;
+            double a = sv[ia], ea = sb[ia], b = 0.0, eb = 0.0;
+            if (sro_is_binary(t)) {
+                ib = --sp;
+                b = sv[ib];
+                eb = sb[ib];
+            }
+            double y, e, pa, epa, pb, epb;
+            bound_step(t, a, ea, b, eb, &y, &e, &unst);
+            unst |= partials(t, a, ea, b, eb, y, &pa, &epa, &pb, &epb);
+            double nd[K], ne[K];
+            for (int k = 0; k < K; k++) {
+                double ta = pa * sd[ia][k], tb = ib >= 0 ? pb * sd[ib][k] : 0.0;
+                nd[k] = ta + tb;
+                ne[k] = fabs(pa) * se[ia][k] + epa * fabs(sd[ia][k]) + (ta != 0.0 ? RND(ta, 0.5) : 0.0);
+                if (ib >= 0)
+                    ne[k] += fabs(pb) * se[ib][k] + epb * fabs(sd[ib][k]) + (tb != 0.0 ? RND(tb, 0.5) : 0.0) +
+                             (ta != 0.0 && tb != 0.0 ? RND(nd[k], 0.5) : 0.0);
+                if (!isfinite(ne[k])) unst = 1;
+            }
+            sv[sp] = y;
+            sb[sp] = e;
+            memcpy(sd[sp], nd, sizeof nd);
+            memcpy(se[sp], ne, sizeof ne);
+            sp++;
+        }
+        if (unst || !isfinite(sv[0]) || !isfinite(sb[0])) unstable_any = 1;
+        double tgt = (double)X[(int64_t)nv * N + dp];
+        double diff = sv[0] - tgt, ediff = sb[0] + RND(diff, 0.5);
+        msum += diff * diff;
+        cnt++;
+        for (int k = 0; k < K; k++) {
+            double c = diff * sd[0][k];
+            gsum[k] += c;
+            gabs[k] += fabs(c);
+            gerr[k] += fabs(diff) * se[0][k] + ediff * fabs(sd[0][k]) + (c != 0.0 ? RND(c, 0.5) : 0.0);
+        }
+    }
+    free(v32);
+    *count_out = cnt;
+    *mse_out = cnt ? msum / (double)cnt : NAN;
+    for (int k = 0; k < K; k++) {
+        grad_out[k] = cnt ? 2.0 * gsum[k] / (double)cnt : NAN;
+        double sum_err = (double)(lane_terms > 1 ? lane_terms : 1) * 1.01 * U32 * gabs[k];
+        bound_out[k] = (!cnt || unstable_any) ? INFINITY : 2.0 * (gerr[k] + sum_err) / (double)cnt;
+    }
 }
 
 void sro_predict64_bound(const int *tok, const double *val, int len, const float *X, int nv,

@@ -85,6 +85,14 @@ void sro_mse64(const int *tok, const double *val, const int *off, const int *len
  * intermediate leaves the fp32 range ambiguously.  Values beyond the fp32
  * range become +-inf as in fp32.  Not pinned by the reference: it is the
  * yardstick for the parity tolerance; checked against fp32 evaluations. */
+/* FP64 MSE and dMSE/dconstant over the points where the fp32 oracle is finite,
+ * with constants of slot k (slot[i] = -1 or 0..7) set to cvals[k], and a bound
+ * on the error of an fp32 adjoint evaluation of each gradient component
+ * (operand rounding carried through the partials, fp32 summation of
+ * lane_terms points per lane).  bound = inf where rounding can flip a branch. */
+void sro_mse_grad64_bound(const int *tok, const double *val, const int *slot, int len, const float *X,
+                          int nv, int64_t N, const float *cvals, int lane_terms, double *mse_out,
+                          double *grad_out, double *bound_out, int64_t *count_out);
 void sro_predict64_bound(const int *tok, const double *val, int len, const float *X, int nv,
                          int64_t N, double *out, double *bound_out);
 

@@ -105,6 +105,8 @@ _SIGNATURES = {
     "sr_get_tuning": [_vp, C.POINTER(_Tuning)],
     "sr_set_tuning": [_vp, C.POINTER(_Tuning)],
     "sr_compile_expr": [_i32p, _f32p, C.c_int32, C.c_int32, _i32p, _i32p, C.POINTER(C.c_int32)],
+    "sr_compile_param_expr": [_i32p, _f32p, _i32p, C.c_int32, C.c_int32, _i32p, _i32p,
+                              C.POINTER(C.c_int32)],
     "sr_cache_config": [_vp, C.c_int32, C.c_int32, C.c_int32],
     "sr_cache_clear": [_vp],
     "sr_cache_state": [_vp, _vp, C.c_int32, C.POINTER(C.c_int32)],
@@ -150,14 +152,19 @@ def lib():
     return L
 
 
-def compile_expr(tokens, values, nv):
-    """Bytecode of one expression: (code int32[n], arg int32[n], stack_need)."""
+def compile_expr(tokens, values, nv, const_slot=None):
+    """Bytecode of one expression: (code int32[n], arg int32[n], stack_need);
+    with const_slot (per token -1 or 0..7), the parametric program."""
     t = np.ascontiguousarray(tokens, np.int32)
     v = np.ascontiguousarray(values, np.float32)
     code = np.empty(max(len(t), 1), np.int32)
     arg = np.empty(max(len(t), 1), np.int32)
     need = C.c_int32()
-    n = lib().sr_compile_expr(t, v, len(t), nv, code, arg, C.byref(need))
+    if const_slot is None:
+        n = lib().sr_compile_expr(t, v, len(t), nv, code, arg, C.byref(need))
+    else:
+        sl = np.ascontiguousarray(const_slot, np.int32)
+        n = lib().sr_compile_param_expr(t, v, sl, len(t), nv, code, arg, C.byref(need))
     if n < 0:
         raise SrError(n, "compile failed")
     return code[:n], arg[:n], need.value

@@ -469,6 +469,28 @@ int sr_compile_expr(const int32_t *tokens, const float *values, int32_t len, int
     return n;
 }
 
+int sr_compile_param_expr(const int32_t *tokens, const float *values, const int32_t *const_slot,
+                          int32_t len, int32_t nv, int32_t *out_code, int32_t *out_arg,
+                          int32_t *stack_need)
+{
+    if (!tokens || !values || !const_slot || !out_code || !out_arg || !stack_need) return SR_ERR_INVALID_ARG;
+    if (len <= 0 || len > MAX_EXPR_LEN || raw_stack_depth(tokens, len) < 0) return SR_ERR_BAD_EXPR;
+    std::vector<int> size(len), need(len), leaf(len), pstack(len);
+    std::vector<CompileFrame> frames(len + 1);
+    std::vector<Ins> ins(len);
+    int sn = 0;
+    int n = compile_expr(tokens, values, len, nv, size.data(), need.data(), frames.data(), ins.data(), &sn,
+                         leaf.data());
+    if (n < 0) return SR_ERR_BAD_EXPR;
+    if (param_postpass(ins.data(), n, leaf.data(), const_slot, pstack.data()) != 0) return SR_ERR_INVALID_ARG;
+    for (int i = 0; i < n; ++i) {
+        out_code[i] = ins[i].code;
+        out_arg[i] = ins[i].arg;
+    }
+    *stack_need = sn;
+    return n;
+}
+
 int sr_get_tuning(sr_ctx *x, sr_tuning *out)
 {
     if (!x || !out) return SR_ERR_INVALID_ARG;

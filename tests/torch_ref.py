@@ -87,4 +87,27 @@ def mse_and_grad(tok, val, X32, y32, cvals8, const_slot, valid_mask):
     loss = ((pred[m] - y64[m]) ** 2).mean()
     loss.backward()
     g = c.grad.numpy() if c.grad is not None else np.zeros(8)
-    return float(loss), np.nan_to_num(g)
+    return float(loss.detach()), np.nan_to_num(g)
+
+
+def fd_grad(tok, val, X32, y32, cvals8, const_slot, valid_mask, h):
+    """FP64 central differences (m(c_hi) - m(c_lo)) / (c_hi - c_lo) over the
+    same valid points, for the 8 slots, where c_hi/c_lo are c_k +- h rounded
+    to fp32 (the constants the fp32 replicas hold)."""
+    X64 = torch.tensor(X32, dtype=torch.float64)
+    y64 = torch.tensor(y32, dtype=torch.float64)
+    m = torch.tensor(valid_mask)
+    base = np.asarray(cvals8, np.float64)
+    g = np.zeros(8)
+    used = set(int(s) for s in const_slot if s >= 0)
+    with torch.no_grad():
+        for k in sorted(used):
+            out, at = [], []
+            for sgn in (1.0, -1.0):
+                c = base.copy()
+                c[k] = np.float64(np.float32(np.float32(c[k]) + np.float32(sgn * h)))
+                pred = eval_expr(tok, val, X64, torch.tensor(c), const_slot)
+                out.append(float(((pred[m] - y64[m]) ** 2).mean()))
+                at.append(c[k])
+            g[k] = (out[0] - out[1]) / (at[0] - at[1])
+    return g
