@@ -1,0 +1,363 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X evaluator on the evolution-loop workload:
+population 1000, 1M datapoints, 3 variables ~ U[1,5], max depth 8, <= 255
+tokens, tournament 7, crossover 0.9 / mutation 0.1, grammar {+,-,*,protected /}
+x {sin, cos, clamped exp, protected log, sqrt|.|}, target a seeded depth-5
+tree (the reference's evaluate -> fitness -> select -> vary step,
+cl_sr_system.cpp:168-333).
+
+One step = one generation on the device: batched persistent evaluation with
+fused MSE, rank, tournament / elitism and mutate / crossover, replayed as one
+hipGraph.  `value` = logical node-evals/s of those generations (sum_e len_e * N
+per generation / wall time); `ms_per_step` = ms per generation.  Beside it, on
+the same initial population and data: the plain one-launch-per-expression
+interpreter, one batched evaluation, the subtree-cached evaluation, and the
+constant-fitting workload (4096 expressions of 50-200 tokens, 2-8 free
+constants, 5 variables, 256k datapoints): adjoint vs central-difference
+gradient evaluations/s.
+
+roofline: the batched interpreter kernel (HIP events around its launch on the
+context stream), counted in FP32 operator-node evaluations (1 flop per
+operator node per datapoint, leaves excluded) against the FP32 vector peak;
+it issues no MFMA.  The HBM side (X|y bytes per launch) is reported too.
+cpu_baseline: the oracle's restatement of evaluate_cpu_mse
+(cl_cpu_eval.cpp:94-168) on a bounded sample of the same population, on this
+host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 2]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+GRAMMAR_UNARY = [10, 11, 29, 22, 27]
+GRAMMAR_BINARY = [1, 2, 3, 8]
+METRIC = "node-evals/s + ms/generation (1000 exprs × 1M dps), plain vs batched+adjoint"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "traffic_k_batched_evolution_1000x1M.json")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def operator_nodes(pop):
+    return int(np.count_nonzero(pop.tokens > 0))
+
+
+def to_population(tok, val, ln, cap):
+    from symbolic_regression_gpu_amd import Population
+    return Population.from_strided(tok, val, ln, cap)
+
+
+def make_target(ctx, P, nv, N, depth, seed, min_len):
+    """A seeded random tree of the grammar (device generator) and its values."""
+    from symbolic_regression_gpu_amd.api import GaParams
+    p = GaParams.make(num_vars=nv, max_tokens=255, max_depth=depth, const_min=-2.0, const_max=2.0,
+                      unary_ops=GRAMMAR_UNARY, binary_ops=GRAMMAR_BINARY)
+    for s in range(seed, seed + 64):
+        ctx.ga_init(p, 64, s)
+        tok, val, ln, cap = ctx.ga_population()
+        for i in range(64):
+            if ln[i] >= min_len:
+                t = tok[i * cap:i * cap + ln[i]]
+                v = val[i * cap:i * cap + ln[i]].astype(np.float64)
+                return P.from_exprs([(t, v)])
+    raise RuntimeError("no target tree of the requested length")
+
+
+def evolution_setup(ctx, N, seed):
+    from symbolic_regression_gpu_amd.api import GaParams
+    nv = 3
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(1.0, 5.0, size=(nv + 1, N)).astype(np.float32)
+    ctx.set_data(X)
+    from symbolic_regression_gpu_amd import Population
+    target = make_target(ctx, Population, nv, N, 5, seed + 1, 15)
+    X[nv] = ctx.eval_predictions(target)[0]
+    X[nv] = np.nan_to_num(X[nv], nan=0.0, posinf=0.0, neginf=0.0)
+    ctx.set_data(X)
+    params = GaParams.make(num_vars=nv, max_tokens=255, max_depth=8, const_min=-2.0, const_max=2.0,
+                           unary_ops=GRAMMAR_UNARY, binary_ops=GRAMMAR_BINARY)
+    ctx.ga_init(params, 1000, seed + 2)
+    tok, val, ln, cap = ctx.ga_population()
+    return X, nv, (tok, val, ln, cap)
+
+
+def time_evals(ctx, pop, mode, reps):
+    """Mean logical node-evals/s and interpreter-kernel ms over reps calls
+    (after one untimed call)."""
+    ctx.eval_mse(pop, mode)
+    wall, kms, launches, logical, executed = 0.0, 0.0, 0, 0, 0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _, _, st = ctx.eval_mse(pop, mode)
+        wall += time.perf_counter() - t0
+        kms += st["interp_kernel_ms"]
+        launches += st["interp_launches"]
+        logical += st["logical_node_evals"]
+        executed += st["executed_node_evals"]
+    return {"node_evals_per_s": logical / wall, "ms_per_eval": 1e3 * wall / reps,
+            "interp_kernel_ms": kms / reps, "interp_launches": launches // reps,
+            "executed_node_evals": executed // reps, "logical_node_evals": logical // reps}
+
+
+def cpu_baseline(X, nv, pop_tok, pop_val, pop_off, pop_len, target_s=12.0):
+    """Oracle evaluate_cpu_mse restatement on a sample of the population: all
+    expressions x the first n datapoints, n sized for ~target_s seconds."""
+    from oracle import oracle as O
+    cores = max(1, min(16, os.cpu_count() or 1))
+    N = X.shape[1]
+    exprs = [(pop_tok[o:o + l], pop_val[o:o + l]) for o, l in zip(pop_off, pop_len)]
+    pop = O.Population(exprs)
+    tokens = int(pop.len.sum())
+
+    def run(n):
+        Xs = np.ascontiguousarray(X[:, :n])
+        t0 = time.perf_counter()
+        O.mse32(pop, Xs, nv, n, threads=cores)
+        return time.perf_counter() - t0
+
+    n = 2048
+    dt = run(n)
+    n = int(min(N, max(2048, n * target_s / max(dt, 1e-3))))
+    dt = run(n)
+    rate = tokens * n / dt
+    return {"value": rate, "unit": "node-evals/s", "cores": cores, "kind": "port",
+            "sample": f"all {len(exprs)} expressions ({tokens} tokens) x the first {n} of {N} "
+                      f"datapoints, {dt:.1f} s, fp64-accumulated MSE as cl_cpu_eval.cpp:94-168"}
+
+
+def constant_fit_bench(ctx, seed, adj_steps, fd_steps, E=4096, N=262144, nv=5):
+    """Adjoint vs central-difference gradient evaluations/s (one grad-eval =
+    dMSE/d(all free constants) of one expression over N points)."""
+    import torch
+    from symbolic_regression_gpu_amd import Population
+    from symbolic_regression_gpu_amd.api import GaParams
+    from symbolic_regression_gpu_amd.autograd import ConstantFit
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(1.0, 5.0, size=(nv + 1, N)).astype(np.float32)
+    ctx.set_data(X)
+    target = make_target(ctx, Population, nv, N, 6, seed + 3, 20)
+    y = ctx.eval_predictions(target)[0]
+    X[nv] = np.nan_to_num(y, nan=0.0, posinf=0.0, neginf=0.0) + \
+        rng.normal(0.0, 0.01, size=N).astype(np.float32)
+    ctx.set_data(X)
+    params = GaParams.make(num_vars=nv, max_tokens=200, max_depth=10, const_min=-2.0, const_max=2.0,
+                           unary_ops=GRAMMAR_UNARY, binary_ops=GRAMMAR_BINARY)
+    exprs, s = [], seed + 4
+    while len(exprs) < E:
+        ctx.ga_init(params, 8192, s)
+        tok, val, ln, cap = ctx.ga_population()
+        for i in range(8192):
+            if ln[i] >= 50 and len(exprs) < E:
+                exprs.append((tok[i * cap:i * cap + ln[i]].copy(),
+                              val[i * cap:i * cap + ln[i]].astype(np.float64)))
+        s += 1
+    slot_rows, vals = [], []
+    for t, v in exprs:
+        sl = np.full(len(t), -1, np.int32)
+        consts = np.flatnonzero(t == 0)
+        k = min(len(consts), int(rng.integers(2, 9)))
+        sl[consts[:k]] = np.arange(k, dtype=np.int32)
+        v = v.copy()
+        v[consts[:k]] = rng.normal(size=k).astype(np.float32)
+        slot_rows.append(sl)
+        vals.append(v)
+    pop = Population.from_exprs([(t, v) for (t, _), v in zip(exprs, vals)])
+    fit = ConstantFit(ctx, pop, np.concatenate(slot_rows))
+    consts = fit.initial().requires_grad_()
+    opt = torch.optim.Adam([consts], lr=1e-2)
+
+    def adam_step():
+        opt.zero_grad()
+        m = fit.mse(consts)
+        m.nansum().backward()
+        opt.step()
+        return m
+
+    m0 = adam_step()  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(adj_steps):
+        m = adam_step()
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t0) / adj_steps
+    fit.fd_grad(consts, 1e-3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(fd_steps):
+        fit.fd_grad(consts, 1e-3)
+    torch.cuda.synchronize()
+    t_fd = (time.perf_counter() - t0) / fd_steps
+    fin = torch.isfinite(m0) & torch.isfinite(m)
+    return {"workload": f"{E} expressions of 50-200 tokens ({fit.tokens} tokens), 2-8 free "
+                        f"constants, {nv} vars, {N} datapoints; Adam lr 1e-2",
+            "adjoint_grad_evals_per_s": E / t_adj, "fd_grad_evals_per_s": E / t_fd,
+            "adjoint_over_fd": t_fd / t_adj, "ms_per_adam_step": 1e3 * t_adj,
+            "ms_per_fd_gradient": 1e3 * t_fd, "adam_steps_timed": adj_steps,
+            "median_mse_first_last": [float(m0[fin].median()), float(m[fin].median())]}
+
+
+def copy_calibration(mib):
+    """A device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--N", type=int, default=1 << 20)
+    ap.add_argument("--reps", type=int, default=5, help="timed batched/plain evaluations")
+    ap.add_argument("--adj-steps", type=int, default=5)
+    ap.add_argument("--fd-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fit", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the evaluator would shard datapoints with one all-reduce of per-expression "
+              "partial sums per generation).", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+
+    import torch
+    from symbolic_regression_gpu_amd import EVAL_BATCHED, EVAL_CACHED, EVAL_PLAIN, Context
+    from symbolic_regression_gpu_amd.api import SelectParams
+    torch.cuda.init()
+    ctx = Context(0)
+    N = args.N
+    log(f"[bench] evolution workload: pop 1000, N={N}")
+    X, nv, (tok, val, ln, cap) = evolution_setup(ctx, N, args.seed)
+    pop0 = to_population(tok, val, ln, cap)
+    flops = operator_nodes(pop0) * N
+    bytes_alg = (nv + 1) * 4 * N
+
+    # roofline + plain vs batched on the initial population
+    log("[bench] batched evaluation")
+    batched = time_evals(ctx, pop0, EVAL_BATCHED, args.reps)
+    log("[bench] plain one-launch-per-expression evaluation")
+    plain = time_evals(ctx, pop0, EVAL_PLAIN, max(2, args.reps // 2))
+    log("[bench] subtree-cached evaluation")
+    ctx.cache_config(2048, 3, 10)
+    ctx.cache_clear()
+    _, _, st_c1 = ctx.eval_mse(pop0, EVAL_CACHED)
+    cached = time_evals(ctx, pop0, EVAL_CACHED, 2)
+    cached["first_call_new_subtrees"] = st_c1["num_new_subtrees"]
+    cached["coverage"] = st_c1["coverage"]
+
+    # the timed generations
+    sel = SelectParams(tournament=7, p_mutation=0.1, elite_frac=0.05)
+    ctx.ga_set_population(tok, val, ln, cap)
+    log(f"[bench] {args.warmup} warm-up + {args.steps} timed generations (hipGraph)")
+    ctx.ga_run(sel, EVAL_BATCHED, args.warmup, 0, args.seed, True)
+    torch.cuda.synchronize()
+    ctx.sync()
+    t0 = time.perf_counter()
+    hist, ms_dev, st = ctx.ga_run(sel, EVAL_BATCHED, args.steps, args.warmup, args.seed, True)
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gen_node_evals = st["logical_node_evals"]
+    value = gen_node_evals / wall
+    best = float(np.nanmin(np.where(np.isfinite(hist[-1]), hist[-1], np.nan)))
+
+    # cached evolution (host-assisted detection, device evaluation and GA)
+    log("[bench] cached generations")
+    ctx.cache_clear()
+    ctx.ga_set_population(tok, val, ln, cap)
+    t0 = time.perf_counter()
+    _, _, st_cg = ctx.ga_run(sel, EVAL_CACHED, max(2, args.steps // 4), 0, args.seed, False)
+    wall_c = time.perf_counter() - t0
+    cached_gen = {"node_evals_per_s": st_cg["logical_node_evals"] / wall_c,
+                  "ms_per_generation": 1e3 * wall_c / max(2, args.steps // 4),
+                  "executed_over_logical": st_cg["executed_node_evals"] / max(st_cg["logical_node_evals"], 1)}
+
+    fit = None
+    if not args.no_fit:
+        log("[bench] constant fitting: adjoint vs finite differences")
+        fit = constant_fit_bench(ctx, args.seed + 10, args.adj_steps, args.fd_steps)
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline")
+        cpu = cpu_baseline(X, nv, pop0.tokens, pop0.values, pop0.offsets, pop0.lengths)
+
+    k_ms = batched["interp_kernel_ms"]
+    achieved = flops / (k_ms * 1e-3) / 1e12
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "node-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded Philox GP population and target tree, X ~ U[1,5] fp32",
+        "config": {
+            "workload": "evolution loop: population 1000, 1M datapoints, 3 vars, max depth 8, "
+                        "<=255 tokens, tournament 7, crossover 0.9 / mutation 0.1, elites 5%, "
+                        "target depth-5 tree; one step = one device generation (hipGraph)",
+            "N": N, "population": 1000, "tokens_initial": int(pop0.tokens.size),
+            "node_evals_per_generation": gen_node_evals // max(args.steps, 1),
+            "best_mse_last_generation": best,
+            "device_ms_per_generation": ms_dev / max(args.steps, 1),
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": traffic,
+            "kernel": "k_batched (MODE_MSE), one launch = 1000 expressions x 1M datapoints",
+            "kernel_ms": k_ms,
+            "flops_per_launch": flops,
+            "hbm_algorithmic_bytes_per_launch": bytes_alg,
+            "hbm_achieved_GBs": bytes_alg / (k_ms * 1e-3) / 1e9,
+            "hbm_frac": bytes_alg / (k_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+        },
+        "cpu_baseline": cpu,
+        "plain_per_expression": plain,
+        "batched_eval": batched,
+        "batched_over_plain": batched["node_evals_per_s"] / plain["node_evals_per_s"],
+        "cached_eval": cached,
+        "cached_evolution": cached_gen,
+        "constant_fit": fit,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -83,6 +83,8 @@ typedef struct {
     int32_t num_kernel_launches;
     int64_t logical_node_evals;  /* sum_e len_e * N */
     int64_t executed_node_evals; /* token evaluations actually run */
+    double interp_kernel_ms;     /* interpreter kernels alone (HIP events on the context stream) */
+    int32_t interp_launches;     /* interpreter kernel launches timed in interp_kernel_ms */
 } sr_stats;
 
 /* Launch shape of the batched interpreter. */

@@ -50,7 +50,8 @@ class _Stats(C.Structure):
         ("avg_subtree_size", C.c_double), ("coverage", C.c_double),
         ("num_new_subtrees", C.c_int32), ("num_cached", C.c_int32),
         ("num_kernel_launches", C.c_int32), ("logical_node_evals", C.c_int64),
-        ("executed_node_evals", C.c_int64),
+        ("executed_node_evals", C.c_int64), ("interp_kernel_ms", C.c_double),
+        ("interp_launches", C.c_int32),
     ]
 
     def as_dict(self):

@@ -156,7 +156,9 @@ struct Ctx {
     SubtreeCache cache;
     GaState ga;
     ParamState par;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // ev[0..3]: transfer / kernel / transfer brackets of one call;
+    // ev[4..5]: the interpreter kernel alone.
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 
     int fail(int code, const std::string &msg)
     {
@@ -196,7 +198,9 @@ struct BatchArgs;
 int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp);
 // Enqueue counter reset + batched kernel (+ finalize for MODE_MSE).  a must
 // carry the program / order / output pointers; the rest is filled in here.
-int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, hipStream_t s);
+// k0/k1 (optional) are recorded right before and after the interpreter launch.
+int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, hipStream_t s,
+                    hipEvent_t k0 = nullptr, hipEvent_t k1 = nullptr);
 
 // Batched evaluation over a compiled program already described by host
 // vectors; uploads, launches, finalizes.  mode: MODE_MSE/MODE_PRED/MODE_COLUMN.

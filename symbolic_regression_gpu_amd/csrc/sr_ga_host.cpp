@@ -348,6 +348,8 @@ int sr_ga_run(sr_ctx *x, const sr_select_params *spp, int mode, int32_t gens, ui
             acc.executed_node_evals += s1.executed_node_evals;
             acc.detect_time_ms += s1.detect_time_ms;
             acc.kernel_time_ms += s1.kernel_time_ms;
+            acc.interp_kernel_ms += s1.interp_kernel_ms;
+            acc.interp_launches += s1.interp_launches;
             acc.transfer_time_ms += s1.transfer_time_ms;
             acc.jit_compile_time_ms += s1.jit_compile_time_ms;
             acc.num_kernel_launches += s1.num_kernel_launches + 2;

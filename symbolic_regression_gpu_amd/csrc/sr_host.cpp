@@ -125,7 +125,8 @@ int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp)
     return SR_OK;
 }
 
-int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, hipStream_t s)
+int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, hipStream_t s,
+                    hipEvent_t k0, hipEvent_t k1)
 {
     hipError_t he;
     a.X = c.X;
@@ -141,8 +142,10 @@ int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, 
     a.chunks_per_queue = lp.cpq;
     a.P = lp.P;
     if ((he = launch_zero_u32(a.counters, lp.nq, s)) != hipSuccess) return c.hip_fail(he, "zero counters");
+    if (k0) hipEventRecord(k0, s);
     if ((he = launch_batched(mode, lp.dpt, lp.xc, a, lp.grid, s)) != hipSuccess)
         return c.hip_fail(he, "batched launch");
+    if (k1) hipEventRecord(k1, s);
     if (mode == MODE_MSE) {
         if ((he = launch_finalize(a.psum, a.pcnt, lp.P, E, c.d_mse.as<float>(), c.d_cnt.as<long long>(),
                                   s)) != hipSuccess)
@@ -193,12 +196,18 @@ int run_batched(Ctx &c, const Program &p, int mode, const float *cache_cols, flo
     a.out_col = a.prog_off + 3 * E;
     a.cache = cache_cols;
     a.out = out_dev;
-    rc = enqueue_batched(c, mode, lp, a, E, s);
+    rc = enqueue_batched(c, mode, lp, a, E, s, st ? c.ev[4] : nullptr, st ? c.ev[5] : nullptr);
     if (rc) return rc;
     hipEventRecord(c.ev[2], s);
     if (st) {
         st->num_kernel_launches += mode == MODE_MSE ? 3 : 2;
         st->executed_node_evals += p.tokens * c.N;
+        // every caller synchronises right after; waiting here costs nothing
+        if ((he = hipEventSynchronize(c.ev[5])) != hipSuccess) return c.hip_fail(he, "interpreter");
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, c.ev[4], c.ev[5]);
+        st->interp_kernel_ms += ms;
+        st->interp_launches += 1;
     }
     return SR_OK;
 }
@@ -226,18 +235,25 @@ static int run_plain(Ctx &c, const Program &p, sr_stats *st)
     const Ins *dprog = c.d_prog.as<Ins>();
     double *psum = c.d_psum.as<double>();
     long long *pcnt = c.d_pcnt.as<long long>();
+    hipEventRecord(c.ev[4], s);
     for (int e = 0; e < E; ++e) {
         int slots = p.need[e] <= 16 ? 16 : 64;
         he = launch_plain(slots, xc, dprog + p.off[e], p.len[e], c.X, c.N, c.nv, psum + (size_t)e * P,
                           pcnt + (size_t)e * P, s);
         if (he != hipSuccess) return c.hip_fail(he, "plain launch");
     }
+    hipEventRecord(c.ev[5], s);
     if ((he = launch_finalize(psum, pcnt, P, E, c.d_mse.as<float>(), c.d_cnt.as<long long>(), s)) != hipSuccess)
         return c.hip_fail(he, "finalize launch");
     hipEventRecord(c.ev[2], s);
     if (st) {
         st->num_kernel_launches += E + 1;
         st->executed_node_evals += p.tokens * c.N;
+        if ((he = hipEventSynchronize(c.ev[5])) != hipSuccess) return c.hip_fail(he, "plain interpreter");
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, c.ev[4], c.ev[5]);
+        st->interp_kernel_ms += ms;
+        st->interp_launches += E;
     }
     return SR_OK;
 }
