@@ -36,6 +36,15 @@
  *   sro_op32 for SR_OP_EXP_CLAMPED (a new opcode 29, not in the reference
  *       evaluator) follows the clamped exp of evolution_tests.cpp:40; parity
  *       for it is closed-form only.
+ *   sro_predict64_bound (FP64 value + first-order error bound of an fp32
+ *       evaluation; no reference counterpart): pinned by the fp32 oracle
+ *       lying inside it on every point of all-opcode populations, by planted
+ *       ulp errors being caught, and by exact / branch-flip special cases.
+ *   sro_mse_grad64_bound (FP64 forward-mode dMSE/dconstants + error bound of
+ *       an fp32 adjoint; no reference counterpart, the reference has no
+ *       gradient code): the gradient is pinned to torch FP64 autograd; the
+ *       bound by a fp32 emulation of the adjoint pass staying inside it and a
+ *       1% error planted in any opcode's partial landing far outside it.
  */
 #ifndef SR_ORACLE_H
 #define SR_ORACLE_H
