@@ -338,7 +338,7 @@ int eval_cached(Ctx &c, const sr_population &pop, sr_stats *st)
     Program ps;
     rc = build_program(c, sp, false, ps, (int)cache.keys.size());
     if (rc) return rc;
-    rc = run_batched(c, ps, MODE_MSE, cache.cols.as<float>(), nullptr, nullptr, st);
+    rc = run_batched(c, ps, MODE_MSE_REF, cache.cols.as<float>(), nullptr, nullptr, st);
     if (rc) return rc;
     if (st) {
         st->detect_time_ms += detect_ms;

@@ -27,12 +27,20 @@ SR_DEV float qnan() { return __builtin_nanf(""); }
 // small arrays so the compiler keeps each one in VGPRs and indexes it with
 // s_set_gpr_idx (the index -- stack pointer or variable id -- is wave-uniform).
 // ---------------------------------------------------------------------------
+// Variable registers per datapoint: at least 9, so a read at a uniform
+// dynamic index goes through GPR-index mode instead of a compare/select chain
+// over every element (which the backend picks for arrays of <= 8 floats).
+template <int XC>
+constexpr int kXRegs = XC > 8 ? XC : 9;
+
 template <int S, int XC>
 struct Lane {
     float (&stk)[S];
-    float (&x)[XC];
+    float (&x)[kXRegs<XC>];
     float &top;
     float &o;  // second operand / loaded leaf
+    float (&xn)[XC];  // next tile's variables (loaded ahead)
+    float &y, &yn;    // target of this / the next tile
 };
 
 // Apply a unary op (10..27, 29) to every lane's top.  op is wave-uniform.
@@ -101,9 +109,9 @@ SR_DEV void with_lanes(F &&f, L &...ls)
         // Separate locals per datapoint: each array is its own alloca, which
         // the compiler promotes to VGPRs (a struct of arrays would not be).
         float stk[S];
-        float x[XC];
-        float top = 0.0f, o = 0.0f;
-        Lane<S, XC> l{stk, x, top, o};
+        float x[kXRegs<XC>], xn[XC];
+        float top = 0.0f, o = 0.0f, y = 0.0f, yn = 0.0f;
+        Lane<S, XC> l{stk, x, top, o, xn, y, yn};
         with_lanes<DPT, S, XC>(f, ls..., l);
     }
 }

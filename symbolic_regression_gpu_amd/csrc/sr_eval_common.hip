@@ -7,15 +7,18 @@ namespace sr {
 hipError_t launch_batched_mse(int dpt, int xc, const BatchArgs &a, int grid, hipStream_t s);
 hipError_t launch_batched_pred(int dpt, int xc, const BatchArgs &a, int grid, hipStream_t s);
 hipError_t launch_batched_col(int dpt, int xc, const BatchArgs &a, int grid, hipStream_t s);
+hipError_t launch_batched_mseref(int dpt, int xc, const BatchArgs &a, int grid, hipStream_t s);
 int batched_occupancy_mse(int dpt, int xc);
 int batched_occupancy_pred(int dpt, int xc);
 int batched_occupancy_col(int dpt, int xc);
+int batched_occupancy_mseref(int dpt, int xc);
 
 hipError_t launch_batched(int mode, int dpt, int xc, const BatchArgs &a, int grid, hipStream_t s)
 {
     if (mode == MODE_MSE) return launch_batched_mse(dpt, xc, a, grid, s);
     if (mode == MODE_PRED) return launch_batched_pred(dpt, xc, a, grid, s);
     if (mode == MODE_COLUMN) return launch_batched_col(dpt, xc, a, grid, s);
+    if (mode == MODE_MSE_REF) return launch_batched_mseref(dpt, xc, a, grid, s);
     return hipErrorInvalidValue;
 }
 
@@ -24,6 +27,7 @@ int batched_occupancy(int mode, int dpt, int xc)
     if (mode == MODE_MSE) return batched_occupancy_mse(dpt, xc);
     if (mode == MODE_PRED) return batched_occupancy_pred(dpt, xc);
     if (mode == MODE_COLUMN) return batched_occupancy_col(dpt, xc);
+    if (mode == MODE_MSE_REF) return batched_occupancy_mseref(dpt, xc);
     return 0;
 }
 

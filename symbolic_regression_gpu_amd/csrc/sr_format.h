@@ -56,9 +56,20 @@ SR_HD bool is_commutative(int t) { return t == OP_ADD || t == OP_MUL; }
 // arg : const bits / variable index / cache slot.
 constexpr int SRC_NONE = 0, SRC_CONST = 1, SRC_VAR = 2, SRC_REF = 3, SRC_POP = 4;
 constexpr int BC_PUSH = 1 << 11, BC_SWAP = 1 << 12;
+// Bits 13..14: instruction class (0 = leaf load, 1 = binary op, 2 = unary op),
+// so the interpreter dispatches operand staging on bits 8..14 in one switch.
+constexpr int BC_CLS_SHIFT = 13;
+constexpr int CL_LOAD = 0, CL_BIN = 1, CL_UN = 2;
 
 SR_HD int bc_op(int code) { return code & 0xff; }
 SR_HD int bc_src(int code) { return (code >> 8) & 7; }
+SR_HD int bc_cls(int code) { return (code >> BC_CLS_SHIFT) & 3; }
+// Staging form: src | push << 3 | swap << 4 | class << 5.
+SR_HD int bc_form(int code) { return (code >> 8) & 0x7f; }
+constexpr int bc_form_of(int src, int push, int swap, int cls)
+{
+    return src | (push ? 8 : 0) | (swap ? 16 : 0) | (cls << 5);
+}
 
 struct Ins {
     int32_t code;
@@ -206,7 +217,7 @@ SR_HD int compile_expr(const int32_t *tok, const float *val, int len, int nv, in
                 int32_t arg;
                 leaf_src(t, val[f.node], nv, src, arg);
                 if (leaf_pos) leaf_pos[n] = f.node;
-                out[n++] = {0 | (src << 8) | push, arg};
+                out[n++] = {0 | (src << 8) | push | (CL_LOAD << BC_CLS_SHIFT), arg};
                 --fsp;
                 continue;
             }
@@ -217,7 +228,7 @@ SR_HD int compile_expr(const int32_t *tok, const float *val, int len, int nv, in
                     int32_t arg;
                     leaf_src(tok[c], val[c], nv, src, arg);
                     if (leaf_pos) leaf_pos[n] = c;
-                    out[n++] = {t | (src << 8) | push, arg};
+                    out[n++] = {t | (src << 8) | push | (CL_UN << BC_CLS_SHIFT), arg};
                     --fsp;
                 } else {
                     f.stage = 1;
@@ -235,10 +246,10 @@ SR_HD int compile_expr(const int32_t *tok, const float *val, int len, int nv, in
                 int32_t arg;
                 leaf_src(tok[a], val[a], nv, src, arg);
                 if (leaf_pos) leaf_pos[n] = a;
-                out[n++] = {0 | (src << 8) | push, arg};
+                out[n++] = {0 | (src << 8) | push | (CL_LOAD << BC_CLS_SHIFT), arg};
                 leaf_src(tok[b], val[b], nv, src, arg);
                 if (leaf_pos) leaf_pos[n] = b;
-                out[n++] = {t | (src << 8), arg};
+                out[n++] = {t | (src << 8) | (CL_BIN << BC_CLS_SHIFT), arg};
                 --fsp;
             } else if (la) {
                 // evaluate B, then top = f(A, top)
@@ -263,13 +274,15 @@ SR_HD int compile_expr(const int32_t *tok, const float *val, int len, int nv, in
         if (f.stage == 1) {
             if (f.second < 0) {
                 if (leaf_pos) leaf_pos[n] = -1;
-                out[n++] = {t | (SRC_NONE << 8), 0};
+                out[n++] = {t | (SRC_NONE << 8) | (CL_UN << BC_CLS_SHIFT), 0};
             } else {
                 int src;
                 int32_t arg;
                 leaf_src(tok[f.second], val[f.second], nv, src, arg);
                 if (leaf_pos) leaf_pos[n] = f.second;
-                out[n++] = {t | (src << 8) | f.swap, arg};
+                out[n++] = {t | (src << 8) | f.swap |
+                                ((is_binary(t) ? CL_BIN : CL_UN) << BC_CLS_SHIFT),
+                            arg};
             }
             --fsp;
             continue;
@@ -281,7 +294,7 @@ SR_HD int compile_expr(const int32_t *tok, const float *val, int len, int nv, in
         }
         // stage 3: o = pop (first subtree's value), top = second subtree
         if (leaf_pos) leaf_pos[n] = -1;
-        out[n++] = {t | (SRC_POP << 8) | f.swap, 0};
+        out[n++] = {t | (SRC_POP << 8) | f.swap | (CL_BIN << BC_CLS_SHIFT), 0};
         --fsp;
     }
     return n;

@@ -101,7 +101,7 @@ int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp)
     hipError_t he;
     lp.xc = pick_xc(c.nv);
     lp.dpt = c.tune.dpt;
-    if (lp.dpt != 1 && lp.dpt != 2 && lp.dpt != 4 && lp.dpt != 8) lp.dpt = 4;
+    if (lp.dpt != 1 && lp.dpt != 2 && lp.dpt != 4) lp.dpt = 4;
     if (lp.xc == 32 && lp.dpt > 2) lp.dpt = 2;
     lp.nq = c.tune.n_queues > 1 ? c.tune.n_queues : 1;
     const int64_t tile = 64 * lp.dpt;
@@ -116,7 +116,7 @@ int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp)
     if (bpc <= 0 || bpc > occ) bpc = occ;
     lp.grid = (int)round_up((int64_t)c.num_cu * bpc, lp.nq);
     if ((he = c.d_counters.ensure(64 * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
-    if (mode == MODE_MSE) {
+    if (is_mse_mode(mode)) {
         if ((he = c.d_psum.ensure((size_t)E * lp.P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
         if ((he = c.d_pcnt.ensure((size_t)E * lp.P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
         if ((he = c.d_mse.ensure((size_t)E * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
@@ -146,7 +146,7 @@ int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, 
     if ((he = launch_batched(mode, lp.dpt, lp.xc, a, lp.grid, s)) != hipSuccess)
         return c.hip_fail(he, "batched launch");
     if (k1) hipEventRecord(k1, s);
-    if (mode == MODE_MSE) {
+    if (is_mse_mode(mode)) {
         if ((he = launch_finalize(a.psum, a.pcnt, lp.P, E, c.d_mse.as<float>(), c.d_cnt.as<long long>(),
                                   s)) != hipSuccess)
             return c.hip_fail(he, "finalize launch");
@@ -200,7 +200,7 @@ int run_batched(Ctx &c, const Program &p, int mode, const float *cache_cols, flo
     if (rc) return rc;
     hipEventRecord(c.ev[2], s);
     if (st) {
-        st->num_kernel_launches += mode == MODE_MSE ? 3 : 2;
+        st->num_kernel_launches += is_mse_mode(mode) ? 3 : 2;
         st->executed_node_evals += p.tokens * c.N;
         // every caller synchronises right after; waiting here costs nothing
         if ((he = hipEventSynchronize(c.ev[5])) != hipSuccess) return c.hip_fail(he, "interpreter");

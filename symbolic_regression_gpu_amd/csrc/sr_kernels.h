@@ -13,7 +13,11 @@
 
 namespace sr {
 
-enum Mode { MODE_MSE = 0, MODE_PRED = 1, MODE_COLUMN = 2 };
+// MODE_MSE_REF: MSE of skeletons whose OP_REF leaves read cache columns; the
+// other modes never see OP_REF, so their instruction loop has no memory loads
+// (and the next tile's X loads stay in flight across it).
+enum Mode { MODE_MSE = 0, MODE_PRED = 1, MODE_COLUMN = 2, MODE_MSE_REF = 3 };
+constexpr bool is_mse_mode(int m) { return m == MODE_MSE || m == MODE_MSE_REF; }
 
 constexpr int kBlock = 256;  // 4 waves per workgroup, one per SIMD
 constexpr int kWave = 64;
@@ -47,19 +51,30 @@ struct BatchArgs {
 // Lane l of a wave evaluates datapoints tile + d*64 + l for d < DPT, so every
 // X / y / cache access is a coalesced 256-byte wave load.
 // ---------------------------------------------------------------------------
+// Program words staged per wave in LDS: the longest program plus one word
+// read ahead past its end.
+constexpr int kProgLds = MAX_EXPR_LEN + 1;
+
 template <int DPT, int S, int XC, int MODE>
 __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
 {
+    // Each wave copies its expression's program into LDS once per work item
+    // (coalesced), then reads instruction i + 1 from LDS while executing
+    // instruction i, so dispatch never waits on memory.  X (and y) of the next
+    // datapoint tile are loaded while the current tile runs.
+    __shared__ Ins s_prog[kBlock / kWave][kProgLds];
     const int lane = threadIdx.x & (kWave - 1);
+    Ins *lp = s_prog[threadIdx.x / kWave];
     const int q = a.n_queues > 1 ? (int)(blockIdx.x % (unsigned)a.n_queues) : 0;
     const int items = a.n_exprs * a.chunks_per_queue;
     const int64_t N = a.N;
+    constexpr int64_t kTile = (int64_t)kWave * DPT;
 
     with_lanes<DPT, S, XC>([&](auto &...L) {
         for (;;) {
             int item = 0;
             if (lane == 0) item = (int)atomicAdd(&a.counters[q], 1u);
-            item = __shfl(item, 0, kWave);
+            item = __builtin_amdgcn_readfirstlane(item);
             if (item >= items) break;
             const int rank = item / a.chunks_per_queue;
             const int chunk = item - rank * a.chunks_per_queue;
@@ -67,6 +82,8 @@ __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
             const Ins *prog =
                 a.progs + (a.prog_off ? (int64_t)a.prog_off[e] : (int64_t)e * a.prog_stride);
             const int n = a.prog_len[e];
+            for (int k = lane; k < n; k += kWave) lp[k] = prog[k];
+            __builtin_amdgcn_wave_barrier();
             int64_t q0 = (int64_t)q * a.slice;
             int64_t q1 = q0 + a.slice < N ? q0 + a.slice : N;
             int64_t c0 = q0 + (int64_t)chunk * a.chunk;
@@ -77,48 +94,64 @@ __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
             if constexpr (MODE == MODE_COLUMN) col_out = a.out + (int64_t)a.out_col[e] * N;
             if constexpr (MODE == MODE_PRED) col_out = a.out + (int64_t)e * N;
 
-            for (int64_t t0 = c0; t0 < c1; t0 += (int64_t)kWave * DPT) {
-                // Per-lane datapoint index, clamped for loads.
+            auto load_tile = [&](int64_t t) {
                 for_each_lane(
                     [&](int d, auto &l) {
-                        int64_t dp = t0 + d * kWave + lane;
+                        int64_t dp = t + d * kWave + lane;
                         int64_t dc = dp < N ? dp : N - 1;
 #pragma unroll
                         for (int k = 0; k < XC; ++k)
-                            if (k < a.nv) l.x[k] = a.X[(int64_t)k * N + dc];
+                            if (k < a.nv) l.xn[k] = a.X[(int64_t)k * N + dc];
+                        if constexpr (is_mse_mode(MODE)) l.yn = a.X[(int64_t)a.nv * N + dc];
                     },
                     L...);
+            };
+            if (c0 < c1) load_tile(c0);
+            for (int64_t t0 = c0; t0 < c1; t0 += kTile) {
+                for_each_lane(
+                    [&](int, auto &l) {
+#pragma unroll
+                        for (int k = 0; k < XC; ++k) l.x[k] = l.xn[k];
+                        l.y = l.yn;
+                    },
+                    L...);
+                if (t0 + kTile < c1) load_tile(t0 + kTile);
                 int sp = 0;
+                int code = __builtin_amdgcn_readfirstlane(lp[0].code);
+                int arg = __builtin_amdgcn_readfirstlane(lp[0].arg);
                 for (int i = 0; i < n; ++i) {
-                    const Ins ins = prog[i];
-                    const int code = ins.code;
+                    const Ins nx = lp[i + 1];
+                    // Stack traffic sits in top-level ifs of its own (a stack
+                    // write nested deeper in the decode makes the compiler copy
+                    // whole stack arrays between branches); constants and
+                    // variables are fetched branch-free.
                     if (code & BC_PUSH) {
                         ((L.stk[sp] = L.top), ...);
                         ++sp;
                     }
                     const int src = bc_src(code);
-                    const int op = bc_op(code);
-                    if (src == SRC_CONST) {
-                        const float c = __int_as_float(ins.arg);
-                        ((L.o = c), ...);
-                    } else if (src == SRC_VAR) {
-                        const int k = ins.arg;
-                        ((L.o = L.x[k]), ...);
-                    } else if (src == SRC_POP) {
+                    if (src == SRC_POP) {
                         --sp;
                         ((L.o = L.stk[sp]), ...);
-                    } else if (src == SRC_REF) {
-                        const float *colr = a.cache + (int64_t)ins.arg * N;
+                    } else if (MODE == MODE_MSE_REF && src == SRC_REF) {
+                        const float *colr = a.cache + (int64_t)arg * N;
                         for_each_lane(
                             [&](int d, auto &l) {
                                 int64_t dp = t0 + d * kWave + lane;
                                 l.o = colr[dp < N ? dp : N - 1];
                             },
                             L...);
+                    } else {
+                        const bool is_var = src == SRC_VAR;
+                        const int xi = is_var ? arg : 0;
+                        const float cv = __int_as_float(arg);
+                        for_each_lane([&](int, auto &l) { l.o = is_var ? l.x[xi] : cv; }, L...);
                     }
-                    if (op == 0) {
-                        ((L.top = L.o), ...);
-                    } else if (op < 10) {
+                    const int cls = bc_cls(code);
+                    if (cls != CL_BIN) {
+                        if (src != SRC_NONE) ((L.top = L.o), ...);
+                        if (cls == CL_UN) apply_unary(bc_op(code), L...);
+                    } else {
                         if (code & BC_SWAP) {
                             for_each_lane(
                                 [&](int, auto &l) {
@@ -128,21 +161,18 @@ __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
                                 },
                                 L...);
                         }
-                        apply_binary(op, L...);
-                    } else {
-                        if (src != SRC_NONE) ((L.top = L.o), ...);
-                        apply_unary(op, L...);
+                        apply_binary(bc_op(code), L...);
                     }
+                    code = __builtin_amdgcn_readfirstlane(nx.code);
+                    arg = __builtin_amdgcn_readfirstlane(nx.arg);
                 }
                 // Epilogue.
                 for_each_lane(
                     [&](int d, auto &l) {
                         int64_t dp = t0 + d * kWave + lane;
-                        if constexpr (MODE == MODE_MSE) {
-                            int64_t dc = dp < N ? dp : N - 1;
-                            float y = a.X[(int64_t)a.nv * N + dc];
+                        if constexpr (is_mse_mode(MODE)) {
                             bool ok = dp < c1 && isfinite(l.top);
-                            float diff = l.top - y;
+                            float diff = l.top - l.y;
                             float sq = diff * diff;
                             sum += ok ? (double)sq : 0.0;
                             cnt += ok ? 1 : 0;
@@ -152,7 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
                     },
                     L...);
             }
-            if constexpr (MODE == MODE_MSE) {
+            if constexpr (is_mse_mode(MODE)) {
                 sum = wave_sum(sum);
                 cnt = wave_sum(cnt);
                 if (lane == 0) {
@@ -161,6 +191,8 @@ __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
                     a.pcnt[(int64_t)e * a.P + slot] = cnt;
                 }
             }
+            // the next item restages lp: finish this item's reads first
+            __builtin_amdgcn_wave_barrier();
         }
     });
 }

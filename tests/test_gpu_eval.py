@@ -83,7 +83,7 @@ def test_batched_tunings_agree(ctx):
     check_population(pop, X, 3, N, base, vbase)
     saved = ctx.tuning
     try:
-        for dpt in (1, 2, 4, 8):
+        for dpt in (1, 2, 4):
             for nq in (1, 8):
                 for chunk in (512, 16384):
                     ctx.tuning = Tuning(dpt=dpt, chunk=chunk, n_queues=nq, blocks_per_cu=0)
