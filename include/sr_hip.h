@@ -170,6 +170,19 @@ int sr_compile_param_expr(const int32_t *tokens, const float *values, const int3
                           int32_t len, int32_t nv, int32_t *out_code, int32_t *out_arg,
                           int32_t *stack_need);
 
+/* hipRTC backend (mode SR_EVAL_JIT; replaces the NVRTC path of
+ * cl_gpu_nvrtc.cu:152-548).  Host-only helpers, no GPU needed: the generated
+ * HIP source for a population (nv variables, dpt datapoints per lane) and a
+ * compile of it for gfx950 (log, compile ms, code-object bytes).  sr_eval_mse
+ * in mode SR_EVAL_JIT compiles once per distinct population (stats report
+ * jit_compile_time_ms next to kernel_time_ms). */
+int sr_jit_source(const sr_population *pop, int32_t nv, int32_t dpt, char *out, int64_t cap,
+                  int64_t *len);
+int sr_jit_compile(const sr_population *pop, int32_t nv, int32_t dpt, char *log_out, int64_t log_cap,
+                   double *compile_ms, int64_t *code_bytes);
+/* The compiled code object itself (out may be NULL to query *len). */
+int sr_jit_code(const sr_population *pop, int32_t nv, int32_t dpt, void *out, int64_t cap, int64_t *len);
+
 int sr_get_tuning(sr_ctx *ctx, sr_tuning *out);
 int sr_set_tuning(sr_ctx *ctx, const sr_tuning *t);
 

@@ -108,6 +108,10 @@ _SIGNATURES = {
     "sr_compile_expr": [_i32p, _f32p, C.c_int32, C.c_int32, _i32p, _i32p, C.POINTER(C.c_int32)],
     "sr_compile_param_expr": [_i32p, _f32p, _i32p, C.c_int32, C.c_int32, _i32p, _i32p,
                               C.POINTER(C.c_int32)],
+    "sr_jit_source": [C.POINTER(_Pop), C.c_int32, C.c_int32, _vp, C.c_int64, C.POINTER(C.c_int64)],
+    "sr_jit_compile": [C.POINTER(_Pop), C.c_int32, C.c_int32, _vp, C.c_int64,
+                       C.POINTER(C.c_double), C.POINTER(C.c_int64)],
+    "sr_jit_code": [C.POINTER(_Pop), C.c_int32, C.c_int32, _vp, C.c_int64, C.POINTER(C.c_int64)],
     "sr_cache_config": [_vp, C.c_int32, C.c_int32, C.c_int32],
     "sr_cache_clear": [_vp],
     "sr_cache_state": [_vp, _vp, C.c_int32, C.POINTER(C.c_int32)],
@@ -142,6 +146,15 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise SrError(-8, f"{LIB_PATH} is missing: run python -m symbolic_regression_gpu_amd.build")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7 /
+    # libhiprtc.so.7, and whichever copy is loaded first serves every later
+    # user of that soname.  Loading torch first makes this library share the
+    # runtime that owns torch's tensors and streams (the autograd shim passes
+    # both across), whatever the caller imported first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     for name, args in _SIGNATURES.items():
         f = getattr(L, name, None)
@@ -169,6 +182,41 @@ def compile_expr(tokens, values, nv, const_slot=None):
     if n < 0:
         raise SrError(n, "compile failed")
     return code[:n], arg[:n], need.value
+
+
+def jit_source(pop, nv: int, dpt: int = 4) -> str:
+    """HIP source the hipRTC backend generates for a population."""
+    n = C.c_int64()
+    rc = lib().sr_jit_source(C.byref(pop._c()), nv, dpt, None, 0, C.byref(n))
+    if rc != 0:
+        raise SrError(rc, "jit source")
+    buf = C.create_string_buffer(n.value + 1)
+    lib().sr_jit_source(C.byref(pop._c()), nv, dpt, buf, n.value + 1, C.byref(n))
+    return buf.value.decode()
+
+
+def jit_compile(pop, nv: int, dpt: int = 4):
+    """Compile the generated module for gfx950 on the host: (ms, code bytes)."""
+    log = C.create_string_buffer(1 << 16)
+    ms = C.c_double()
+    nb = C.c_int64()
+    rc = lib().sr_jit_compile(C.byref(pop._c()), nv, dpt, log, 1 << 16, C.byref(ms), C.byref(nb))
+    if rc != 0:
+        raise SrError(rc, log.value.decode())
+    return ms.value, nb.value
+
+
+def jit_code(pop, nv: int, dpt: int = 4) -> bytes:
+    """The gfx950 code object the hipRTC backend builds for a population."""
+    n = C.c_int64()
+    rc = lib().sr_jit_code(C.byref(pop._c()), nv, dpt, None, 0, C.byref(n))
+    if rc != 0:
+        raise SrError(rc, "jit code")
+    buf = C.create_string_buffer(n.value)
+    rc = lib().sr_jit_code(C.byref(pop._c()), nv, dpt, buf, n.value, C.byref(n))
+    if rc != 0:
+        raise SrError(rc, "jit code")
+    return buf.raw[:n.value]
 
 
 class Population:

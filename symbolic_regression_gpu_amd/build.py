@@ -30,13 +30,25 @@ FLAGS = [
 ]
 
 
+def embed_ops_header() -> None:
+    """sr_ops.h as a C++ raw string literal, compiled into the hipRTC prelude."""
+    with open(os.path.join(CSRC, "sr_ops.h")) as f:
+        text = f.read()
+    out = 'R"SROPS(' + text + ')SROPS"\n'
+    path = os.path.join(BUILD, "sr_ops_src.inc")
+    old = open(path).read() if os.path.exists(path) else None
+    if old != out:
+        with open(path, "w") as f:
+            f.write(out)
+
+
 def sources():
     return sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
 def _compile(src: str) -> tuple[str, str]:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, "-x", "hip", *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, "-x", "hip", *FLAGS, "-I", BUILD, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
@@ -45,6 +57,7 @@ def _compile(src: str) -> tuple[str, str]:
 
 def build(jobs: int | None = None, verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
+    embed_ops_header()
     srcs = sources()
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

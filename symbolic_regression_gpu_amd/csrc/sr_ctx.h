@@ -132,6 +132,24 @@ struct ParamState {
     DevBuf progs, meta, psum, pcnt, pgrad, tape, fd_mse;
 };
 
+// hipRTC module of the last JIT-evaluated population.
+struct JitState {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    // The code object the module was loaded from.  The runtime may read the
+    // image after hipModuleLoadData returns (deferred loading), so it lives as
+    // long as the module.
+    std::vector<char> image;
+    uint64_t key = 0;
+    int occ = 1;
+    int64_t code_bytes = 0;
+    double last_compile_ms = 0.0;
+    JitState() = default;
+    JitState(const JitState &) = delete;
+    JitState &operator=(const JitState &) = delete;
+    ~JitState();
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -156,6 +174,7 @@ struct Ctx {
     SubtreeCache cache;
     GaState ga;
     ParamState par;
+    JitState jit;
     // ev[0..3]: transfer / kernel / transfer brackets of one call;
     // ev[4..5]: the interpreter kernel alone.
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};

@@ -12,6 +12,7 @@
 #include <stdint.h>
 
 #include "sr_format.h"
+#include "sr_ops.h"
 
 #define SR_DEV __device__ __forceinline__
 
@@ -47,31 +48,27 @@ struct Lane {
 template <typename... L>
 SR_DEV void apply_unary(int op, L &...l)
 {
+    using namespace sr_ops;
     switch (op) {
-    case OP_SIN: ((l.top = sinf(l.top)), ...); break;
-    case OP_COS: ((l.top = cosf(l.top)), ...); break;
-    case OP_TAN: ((l.top = tanf(l.top)), ...); break;
-    case OP_SINH: ((l.top = sinhf(l.top)), ...); break;
-    case OP_COSH: ((l.top = coshf(l.top)), ...); break;
-    case OP_TANH: ((l.top = tanhf(l.top)), ...); break;
-    case OP_EXP: ((l.top = expf(l.top)), ...); break;
-    case OP_LOG: ((l.top = logf(l.top)), ...); break;
-    case OP_INV: ((l.top = 1.0f / l.top), ...); break;
-    case OP_ASIN: ((l.top = asinf(l.top)), ...); break;
-    case OP_ACOS: ((l.top = acosf(l.top)), ...); break;
-    case OP_ATAN: ((l.top = atanf(l.top)), ...); break;
-    case OP_LOOSE_LOG:
-        ((l.top = (l.top == 0.0f) ? -kMaxVal : logf(fabsf(l.top))), ...);
-        break;
-    case OP_LOOSE_INV:
-        ((l.top = 1.0f / (fabsf(l.top) <= kDelta ? (l.top < 0.0f ? -kDelta : kDelta) : l.top)),
-         ...);
-        break;
-    case OP_ABS: ((l.top = fabsf(l.top)), ...); break;
-    case OP_NEG: ((l.top = -l.top), ...); break;
-    case OP_SQRT: ((l.top = sqrtf(l.top)), ...); break;
-    case OP_LOOSE_SQRT: ((l.top = sqrtf(fabsf(l.top))), ...); break;
-    case OP_EXP_CLAMPED: ((l.top = (l.top > 10.0f) ? kMaxVal : expf(l.top)), ...); break;
+    case OP_SIN: ((l.top = f_sin(l.top)), ...); break;
+    case OP_COS: ((l.top = f_cos(l.top)), ...); break;
+    case OP_TAN: ((l.top = f_tan(l.top)), ...); break;
+    case OP_SINH: ((l.top = f_sinh(l.top)), ...); break;
+    case OP_COSH: ((l.top = f_cosh(l.top)), ...); break;
+    case OP_TANH: ((l.top = f_tanh(l.top)), ...); break;
+    case OP_EXP: ((l.top = f_exp(l.top)), ...); break;
+    case OP_LOG: ((l.top = f_log(l.top)), ...); break;
+    case OP_INV: ((l.top = f_inv(l.top)), ...); break;
+    case OP_ASIN: ((l.top = f_asin(l.top)), ...); break;
+    case OP_ACOS: ((l.top = f_acos(l.top)), ...); break;
+    case OP_ATAN: ((l.top = f_atan(l.top)), ...); break;
+    case OP_LOOSE_LOG: ((l.top = f_loose_log(l.top)), ...); break;
+    case OP_LOOSE_INV: ((l.top = f_loose_inv(l.top)), ...); break;
+    case OP_ABS: ((l.top = f_abs(l.top)), ...); break;
+    case OP_NEG: ((l.top = f_neg(l.top)), ...); break;
+    case OP_SQRT: ((l.top = f_sqrt(l.top)), ...); break;
+    case OP_LOOSE_SQRT: ((l.top = f_loose_sqrt(l.top)), ...); break;
+    case OP_EXP_CLAMPED: ((l.top = f_exp_clamped(l.top)), ...); break;
     default: ((l.top = 0.0f), ...); break;
     }
 }
@@ -80,20 +77,17 @@ SR_DEV void apply_unary(int op, L &...l)
 template <typename... L>
 SR_DEV void apply_binary(int op, L &...l)
 {
+    using namespace sr_ops;
     switch (op) {
-    case OP_ADD: ((l.top = l.top + l.o), ...); break;
-    case OP_SUB: ((l.top = l.top - l.o), ...); break;
-    case OP_MUL: ((l.top = l.top * l.o), ...); break;
-    case OP_DIV: ((l.top = (l.o == 0.0f) ? qnan() : l.top / l.o), ...); break;
-    case OP_POW: ((l.top = powf(l.top, l.o)), ...); break;
-    case OP_MIN: ((l.top = fminf(l.top, l.o)), ...); break;
-    case OP_MAX: ((l.top = fmaxf(l.top, l.o)), ...); break;
-    case OP_LOOSE_DIV:
-        ((l.top = l.top / (fabsf(l.o) <= kDelta ? (l.o < 0.0f ? -kDelta : kDelta) : l.o)), ...);
-        break;
-    case OP_LOOSE_POW:
-        ((l.top = (l.top == 0.0f && l.o == 0.0f) ? 0.0f : powf(fabsf(l.top), l.o)), ...);
-        break;
+    case OP_ADD: ((l.top = f_add(l.top, l.o)), ...); break;
+    case OP_SUB: ((l.top = f_sub(l.top, l.o)), ...); break;
+    case OP_MUL: ((l.top = f_mul(l.top, l.o)), ...); break;
+    case OP_DIV: ((l.top = f_div(l.top, l.o)), ...); break;
+    case OP_POW: ((l.top = f_pow(l.top, l.o)), ...); break;
+    case OP_MIN: ((l.top = f_min(l.top, l.o)), ...); break;
+    case OP_MAX: ((l.top = f_max(l.top, l.o)), ...); break;
+    case OP_LOOSE_DIV: ((l.top = f_loose_div(l.top, l.o)), ...); break;
+    case OP_LOOSE_POW: ((l.top = f_loose_pow(l.top, l.o)), ...); break;
     default: ((l.top = 0.0f), ...); break;
     }
 }

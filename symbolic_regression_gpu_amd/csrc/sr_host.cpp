@@ -259,8 +259,7 @@ static int run_plain(Ctx &c, const Program &p, sr_stats *st)
 }
 
 int eval_cached(Ctx &c, const sr_population &pop, sr_stats *st);  // sr_cache.cpp
-int eval_jit(Ctx &c, const sr_population &pop, float *mse_out, int64_t *valid_out,
-             sr_stats *st);  // sr_jit.cpp
+int eval_jit(Ctx &c, const sr_population &pop, sr_stats *st);  // sr_jit.cpp
 
 }  // namespace sr
 
@@ -400,11 +399,8 @@ int sr_eval_mse(sr_ctx *x, const sr_population *pop, int mode, float *mse_out, i
     if (E == 0) return SR_OK;
 
     if (mode == SR_EVAL_JIT) {
-        rc = eval_jit(c, *pop, mse_out, valid_out, S);
-        S->total_eval_time_ms = now_ms() - t0;
-        return rc;
-    }
-    if (mode == SR_EVAL_CACHED) {
+        rc = eval_jit(c, *pop, S);
+    } else if (mode == SR_EVAL_CACHED) {
         rc = eval_cached(c, *pop, S);
     } else if (mode == SR_EVAL_PLAIN || mode == SR_EVAL_BATCHED) {
         Program p;
