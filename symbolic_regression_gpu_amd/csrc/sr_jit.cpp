@@ -270,8 +270,19 @@ JitState::~JitState()
     if (mod) hipModuleUnload(mod);
 }
 
+// Device execution of generated modules is switched off: the first GPU run of
+// this path (100 expressions, 10 variables, 100k datapoints) stopped making
+// progress after the compile, and again after the code-object lifetime fix;
+// the cause is not identified yet.  Source generation and host compilation
+// (sr_jit_source / sr_jit_compile / sr_jit_code) stay available.
+static constexpr bool kJitDeviceEnabled = false;
+
 int eval_jit(Ctx &c, const sr_population &pop, sr_stats *st)
 {
+    if (!kJitDeviceEnabled)
+        return c.fail(SR_ERR_UNSUPPORTED,
+                      "SR_EVAL_JIT: device execution of hipRTC modules is disabled in this build "
+                      "(host-side sr_jit_source / sr_jit_compile work)");
     const int E = pop.num_exprs;
     Program check;
     int rc = build_program(c, pop, false, check);  // validation + lengths for the order

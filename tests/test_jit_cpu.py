@@ -42,3 +42,4 @@ def test_malformed_rejected():
     for tok, val in [([1, 0], [0, 1.0]), ([-1], [-1.0]), ([-2], [0.0])]:
         with pytest.raises(SrError):
             jit_source(Population.from_exprs([(tok, val)]), nv=2)
+
