@@ -164,11 +164,16 @@ int sr_sync(sr_ctx *ctx);
 int sr_compile_expr(const int32_t *tokens, const float *values, int32_t len, int32_t nv,
                     int32_t *out_code, int32_t *out_arg, int32_t *stack_need);
 /* Same, as the parametric program the adjoint pass runs: const_slot [len] is
- * -1 or the free-constant slot (0..7) of each token; SRC_POP instructions get
- * the index of the instruction whose output they pop. */
+ * -1 or the free-constant slot (0..7) of each token.  Code bit 15 marks
+ * instructions whose value depends on a free constant, bits 20..30 hold 1 +
+ * the tape row of a value the reverse pass reads; a SRC_POP's arg is the tape
+ * row of the popped value.  out_aux (optional) [len]: for an active
+ * instruction the tape row of the previous top when it is an operand (else
+ * -1), for an inactive one the first instruction of its subtree.
+ * *tape_rows (optional): tape rows the program needs. */
 int sr_compile_param_expr(const int32_t *tokens, const float *values, const int32_t *const_slot,
-                          int32_t len, int32_t nv, int32_t *out_code, int32_t *out_arg,
-                          int32_t *stack_need);
+                          int32_t len, int32_t nv, int32_t *out_code, int32_t *out_arg, int32_t *out_aux,
+                          int32_t *stack_need, int32_t *tape_rows);
 
 /* hipRTC backend (mode SR_EVAL_JIT; replaces the NVRTC path of
  * cl_gpu_nvrtc.cu:152-548).  Host-only helpers, no GPU needed: the generated

@@ -106,8 +106,8 @@ _SIGNATURES = {
     "sr_get_tuning": [_vp, C.POINTER(_Tuning)],
     "sr_set_tuning": [_vp, C.POINTER(_Tuning)],
     "sr_compile_expr": [_i32p, _f32p, C.c_int32, C.c_int32, _i32p, _i32p, C.POINTER(C.c_int32)],
-    "sr_compile_param_expr": [_i32p, _f32p, _i32p, C.c_int32, C.c_int32, _i32p, _i32p,
-                              C.POINTER(C.c_int32)],
+    "sr_compile_param_expr": [_i32p, _f32p, _i32p, C.c_int32, C.c_int32, _i32p, _i32p, _vp,
+                              C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "sr_jit_source": [C.POINTER(_Pop), C.c_int32, C.c_int32, _vp, C.c_int64, C.POINTER(C.c_int64)],
     "sr_jit_compile": [C.POINTER(_Pop), C.c_int32, C.c_int32, _vp, C.c_int64,
                        C.POINTER(C.c_double), C.POINTER(C.c_int64)],
@@ -167,8 +167,9 @@ def lib():
 
 
 def compile_expr(tokens, values, nv, const_slot=None):
-    """Bytecode of one expression: (code int32[n], arg int32[n], stack_need);
-    with const_slot (per token -1 or 0..7), the parametric program."""
+    """Bytecode of one expression: (code int32[n], arg int32[n], stack_need).
+    With const_slot (per token -1 or 0..7), the parametric program the adjoint
+    pass runs: (code, arg, stack_need, aux int32[n], tape_rows)."""
     t = np.ascontiguousarray(tokens, np.int32)
     v = np.ascontiguousarray(values, np.float32)
     code = np.empty(max(len(t), 1), np.int32)
@@ -176,12 +177,17 @@ def compile_expr(tokens, values, nv, const_slot=None):
     need = C.c_int32()
     if const_slot is None:
         n = lib().sr_compile_expr(t, v, len(t), nv, code, arg, C.byref(need))
-    else:
-        sl = np.ascontiguousarray(const_slot, np.int32)
-        n = lib().sr_compile_param_expr(t, v, sl, len(t), nv, code, arg, C.byref(need))
+        if n < 0:
+            raise SrError(n, "compile failed")
+        return code[:n], arg[:n], need.value
+    sl = np.ascontiguousarray(const_slot, np.int32)
+    aux = np.empty(max(len(t), 1), np.int32)
+    rows = C.c_int32()
+    n = lib().sr_compile_param_expr(t, v, sl, len(t), nv, code, arg, aux.ctypes.data, C.byref(need),
+                                    C.byref(rows))
     if n < 0:
         raise SrError(n, "compile failed")
-    return code[:n], arg[:n], need.value
+    return code[:n], arg[:n], need.value, aux[:n], rows.value
 
 
 def jit_source(pop, nv: int, dpt: int = 4) -> str:

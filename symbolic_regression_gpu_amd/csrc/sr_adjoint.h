@@ -24,6 +24,7 @@ namespace sr {
 
 struct ParamArgs {
     const Ins *progs;
+    const int32_t *aux;       // per instruction, packed like progs (sr_adjoint_prog.h)
     const int32_t *prog_off;
     const int32_t *prog_len;
     const int32_t *order;
@@ -39,7 +40,7 @@ struct ParamArgs {
     int n_exprs, nv, n_queues, chunks_per_queue, P;
     int replicas;           // forward: 1 = plain, 2*MAX_SLOTS = central differences
     float fd_h;
-    int tape_stride;        // instructions per wave in the global tape
+    int tape_stride;        // tape rows per wave
 };
 
 // d f(u) / du for unary op, given input u and output y.

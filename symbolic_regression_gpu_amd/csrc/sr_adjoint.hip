@@ -1,37 +1,55 @@
 // sr_adjoint.hip -- parametric forward (with finite-difference replicas) and
 // reverse-mode adjoint kernels for constant fitting.
 #include "sr_adjoint.h"
+#include "sr_adjoint_prog.h"
 #include "sr_launch.h"
 #include "sr_param_launch.h"
 
 namespace sr {
 
-// Constant operand of a parametric program.
-SR_DEV float param_const(const ParamArgs &a, int code, int32_t arg, int e, int pert_slot, float delta)
+// Copy expression e's program into this wave's LDS slot, with every free
+// constant's current value (plus the finite-difference step on slot pert)
+// written into its operand word, so the interpreter reads it as a literal.
+SR_DEV void stage_param_program(const ParamArgs &a, int e, int n, int lane, int pert, float delta, Ins *lp,
+                                int32_t *la)
 {
-    const int s = bc_slot(code);
-    if (s < 0) return __int_as_float(arg);
-    const float c = a.cvals[(int64_t)e * MAX_SLOTS + s];
-    return s == pert_slot ? c + delta : c;
+    const Ins *prog = a.progs + a.prog_off[e];
+    const int32_t *aux = a.aux ? a.aux + a.prog_off[e] : nullptr;
+    for (int k = lane; k < n; k += kWave) {
+        Ins ins = prog[k];
+        const int s = bc_src(ins.code) == SRC_CONST ? bc_slot(ins.code) : -1;
+        if (s >= 0) {
+            const float c = a.cvals[(int64_t)e * MAX_SLOTS + s];
+            ins.arg = __float_as_int(s == pert ? c + delta : c);
+        }
+        lp[k] = ins;
+        if (la) la[k] = aux[k];
+    }
+    __builtin_amdgcn_wave_barrier();
 }
 
 // ---------------------------------------------------------------------------
 // Forward: MSE of every (replica, expression).  Replica r of a
 // central-difference batch perturbs free constant r/2 by +h (r even) or -h.
+// Same interpreter as the batched evaluation (program in LDS, one-ahead
+// instruction reads, next tile's X loaded during the current one).
 // ---------------------------------------------------------------------------
 template <int DPT, int XC>
 __global__ __launch_bounds__(kBlock) void k_param_fwd(ParamArgs a)
 {
+    __shared__ Ins s_prog[kBlock / kWave][kProgLds];
     const int lane = threadIdx.x & (kWave - 1);
+    Ins *lp = s_prog[threadIdx.x / kWave];
     const int q = a.n_queues > 1 ? (int)(blockIdx.x % (unsigned)a.n_queues) : 0;
     const int per_rep = a.n_exprs * a.chunks_per_queue;
     const int items = a.replicas * per_rep;
     const int64_t N = a.N;
-    with_lanes<DPT, kStackSlots, XC>([&](auto &...L) {
+    constexpr int64_t kTile = (int64_t)kWave * DPT;
+    with_lanes<DPT, kStackRegs, XC>([&](auto &...L) {
         for (;;) {
             int item = 0;
             if (lane == 0) item = (int)atomicAdd(&a.counters[q], 1u);
-            item = __shfl(item, 0, kWave);
+            item = __builtin_amdgcn_readfirstlane(item);
             if (item >= items) break;
             const int rep = item / per_rep;
             const int rem = item - rep * per_rep;
@@ -41,7 +59,6 @@ __global__ __launch_bounds__(kBlock) void k_param_fwd(ParamArgs a)
             const int pert = a.replicas > 1 ? rep >> 1 : -1;
             const float delta = a.replicas > 1 ? ((rep & 1) ? -a.fd_h : a.fd_h) : 0.0f;
             const bool skip = pert >= a.nslots[e];
-            const Ins *prog = a.progs + a.prog_off[e];
             const int n = a.prog_len[e];
             const int64_t q0 = (int64_t)q * a.slice;
             const int64_t q1 = q0 + a.slice < N ? q0 + a.slice : N;
@@ -49,60 +66,44 @@ __global__ __launch_bounds__(kBlock) void k_param_fwd(ParamArgs a)
             const int64_t c1 = skip ? c0 : (c0 + a.chunk < q1 ? c0 + a.chunk : q1);
             double sum = 0.0;
             long long cnt = 0;
-            for (int64_t t0 = c0; t0 < c1; t0 += (int64_t)kWave * DPT) {
+            if (!skip) stage_param_program(a, e, n, lane, pert, delta, lp, nullptr);
+            auto load_tile = [&](int64_t t) {
                 for_each_lane(
                     [&](int d, auto &l) {
-                        int64_t dp = t0 + d * kWave + lane;
+                        int64_t dp = t + d * kWave + lane;
                         int64_t dc = dp < N ? dp : N - 1;
 #pragma unroll
                         for (int k = 0; k < XC; ++k)
-                            if (k < a.nv) l.x[k] = a.X[(int64_t)k * N + dc];
+                            if (k < a.nv) l.xn[k] = a.X[(int64_t)k * N + dc];
+                        l.yn = a.X[(int64_t)a.nv * N + dc];
                     },
                     L...);
+            };
+            auto noref = [](auto &...) {};
+            if (c0 < c1) load_tile(c0);
+            for (int64_t t0 = c0; t0 < c1; t0 += kTile) {
+                for_each_lane(
+                    [&](int, auto &l) {
+#pragma unroll
+                        for (int k = 0; k < XC; ++k) l.x[k] = l.xn[k];
+                        l.y = l.yn;
+                    },
+                    L...);
+                if (t0 + kTile < c1) load_tile(t0 + kTile);
                 int sp = 0;
+                int code = __builtin_amdgcn_readfirstlane(lp[0].code);
+                int arg = __builtin_amdgcn_readfirstlane(lp[0].arg);
                 for (int i = 0; i < n; ++i) {
-                    const Ins ins = prog[i];
-                    const int code = ins.code;
-                    if (code & BC_PUSH) {
-                        ((L.stk[sp] = L.top), ...);
-                        ++sp;
-                    }
-                    const int src = bc_src(code);
-                    const int op = bc_op(code);
-                    if (src == SRC_CONST) {
-                        const float c = param_const(a, code, ins.arg, e, pert, delta);
-                        ((L.o = c), ...);
-                    } else if (src == SRC_VAR) {
-                        const int k = ins.arg;
-                        ((L.o = L.x[k]), ...);
-                    } else if (src == SRC_POP) {
-                        --sp;
-                        ((L.o = L.stk[sp]), ...);
-                    }
-                    if (op == 0) {
-                        ((L.top = L.o), ...);
-                    } else if (op < 10) {
-                        if (code & BC_SWAP)
-                            for_each_lane(
-                                [&](int, auto &l) {
-                                    float t = l.top;
-                                    l.top = l.o;
-                                    l.o = t;
-                                },
-                                L...);
-                        apply_binary(op, L...);
-                    } else {
-                        if (src != SRC_NONE) ((L.top = L.o), ...);
-                        apply_unary(op, L...);
-                    }
+                    const Ins nx = lp[i + 1];
+                    interp_step<false>(code, arg, __int_as_float(arg), sp, noref, L...);
+                    code = __builtin_amdgcn_readfirstlane(nx.code);
+                    arg = __builtin_amdgcn_readfirstlane(nx.arg);
                 }
                 for_each_lane(
                     [&](int d, auto &l) {
                         int64_t dp = t0 + d * kWave + lane;
-                        int64_t dc = dp < N ? dp : N - 1;
-                        float y = a.X[(int64_t)a.nv * N + dc];
                         bool ok = dp < c1 && isfinite(l.top);
-                        float diff = l.top - y;
+                        float diff = l.top - l.y;
                         float sq = diff * diff;
                         sum += ok ? (double)sq : 0.0;
                         cnt += ok ? 1 : 0;
@@ -117,167 +118,270 @@ __global__ __launch_bounds__(kBlock) void k_param_fwd(ParamArgs a)
                 a.psum[row * a.P + slot] = sum;
                 a.pcnt[row * a.P + slot] = cnt;
             }
+            __builtin_amdgcn_wave_barrier();
         }
     });
 }
 
 // ---------------------------------------------------------------------------
-// Tape: one 256-byte row (64 lanes) per instruction in this wave's region of
-// a global scratch buffer; forward stores are fire-and-forget and the reverse
-// sweep loads each row one step ahead of its use.
+// Adjoint.  Per tile of 64 * DPT datapoints: the forward pass stores every
+// value the reverse pass reads on this wave's tape (rows assigned on the
+// host; one 64 * DPT float row each, lane-major so a lane's DPT values are
+// one vector store); the reverse pass walks the active instructions from the
+// last one down, jumping over subtrees without a free constant, with an
+// adjoint stack mirroring the value stack (a popped operand's adjoint is
+// pushed and returned at the instruction that pushed the value).
 // ---------------------------------------------------------------------------
-struct GlobalTape {
-    float *base;  // this wave's region, lane-offset; row i at base[i*64]
-    SR_DEV float get(int i) const { return base[(int64_t)i * kWave]; }
-    SR_DEV void set(int i, float v) { base[(int64_t)i * kWave] = v; }
-};
+constexpr int kAdjRegs = 9;  // > 8 floats: uniform dynamic indexing uses GPR-index mode
 
-// Accumulator view for apply_unary / apply_binary on a single datapoint.
-struct TopO {
+template <int XC>
+struct AdjLane {
+    float (&stk)[kStackRegs];
+    float (&x)[kXRegs<XC>];
     float &top;
     float &o;
+    float (&astk)[kAdjRegs];
+    float (&tg)[kAdjRegs];
+    float (&gacc)[MAX_SLOTS];
+    float &at;
 };
 
-template <int XC, typename Tape>
-SR_DEV void adjoint_items(const ParamArgs &a, Tape &tape, int lane, int q)
+template <int DPT, int XC, typename F, typename... L>
+SR_DEV void with_adj_lanes(F &&f, L &...ls)
 {
-    const int items = a.n_exprs * a.chunks_per_queue;
-    const int64_t N = a.N;
-    float stk[kStackSlots], astk[kStackSlots], x[XC], tg[MAX_SLOTS], gacc[MAX_SLOTS];
-    for (;;) {
-        int item = 0;
-        if (lane == 0) item = (int)atomicAdd(&a.counters[q], 1u);
-        item = __shfl(item, 0, kWave);
-        if (item >= items) break;
-        const int rank = item / a.chunks_per_queue;
-        const int chunk = item - rank * a.chunks_per_queue;
-        const int e = a.order[rank];
-        const Ins *prog = a.progs + a.prog_off[e];
-        const int n = a.prog_len[e];
-        const int64_t q0 = (int64_t)q * a.slice;
-        const int64_t q1 = q0 + a.slice < N ? q0 + a.slice : N;
-        const int64_t c0 = q0 + (int64_t)chunk * a.chunk;
-        const int64_t c1 = c0 + a.chunk < q1 ? c0 + a.chunk : q1;
-        double sum = 0.0;
-        long long cnt = 0;
-#pragma unroll
-        for (int k = 0; k < MAX_SLOTS; ++k) gacc[k] = 0.0f;
-        for (int64_t t0 = c0; t0 < c1; t0 += kWave) {
-            const int64_t dp = t0 + lane;
-            const int64_t dc = dp < N ? dp : N - 1;
-#pragma unroll
-            for (int k = 0; k < XC; ++k)
-                if (k < a.nv) x[k] = a.X[(int64_t)k * N + dc];
-            // forward with tape
-            float top = 0.0f, o = 0.0f;
-            int sp = 0;
-            for (int i = 0; i < n; ++i) {
-                const Ins ins = prog[i];
-                const int code = ins.code;
-                if (code & BC_PUSH) stk[sp++] = top;
-                const int src = bc_src(code);
-                const int op = bc_op(code);
-                if (src == SRC_CONST) o = param_const(a, code, ins.arg, e, -1, 0.0f);
-                else if (src == SRC_VAR) o = x[ins.arg];
-                else if (src == SRC_POP) o = stk[--sp];
-                if (op == 0) {
-                    top = o;
-                } else if (op < 10) {
-                    if (code & BC_SWAP) {
-                        float t = top;
-                        top = o;
-                        o = t;
-                    }
-                    TopO l{top, o};
-                    apply_binary(op, l);
-                } else {
-                    if (src != SRC_NONE) top = o;
-                    TopO l{top, o};
-                    apply_unary(op, l);
-                }
-                tape.set(i, top);
-            }
-            const float y = a.X[(int64_t)a.nv * N + dc];
-            const bool ok = dp < c1 && isfinite(top);
-            const float diff = top - y;
-            const float sq = diff * diff;
-            sum += ok ? (double)sq : 0.0;
-            cnt += ok ? 1 : 0;
-            // reverse sweep
-            if (__ballot(ok)) {
-#pragma unroll
-                for (int k = 0; k < MAX_SLOTS; ++k) tg[k] = 0.0f;
-                float at = ok ? diff : 0.0f;
-                float out_i = top;
-                int asp = 0;
-                float x_next = n > 1 ? tape.get(n - 2) : 0.0f;
-                for (int i = n - 1; i >= 0; --i) {
-                    const Ins ins = prog[i];
-                    const int code = ins.code;
-                    const int src = bc_src(code);
-                    const int op = bc_op(code);
-                    const float x_in = x_next;  // top before instruction i (tape row i-1)
-                    x_next = i > 1 ? tape.get(i - 2) : 0.0f;
-                    float ov = 0.0f;
-                    if (src == SRC_CONST) ov = param_const(a, code, ins.arg, e, -1, 0.0f);
-                    else if (src == SRC_VAR) ov = x[ins.arg];
-                    else if (src == SRC_POP) ov = tape.get(ins.arg);
-                    float a_o = 0.0f, a_in = 0.0f;
-                    if (op == 0) {
-                        a_o = at;
-                    } else if (op < 10) {
-                        const bool sw = (code & BC_SWAP) != 0;
-                        const float A = sw ? ov : x_in;
-                        const float B = sw ? x_in : ov;
-                        float dA, dB;
-                        d_binary(op, A, B, out_i, dA, dB);
-                        const float gA = at * dA, gB = at * dB;
-                        a_o = sw ? gA : gB;
-                        a_in = sw ? gB : gA;
-                    } else {
-                        const float u = src != SRC_NONE ? ov : x_in;
-                        const float g = at * d_unary(op, u, out_i);
-                        if (src != SRC_NONE) a_o = g;
-                        else a_in = g;
-                    }
-                    if (src == SRC_CONST) {
-                        const int s = bc_slot(code);
-                        if (s >= 0) tg[s] += a_o;
-                    } else if (src == SRC_POP) {
-                        astk[asp++] = a_o;
-                    }
-                    if (code & BC_PUSH) a_in = astk[--asp];
-                    at = a_in;
-                    out_i = x_in;
-                }
-#pragma unroll
-                for (int k = 0; k < MAX_SLOTS; ++k) gacc[k] += ok ? tg[k] : 0.0f;
-            }
-        }
-        sum = wave_sum(sum);
-        cnt = wave_sum(cnt);
-        double g[MAX_SLOTS];
-#pragma unroll
-        for (int k = 0; k < MAX_SLOTS; ++k) g[k] = wave_sum((double)gacc[k]);
-        if (lane == 0) {
-            const int slot = q * a.chunks_per_queue + chunk;
-            a.psum[(int64_t)e * a.P + slot] = sum;
-            a.pcnt[(int64_t)e * a.P + slot] = cnt;
-#pragma unroll
-            for (int k = 0; k < MAX_SLOTS; ++k)
-                a.pgrad[((int64_t)e * a.P + slot) * MAX_SLOTS + k] = g[k];
-        }
+    if constexpr (sizeof...(L) == DPT) {
+        f(ls...);
+    } else {
+        float stk[kStackRegs];
+        float x[kXRegs<XC>];
+        float top = 0.0f, o = 0.0f, at = 0.0f;
+        float astk[kAdjRegs], tg[kAdjRegs];
+        float gacc[MAX_SLOTS];
+        AdjLane<XC> l{stk, x, top, o, astk, tg, gacc, at};
+        with_adj_lanes<DPT, XC>(f, ls..., l);
     }
 }
 
-template <int XC>
+// Tape row r of this wave: 64 lanes x DPT floats, lane-major.
+template <int DPT>
+struct WaveTape {
+    float *base;  // this wave's region
+    int lane;
+    template <typename... L>
+    SR_DEV void store(int r, L &...l) const
+    {
+        float *p = base + ((int64_t)r * kWave + lane) * DPT;
+        if constexpr (DPT == 4) {
+            float4 v;
+            float t[4];
+            int d = 0;
+            ((t[d++] = l.top), ...);
+            v.x = t[0];
+            v.y = t[1];
+            v.z = t[2];
+            v.w = t[3];
+            *reinterpret_cast<float4 *>(p) = v;
+        } else if constexpr (DPT == 2) {
+            float t[2];
+            int d = 0;
+            ((t[d++] = l.top), ...);
+            *reinterpret_cast<float2 *>(p) = make_float2(t[0], t[1]);
+        } else {
+            ((*p = l.top), ...);
+        }
+    }
+    SR_DEV void load(int r, float (&v)[DPT]) const
+    {
+        const float *p = base + ((int64_t)r * kWave + lane) * DPT;
+        if constexpr (DPT == 4) {
+            const float4 t = *reinterpret_cast<const float4 *>(p);
+            v[0] = t.x;
+            v[1] = t.y;
+            v[2] = t.z;
+            v[3] = t.w;
+        } else if constexpr (DPT == 2) {
+            const float2 t = *reinterpret_cast<const float2 *>(p);
+            v[0] = t.x;
+            v[1] = t.y;
+        } else {
+            v[0] = *p;
+        }
+    }
+};
+
+template <int DPT, int XC>
 __global__ __launch_bounds__(kBlock) void k_adjoint(ParamArgs a)
 {
+    __shared__ Ins s_prog[kBlock / kWave][kProgLds];
+    __shared__ int32_t s_aux[kBlock / kWave][kProgLds];
     const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    Ins *lp = s_prog[w];
+    int32_t *la = s_aux[w];
     const int q = a.n_queues > 1 ? (int)(blockIdx.x % (unsigned)a.n_queues) : 0;
-    const int64_t wave = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-    GlobalTape tape{a.tape + wave * a.tape_stride * kWave + lane};
-    adjoint_items<XC>(a, tape, lane, q);
+    const int64_t wave = (int64_t)blockIdx.x * (kBlock / kWave) + w;
+    const WaveTape<DPT> tape{a.tape + wave * (int64_t)a.tape_stride * kWave * DPT, lane};
+    const int items = a.n_exprs * a.chunks_per_queue;
+    const int64_t N = a.N;
+    constexpr int64_t kTile = (int64_t)kWave * DPT;
+
+    with_adj_lanes<DPT, XC>([&](auto &...L) {
+        for (;;) {
+            int item = 0;
+            if (lane == 0) item = (int)atomicAdd(&a.counters[q], 1u);
+            item = __builtin_amdgcn_readfirstlane(item);
+            if (item >= items) break;
+            const int rank = item / a.chunks_per_queue;
+            const int chunk = item - rank * a.chunks_per_queue;
+            const int e = a.order[rank];
+            const int n = a.prog_len[e];
+            stage_param_program(a, e, n, lane, -1, 0.0f, lp, la);
+            const int64_t q0 = (int64_t)q * a.slice;
+            const int64_t q1 = q0 + a.slice < N ? q0 + a.slice : N;
+            const int64_t c0 = q0 + (int64_t)chunk * a.chunk;
+            const int64_t c1 = c0 + a.chunk < q1 ? c0 + a.chunk : q1;
+            double sum = 0.0;
+            long long cnt = 0;
+            for_each_lane(
+                [&](int, auto &l) {
+#pragma unroll
+                    for (int k = 0; k < MAX_SLOTS; ++k) l.gacc[k] = 0.0f;
+                },
+                L...);
+            auto noref = [](auto &...) {};
+            for (int64_t t0 = c0; t0 < c1; t0 += kTile) {
+                float yv[DPT];
+                bool okv[DPT];
+                for_each_lane(
+                    [&](int d, auto &l) {
+                        int64_t dp = t0 + d * kWave + lane;
+                        int64_t dc = dp < N ? dp : N - 1;
+#pragma unroll
+                        for (int k = 0; k < XC; ++k)
+                            if (k < a.nv) l.x[k] = a.X[(int64_t)k * N + dc];
+                        yv[d] = a.X[(int64_t)a.nv * N + dc];
+                    },
+                    L...);
+                // forward, taping the values the reverse pass reads
+                int sp = 0;
+                int code = __builtin_amdgcn_readfirstlane(lp[0].code);
+                int arg = __builtin_amdgcn_readfirstlane(lp[0].arg);
+                for (int i = 0; i < n; ++i) {
+                    const Ins nx = lp[i + 1];
+                    interp_step<false>(code, arg, __int_as_float(arg), sp, noref, L...);
+                    const int tr = bc_tape(code);
+                    if (tr >= 0) tape.store(tr, L...);
+                    code = __builtin_amdgcn_readfirstlane(nx.code);
+                    arg = __builtin_amdgcn_readfirstlane(nx.arg);
+                }
+                bool any = false;
+                for_each_lane(
+                    [&](int d, auto &l) {
+                        int64_t dp = t0 + d * kWave + lane;
+                        okv[d] = dp < c1 && isfinite(l.top);
+                        float diff = l.top - yv[d];
+                        float sq = diff * diff;
+                        sum += okv[d] ? (double)sq : 0.0;
+                        cnt += okv[d] ? 1 : 0;
+                        l.at = okv[d] ? diff : 0.0f;
+#pragma unroll
+                        for (int k = 0; k < kAdjRegs; ++k) l.tg[k] = 0.0f;
+                        any = any || okv[d];
+                    },
+                    L...);
+                if (!__any(any)) continue;
+                // reverse over the active instructions
+                int asp = 0;
+                int i = n - 1;
+                while (i >= 0) {
+                    const int rc = __builtin_amdgcn_readfirstlane(lp[i].code);
+                    const int ra = __builtin_amdgcn_readfirstlane(lp[i].arg);
+                    const int rx = __builtin_amdgcn_readfirstlane(la[i]);
+                    if (!(rc & BC_ACTIVE)) {
+                        // a subtree without free constants: skip [rx, i]
+                        const int sc = __builtin_amdgcn_readfirstlane(lp[rx].code);
+                        if (sc & BC_PUSH) {
+                            --asp;
+                            ((L.at = L.astk[asp]), ...);
+                        } else {
+                            ((L.at = 0.0f), ...);
+                        }
+                        i = rx - 1;
+                        continue;
+                    }
+                    const int src = bc_src(rc);
+                    const int op = bc_op(rc);
+                    float yo[DPT], xi[DPT], pv[DPT];
+                    tape.load(bc_tape(rc), yo);
+                    if (rx >= 0) tape.load(rx, xi);
+                    if (src == SRC_POP) tape.load(ra, pv);
+                    const float cv = __int_as_float(ra);
+                    const int slot = src == SRC_CONST ? bc_slot(rc) : -1;
+                    for_each_lane(
+                        [&](int d, auto &l) {
+                            const float x_in = rx >= 0 ? xi[d] : 0.0f;
+                            float ov = src == SRC_POP ? pv[d] : (src == SRC_VAR ? l.x[ra] : cv);
+                            float a_o = 0.0f, a_in = 0.0f;
+                            const int cls = bc_cls(rc);
+                            if (cls == CL_LOAD) {
+                                a_o = l.at;
+                            } else if (cls == CL_BIN) {
+                                const bool sw = (rc & BC_SWAP) != 0;
+                                const float A = sw ? ov : x_in;
+                                const float B = sw ? x_in : ov;
+                                float dA, dB;
+                                d_binary(op, A, B, yo[d], dA, dB);
+                                const float gA = l.at * dA, gB = l.at * dB;
+                                a_o = sw ? gA : gB;
+                                a_in = sw ? gB : gA;
+                            } else {
+                                const float u = src != SRC_NONE ? ov : x_in;
+                                const float g = l.at * d_unary(op, u, yo[d]);
+                                if (src != SRC_NONE) a_o = g;
+                                else a_in = g;
+                            }
+                            l.o = a_o;
+                            l.at = a_in;
+                        },
+                        L...);
+                    if (slot >= 0) {
+                        ((L.tg[slot] += L.o), ...);
+                    } else if (src == SRC_POP) {
+                        ((L.astk[asp] = L.o), ...);
+                        ++asp;
+                    }
+                    if (rc & BC_PUSH) {
+                        --asp;
+                        ((L.at = L.astk[asp]), ...);
+                    }
+                    --i;
+                }
+                for_each_lane(
+                    [&](int d, auto &l) {
+#pragma unroll
+                        for (int k = 0; k < MAX_SLOTS; ++k) l.gacc[k] += okv[d] ? l.tg[k] : 0.0f;
+                    },
+                    L...);
+            }
+            sum = wave_sum(sum);
+            cnt = wave_sum(cnt);
+            double g[MAX_SLOTS];
+#pragma unroll
+            for (int k = 0; k < MAX_SLOTS; ++k) {
+                double t = 0.0;
+                for_each_lane([&](int, auto &l) { t += (double)l.gacc[k]; }, L...);
+                g[k] = wave_sum(t);
+            }
+            if (lane == 0) {
+                const int slot = q * a.chunks_per_queue + chunk;
+                a.psum[(int64_t)e * a.P + slot] = sum;
+                a.pcnt[(int64_t)e * a.P + slot] = cnt;
+#pragma unroll
+                for (int k = 0; k < MAX_SLOTS; ++k) a.pgrad[((int64_t)e * a.P + slot) * MAX_SLOTS + k] = g[k];
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    });
 }
 
 // mse[e], and grad[e][k] = 2 * sum_k / count (NaN when no finite point).
@@ -334,26 +438,26 @@ int param_fwd_occupancy(int dpt, int xc)
     return 0;
 }
 
-hipError_t launch_adjoint(int xc, const ParamArgs &a, int grid, hipStream_t s)
+hipError_t launch_adjoint(int dpt, int xc, const ParamArgs &a, int grid, hipStream_t s)
 {
-#define SR_AD(XCV)                                                                    \
-    if (xc == XCV) {                                                                  \
-        hipLaunchKernelGGL((k_adjoint<XCV>), dim3(grid), dim3(kBlock), 0, s, a);       \
-        return hipGetLastError();                                                     \
+#define SR_AD(D, XCV)                                                                    \
+    if (dpt == D && xc == XCV) {                                                         \
+        hipLaunchKernelGGL((k_adjoint<D, XCV>), dim3(grid), dim3(kBlock), 0, s, a);       \
+        return hipGetLastError();                                                        \
     }
-    SR_AD(4) SR_AD(16) SR_AD(32)
+    SR_AD(1, 4) SR_AD(2, 4) SR_AD(1, 16) SR_AD(2, 16) SR_AD(1, 32)
 #undef SR_AD
     return hipErrorInvalidValue;
 }
 
-int adjoint_occupancy(int xc)
+int adjoint_occupancy(int dpt, int xc)
 {
     int b = 0;
-#define SR_AD(XCV)                                                                              \
-    if (xc == XCV)                                                                              \
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void *)k_adjoint<XCV>,   \
+#define SR_AD(D, XCV)                                                                             \
+    if (dpt == D && xc == XCV)                                                                    \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void *)k_adjoint<D, XCV>,  \
                                                             kBlock, 0) == hipSuccess ? b : 0;
-    SR_AD(4) SR_AD(16) SR_AD(32)
+    SR_AD(1, 4) SR_AD(2, 4) SR_AD(1, 16) SR_AD(2, 16) SR_AD(1, 32)
 #undef SR_AD
     return 0;
 }

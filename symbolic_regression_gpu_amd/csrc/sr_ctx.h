@@ -127,9 +127,10 @@ struct GaState {
 struct ParamState {
     bool ready = false;
     int E = 0, max_len = 0;
+    int tape_rows = 1;  // most tape rows any program needs
     int64_t tokens = 0;
     std::vector<int32_t> nslots;
-    DevBuf progs, meta, psum, pcnt, pgrad, tape, fd_mse;
+    DevBuf progs, aux, meta, psum, pcnt, pgrad, tape, fd_mse;
 };
 
 // hipRTC module of the last JIT-evaluated population.

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <numeric>
 
+#include "sr_adjoint_prog.h"
 #include "sr_ctx.h"
 #include "sr_kernels.h"
 #include "sr_launch.h"
@@ -482,8 +483,8 @@ int sr_compile_expr(const int32_t *tokens, const float *values, int32_t len, int
 }
 
 int sr_compile_param_expr(const int32_t *tokens, const float *values, const int32_t *const_slot,
-                          int32_t len, int32_t nv, int32_t *out_code, int32_t *out_arg,
-                          int32_t *stack_need)
+                          int32_t len, int32_t nv, int32_t *out_code, int32_t *out_arg, int32_t *out_aux,
+                          int32_t *stack_need, int32_t *tape_rows)
 {
     if (!tokens || !values || !const_slot || !out_code || !out_arg || !stack_need) return SR_ERR_INVALID_ARG;
     if (len <= 0 || len > MAX_EXPR_LEN || raw_stack_depth(tokens, len) < 0) return SR_ERR_BAD_EXPR;
@@ -495,9 +496,14 @@ int sr_compile_param_expr(const int32_t *tokens, const float *values, const int3
                          leaf.data());
     if (n < 0) return SR_ERR_BAD_EXPR;
     if (param_postpass(ins.data(), n, leaf.data(), const_slot, pstack.data()) != 0) return SR_ERR_INVALID_ARG;
+    std::vector<int32_t> aux(n);
+    const int rows = adjoint_postpass(ins.data(), n, aux.data());
+    if (rows < 0) return SR_ERR_CAPACITY;
+    if (tape_rows) *tape_rows = rows;
     for (int i = 0; i < n; ++i) {
         out_code[i] = ins[i].code;
         out_arg[i] = ins[i].arg;
+        if (out_aux) out_aux[i] = aux[i];
     }
     *stack_need = sn;
     return n;

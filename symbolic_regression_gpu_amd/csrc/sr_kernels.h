@@ -51,6 +51,49 @@ struct BatchArgs {
 // Lane l of a wave evaluates datapoints tile + d*64 + l for d < DPT, so every
 // X / y / cache access is a coalesced 256-byte wave load.
 // ---------------------------------------------------------------------------
+// One instruction of the accumulator program on every datapoint a lane owns.
+// cv is the value of a constant operand (the literal's bits, or a free
+// constant of a parametric program); ref(l...) loads OP_REF operands into
+// l.o (only reached when REF).  Stack traffic sits in top-level ifs of its
+// own (a stack write nested deeper in the decode makes the compiler copy whole
+// stack arrays between branches); constants and variables are fetched
+// branch-free.
+template <bool REF, typename RefLoad, typename... L>
+SR_DEV void interp_step(int code, int arg, float cv, int &sp, RefLoad &&ref, L &...l)
+{
+    if (code & BC_PUSH) {
+        ((l.stk[sp] = l.top), ...);
+        ++sp;
+    }
+    const int src = bc_src(code);
+    if (src == SRC_POP) {
+        --sp;
+        ((l.o = l.stk[sp]), ...);
+    } else if (REF && src == SRC_REF) {
+        ref(l...);
+    } else {
+        const bool is_var = src == SRC_VAR;
+        const int xi = is_var ? arg : 0;
+        for_each_lane([&](int, auto &ll) { ll.o = is_var ? ll.x[xi] : cv; }, l...);
+    }
+    const int cls = bc_cls(code);
+    if (cls != CL_BIN) {
+        if (src != SRC_NONE) ((l.top = l.o), ...);
+        if (cls == CL_UN) apply_unary(bc_op(code), l...);
+    } else {
+        if (code & BC_SWAP) {
+            for_each_lane(
+                [&](int, auto &ll) {
+                    float t = ll.top;
+                    ll.top = ll.o;
+                    ll.o = t;
+                },
+                l...);
+        }
+        apply_binary(bc_op(code), l...);
+    }
+}
+
 // Program words staged per wave in LDS: the longest program plus one word
 // read ahead past its end.
 constexpr int kProgLds = MAX_EXPR_LEN + 1;
@@ -119,50 +162,18 @@ __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
                 int sp = 0;
                 int code = __builtin_amdgcn_readfirstlane(lp[0].code);
                 int arg = __builtin_amdgcn_readfirstlane(lp[0].arg);
+                auto ref = [&](auto &...ll) {
+                    const float *colr = a.cache + (int64_t)arg * N;
+                    for_each_lane(
+                        [&](int d, auto &l) {
+                            int64_t dp = t0 + d * kWave + lane;
+                            l.o = colr[dp < N ? dp : N - 1];
+                        },
+                        ll...);
+                };
                 for (int i = 0; i < n; ++i) {
                     const Ins nx = lp[i + 1];
-                    // Stack traffic sits in top-level ifs of its own (a stack
-                    // write nested deeper in the decode makes the compiler copy
-                    // whole stack arrays between branches); constants and
-                    // variables are fetched branch-free.
-                    if (code & BC_PUSH) {
-                        ((L.stk[sp] = L.top), ...);
-                        ++sp;
-                    }
-                    const int src = bc_src(code);
-                    if (src == SRC_POP) {
-                        --sp;
-                        ((L.o = L.stk[sp]), ...);
-                    } else if (MODE == MODE_MSE_REF && src == SRC_REF) {
-                        const float *colr = a.cache + (int64_t)arg * N;
-                        for_each_lane(
-                            [&](int d, auto &l) {
-                                int64_t dp = t0 + d * kWave + lane;
-                                l.o = colr[dp < N ? dp : N - 1];
-                            },
-                            L...);
-                    } else {
-                        const bool is_var = src == SRC_VAR;
-                        const int xi = is_var ? arg : 0;
-                        const float cv = __int_as_float(arg);
-                        for_each_lane([&](int, auto &l) { l.o = is_var ? l.x[xi] : cv; }, L...);
-                    }
-                    const int cls = bc_cls(code);
-                    if (cls != CL_BIN) {
-                        if (src != SRC_NONE) ((L.top = L.o), ...);
-                        if (cls == CL_UN) apply_unary(bc_op(code), L...);
-                    } else {
-                        if (code & BC_SWAP) {
-                            for_each_lane(
-                                [&](int, auto &l) {
-                                    float t = l.top;
-                                    l.top = l.o;
-                                    l.o = t;
-                                },
-                                L...);
-                        }
-                        apply_binary(bc_op(code), L...);
-                    }
+                    interp_step<MODE == MODE_MSE_REF>(code, arg, __int_as_float(arg), sp, ref, L...);
                     code = __builtin_amdgcn_readfirstlane(nx.code);
                     arg = __builtin_amdgcn_readfirstlane(nx.arg);
                 }

@@ -9,6 +9,7 @@
 #include <numeric>
 
 #include "sr_adjoint.h"
+#include "sr_adjoint_prog.h"
 #include "sr_ctx.h"
 #include "sr_launch.h"
 #include "sr_param_launch.h"
@@ -25,8 +26,9 @@ struct ParamPlan {
 static int param_plan(Ctx &c, bool adjoint, ParamPlan &pp)
 {
     pp.xc = pick_xc(c.nv);
-    pp.dpt = adjoint ? 1 : std::min(c.tune.dpt, pp.xc == 32 ? 1 : 4);
-    if (pp.dpt == 8) pp.dpt = 4;
+    const int max_dpt = adjoint ? (pp.xc == 32 ? 1 : 2) : (pp.xc == 32 ? 1 : 4);
+    pp.dpt = std::max(1, std::min(c.tune.dpt, max_dpt));
+    if (pp.dpt == 3) pp.dpt = 2;
     pp.nq = c.tune.n_queues > 1 ? c.tune.n_queues : 1;
     const int64_t tile = 64 * pp.dpt;
     pp.slice = rup((c.N + pp.nq - 1) / pp.nq, 512);
@@ -34,7 +36,7 @@ static int param_plan(Ctx &c, bool adjoint, ParamPlan &pp)
     if (pp.chunk > pp.slice) pp.chunk = pp.slice;
     pp.cpq = (int)((pp.slice + pp.chunk - 1) / pp.chunk);
     pp.P = pp.nq * pp.cpq;
-    int occ = adjoint ? adjoint_occupancy(pp.xc) : param_fwd_occupancy(pp.dpt, pp.xc);
+    int occ = adjoint ? adjoint_occupancy(pp.dpt, pp.xc) : param_fwd_occupancy(pp.dpt, pp.xc);
     if (occ <= 0) return c.fail(SR_ERR_UNSUPPORTED, "no constant-fitting kernel for this variable count");
     pp.grid = (int)rup((int64_t)c.num_cu * occ, pp.nq);
     return SR_OK;
@@ -52,6 +54,7 @@ static ParamArgs base_args(Ctx &c, const ParamPlan &pp, const float *cvals)
     ParamState &s = c.par;
     ParamArgs a{};
     a.progs = s.progs.as<Ins>();
+    a.aux = s.aux.as<int32_t>();
     a.prog_off = s.meta.as<int32_t>();
     a.prog_len = a.prog_off + s.E;
     a.order = a.prog_off + 2 * s.E;
@@ -117,7 +120,9 @@ int sr_param_prepare(sr_ctx *x, const sr_population *pop, const int32_t *const_s
     if (rc) return rc;
     ParamState &s = c.par;
     std::vector<Ins> ins;
+    std::vector<int32_t> aux_all, aux_tmp(MAX_EXPR_LEN);
     std::vector<int32_t> off(E), len(E), nslots(E), order(E);
+    int tape_rows = 1;
     std::vector<float> vals, cv((size_t)E * MAX_SLOTS, 0.0f);
     std::vector<int> size(MAX_EXPR_LEN), need(MAX_EXPR_LEN), leaf(MAX_EXPR_LEN), pstack(MAX_EXPR_LEN);
     std::vector<CompileFrame> frames(MAX_EXPR_LEN + 1);
@@ -147,6 +152,10 @@ int sr_param_prepare(sr_ctx *x, const sr_population *pop, const int32_t *const_s
         if (n < 0 || sn > kStackSlots) return c.fail(SR_ERR_BAD_EXPR, "expression " + std::to_string(e));
         if (param_postpass(tmp.data(), n, leaf.data(), slot, pstack.data()) != 0)
             return c.fail(SR_ERR_INVALID_ARG, "bad free-constant slot");
+        const int rows = adjoint_postpass(tmp.data(), n, aux_tmp.data());
+        if (rows < 0) return c.fail(SR_ERR_CAPACITY, "expression " + std::to_string(e) + ": adjoint tape too long");
+        tape_rows = std::max(tape_rows, rows);
+        aux_all.insert(aux_all.end(), aux_tmp.begin(), aux_tmp.begin() + n);
         off[e] = (int32_t)ins.size();
         len[e] = n;
         nslots[e] = used;
@@ -157,6 +166,7 @@ int sr_param_prepare(sr_ctx *x, const sr_population *pop, const int32_t *const_s
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return len[a] > len[b]; });
     hipError_t he;
     if ((he = s.progs.ensure(ins.size() * sizeof(Ins))) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    if ((he = s.aux.ensure(aux_all.size() * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
     if ((he = s.meta.ensure((size_t)E * 16)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
     std::vector<int32_t> meta((size_t)E * 4);
     std::memcpy(meta.data(), off.data(), E * 4);
@@ -165,10 +175,13 @@ int sr_param_prepare(sr_ctx *x, const sr_population *pop, const int32_t *const_s
     std::memcpy(meta.data() + 3 * E, nslots.data(), E * 4);
     he = hipMemcpyAsync(s.progs.p, ins.data(), ins.size() * sizeof(Ins), hipMemcpyHostToDevice, c.stream);
     if (he == hipSuccess) he = hipMemcpyAsync(s.meta.p, meta.data(), meta.size() * 4, hipMemcpyHostToDevice, c.stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(s.aux.p, aux_all.data(), aux_all.size() * 4, hipMemcpyHostToDevice, c.stream);
     if (he == hipSuccess) he = hipStreamSynchronize(c.stream);
     if (he != hipSuccess) return c.hip_fail(he, "H2D parametric programs");
     s.E = E;
     s.max_len = max_len;
+    s.tape_rows = tape_rows;
     s.nslots = nslots;
     s.tokens = check.tokens;
     s.ready = true;
@@ -223,7 +236,7 @@ int sr_param_grad(sr_ctx *x, const float *d_cvals, float *d_mse, float *d_grad)
     if ((he = s.psum.ensure((size_t)s.E * pp.P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
     if ((he = s.pcnt.ensure((size_t)s.E * pp.P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
     if ((he = s.pgrad.ensure((size_t)s.E * pp.P * MAX_SLOTS * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
-    if ((he = s.tape.ensure(waves * (size_t)s.max_len * kWave * 4)) != hipSuccess)
+    if ((he = s.tape.ensure(waves * (size_t)s.tape_rows * kWave * pp.dpt * 4)) != hipSuccess)
         return c.hip_fail(he, "hipMalloc tape");
     if ((he = c.d_counters.ensure(256)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
     ParamArgs a = base_args(c, pp, d_cvals);
@@ -231,10 +244,10 @@ int sr_param_grad(sr_ctx *x, const float *d_cvals, float *d_mse, float *d_grad)
     a.pcnt = s.pcnt.as<long long>();
     a.pgrad = s.pgrad.as<double>();
     a.tape = s.tape.as<float>();
-    a.tape_stride = s.max_len;
+    a.tape_stride = s.tape_rows;
     hipStream_t st = c.stream;
     if ((he = launch_zero_u32(a.counters, pp.nq, st)) != hipSuccess) return c.hip_fail(he, "zero");
-    if ((he = launch_adjoint(pp.xc, a, pp.grid, st)) != hipSuccess) return c.hip_fail(he, "adjoint");
+    if ((he = launch_adjoint(pp.dpt, pp.xc, a, pp.grid, st)) != hipSuccess) return c.hip_fail(he, "adjoint");
     if ((he = launch_finalize_grad(a.psum, a.pcnt, a.pgrad, pp.P, s.E, d_mse, d_grad, st)) != hipSuccess)
         return c.hip_fail(he, "finalize grad");
     return SR_OK;

@@ -98,77 +98,97 @@ def d_binary(op, a, b, y):
     raise ValueError(op)
 
 
+BC_ACTIVE = 1 << 15
+BC_TAPE_SHIFT = 20
+CL_LOAD, CL_BIN, CL_UN = 0, 1, 2
+
+
 def mse_and_grad(tok, val, slot, X, nv, cvals):
     """(mse, grad[8]) of one expression over X [nv+1, N] (y last) the way the
-    adjoint kernel computes them: fp32 values and adjoints, per-point
+    adjoint kernel computes them: forward pass taping the rows the program
+    asks for, reverse pass over the active instructions only (jumping over
+    subtrees without a free constant), fp32 values and adjoints, per-point
     gradients summed in fp64 here (the kernel sums in fp32 per lane)."""
-    code, arg, _ = compile_expr(tok, val, nv, slot)
+    code, arg, _, aux, rows = compile_expr(tok, val, nv, slot)
     N = X.shape[1]
     xs = [X[k].astype(f32) for k in range(nv)]
 
-    def src_val(c, a):
-        s = (c >> 8) & 7
-        if s == SRC_CONST:
-            k = ((c >> 16) & 15) - 1
-            v = cvals[k] if k >= 0 else np.array([a], np.int32).view(f32)[0]
-            return np.full(N, v, f32)
-        if s == SRC_VAR:
-            return xs[a] if a < nv else np.zeros(N, f32)
-        return None
+    def cval(c, a):
+        k = ((c >> 16) & 15) - 1
+        return cvals[k] if k >= 0 else np.array([a], np.int32).view(f32)[0]
 
     n = len(code)
-    tape, stk = [None] * n, []
+    tape = np.zeros((max(rows, 1), N), f32)
+    stk = []
     top = np.zeros(N, f32)
     for i in range(n):
         c, a = int(code[i]), int(arg[i])
-        s, op = (c >> 8) & 7, c & 0xFF
+        s, op, cls = (c >> 8) & 7, c & 0xFF, (c >> 13) & 3
         if c & BC_PUSH:
             stk.append(top)
-        o = src_val(c, a) if s in (SRC_CONST, SRC_VAR) else (stk.pop() if s == SRC_POP else None)
-        if op == 0:
+        if s == SRC_POP:
+            o = stk.pop()
+        elif s == SRC_VAR:
+            o = xs[a] if a < nv else np.zeros(N, f32)
+        else:
+            o = np.full(N, cval(c, a), f32)
+        if cls == CL_LOAD:
             top = o
-        elif op < 10:
+        elif cls == CL_BIN:
             A, B = (o, top) if c & BC_SWAP else (top, o)
             top = binary(op, A, B)
         else:
             top = unary(op, o if s != SRC_NONE else top)
-        tape[i] = top
+        r = ((c >> BC_TAPE_SHIFT) & 0x7FF) - 1
+        if r >= 0:
+            tape[r] = top
     y = X[nv].astype(f32)
     ok = np.isfinite(top)
     diff = (top - y).astype(f32)
     at = np.where(ok, diff, f32(0)).astype(f32)
-    out_i, astk = top, []
+    astk = []
     tg = np.zeros((8, N), f32)
-    for i in range(n - 1, -1, -1):
-        c, a = int(code[i]), int(arg[i])
-        s, op = (c >> 8) & 7, c & 0xFF
-        x_in = tape[i - 1] if i > 0 else np.zeros(N, f32)
-        ov = src_val(c, a) if s in (SRC_CONST, SRC_VAR) else (tape[a] if s == SRC_POP else None)
+    i = n - 1
+    while i >= 0:
+        c, a, x = int(code[i]), int(arg[i]), int(aux[i])
+        if not c & BC_ACTIVE:
+            at = astk.pop() if code[x] & BC_PUSH else np.zeros(N, f32)
+            i = x - 1
+            continue
+        s, op, cls = (c >> 8) & 7, c & 0xFF, (c >> 13) & 3
+        yo = tape[((c >> BC_TAPE_SHIFT) & 0x7FF) - 1]
+        x_in = tape[x] if x >= 0 else np.zeros(N, f32)
+        if s == SRC_POP:
+            ov = tape[a]
+        elif s == SRC_VAR:
+            ov = xs[a] if a < nv else np.zeros(N, f32)
+        else:
+            ov = np.full(N, cval(c, a), f32)
         a_o = np.zeros(N, f32)
         a_in = np.zeros(N, f32)
-        if op == 0:
+        if cls == CL_LOAD:
             a_o = at
-        elif op < 10:
+        elif cls == CL_BIN:
             sw = bool(c & BC_SWAP)
             A, B = (ov, x_in) if sw else (x_in, ov)
-            dA, dB = d_binary(op, A, B, out_i)
+            dA, dB = d_binary(op, A, B, yo)
             gA, gB = (at * dA).astype(f32), (at * dB).astype(f32)
             a_o, a_in = (gA, gB) if sw else (gB, gA)
         else:
-            g = (at * d_unary(op, ov if s != SRC_NONE else x_in, out_i)).astype(f32)
+            g = (at * d_unary(op, ov if s != SRC_NONE else x_in, yo)).astype(f32)
             if s != SRC_NONE:
                 a_o = g
             else:
                 a_in = g
-        if s == SRC_CONST:
-            k = ((c >> 16) & 15) - 1
-            if k >= 0:
-                tg[k] += a_o
+        k = ((c >> 16) & 15) - 1
+        if s == SRC_CONST and k >= 0:
+            tg[k] += a_o
         elif s == SRC_POP:
             astk.append(a_o)
         if c & BC_PUSH:
             a_in = astk.pop()
-        at, out_i = a_in, x_in
+        at = a_in
+        i -= 1
     cnt = int(ok.sum())
     if cnt == 0:
         return float("nan"), np.full(8, np.nan)
