@@ -10,11 +10,12 @@ namespace sr {
 struct BatchArgs;
 
 constexpr int kStackSlots = 8;  // bytecode stack need <= 8 for any tree of <= 512 leaves
-// Registers allotted to each datapoint's stack.  More than 8 on purpose: the
-// backend reads / writes a VGPR array of <= 8 floats at a uniform dynamic index
-// through a compare + v_cndmask chain over every element (16 VALU per access
-// for 8 slots), and larger arrays through GPR-index mode (one v_mov).
-constexpr int kStackRegs = 16;
+// Registers allotted to each datapoint's stack.  One more than the 8 slots a
+// program may use, on purpose: the backend reads / writes a VGPR array of <= 8
+// floats at a uniform dynamic index through a compare + v_cndmask chain over
+// every element (16 VALU per access for 8 slots), and larger arrays through
+// GPR-index mode (one v_mov).
+constexpr int kStackRegs = 9;
 
 // Batched persistent interpreter.  mode: 0 MSE partials, 1 predictions,
 // 2 cache columns.  Returns hipErrorInvalidValue for an unsupported (dpt, xc).
