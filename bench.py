@@ -177,6 +177,7 @@ def constant_fit_bench(ctx, seed, adj_steps, fd_steps, E=4096, N=262144, nv=5):
     pop = Population.from_exprs([(t, v) for (t, _), v in zip(exprs, vals)])
     fit = ConstantFit(ctx, pop, np.concatenate(slot_rows))
     consts = fit.initial().requires_grad_()
+    fwd_dpt, adj_dpt = ctx.param_autotune(consts.detach().data_ptr())
     opt = torch.optim.Adam([consts], lr=1e-2)
 
     def adam_step():
@@ -186,7 +187,7 @@ def constant_fit_bench(ctx, seed, adj_steps, fd_steps, E=4096, N=262144, nv=5):
         opt.step()
         return m
 
-    m0 = adam_step()  # warm-up
+    m0 = adam_step().detach()  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(adj_steps):
@@ -200,13 +201,15 @@ def constant_fit_bench(ctx, seed, adj_steps, fd_steps, E=4096, N=262144, nv=5):
         fit.fd_grad(consts, 1e-3)
     torch.cuda.synchronize()
     t_fd = (time.perf_counter() - t0) / fd_steps
+    m = m.detach()
     fin = torch.isfinite(m0) & torch.isfinite(m)
     return {"workload": f"{E} expressions of 50-200 tokens ({fit.tokens} tokens), 2-8 free "
                         f"constants, {nv} vars, {N} datapoints; Adam lr 1e-2",
             "adjoint_grad_evals_per_s": E / t_adj, "fd_grad_evals_per_s": E / t_fd,
             "adjoint_over_fd": t_fd / t_adj, "ms_per_adam_step": 1e3 * t_adj,
             "ms_per_fd_gradient": 1e3 * t_fd, "adam_steps_timed": adj_steps,
-            "median_mse_first_last": [float(m0[fin].median()), float(m[fin].median())]}
+            "median_mse_first_last": [float(m0[fin].median()), float(m[fin].median())],
+            "tuning": {"forward_dpt": fwd_dpt, "adjoint_dpt": adj_dpt}}
 
 
 def copy_calibration(mib):
@@ -254,6 +257,8 @@ def main():
     X, nv, (tok, val, ln, cap) = evolution_setup(ctx, N, args.seed)
     pop0 = to_population(tok, val, ln, cap)
     flops = operator_nodes(pop0) * N
+    log("[bench] autotuning the batched launch shape")
+    tuned, tune_ms = ctx.autotune(pop0, max_sample=256)
     bytes_alg = (nv + 1) * 4 * N
 
     # roofline + plain vs batched on the initial population
@@ -333,6 +338,8 @@ def main():
             "node_evals_per_generation": gen_node_evals // max(args.steps, 1),
             "best_mse_last_generation": best,
             "device_ms_per_generation": ms_dev / max(args.steps, 1),
+            "tuning": {"dpt": tuned.dpt, "chunk": tuned.chunk, "n_queues": tuned.n_queues,
+                       "blocks_per_cu": tuned.blocks_per_cu},
         },
         "roofline": {
             "bound": "valu",

@@ -188,6 +188,18 @@ int sr_jit_compile(const sr_population *pop, int32_t nv, int32_t dpt, char *log_
 /* The compiled code object itself (out may be NULL to query *len). */
 int sr_jit_code(const sr_population *pop, int32_t nv, int32_t dpt, void *out, int64_t cap, int64_t *len);
 
+/* Autotuner (no counterpart in the reference): times every valid launch
+ * shape of the batched interpreter (datapoints per lane, chunk, XCD queues,
+ * workgroups per CU) on a length-stratified sample of at most max_sample
+ * expressions of pop (0: 256), sets the fastest as the context tuning and
+ * remembers it per (device, variable-register class, N).  best_ms: kernel ms
+ * of the winner on the sample (0 when reused). */
+int sr_autotune(sr_ctx *ctx, const sr_population *pop, int32_t max_sample, sr_tuning *best, double *best_ms);
+/* Same for the constant-fitting kernels of the prepared population: picks
+ * datapoints per lane for the forward (finite differences) and adjoint
+ * kernels at the given constants (device pointer [E*8]). */
+int sr_param_autotune(sr_ctx *ctx, const float *d_cvals, int32_t *fwd_dpt, int32_t *adj_dpt);
+
 int sr_get_tuning(sr_ctx *ctx, sr_tuning *out);
 int sr_set_tuning(sr_ctx *ctx, const sr_tuning *t);
 

@@ -104,6 +104,8 @@ _SIGNATURES = {
     "sr_eval_mse": [_vp, C.POINTER(_Pop), C.c_int, _f32p, _i64p, C.POINTER(_Stats)],
     "sr_eval_predictions": [_vp, C.POINTER(_Pop), _f32p],
     "sr_get_tuning": [_vp, C.POINTER(_Tuning)],
+    "sr_autotune": [_vp, C.POINTER(_Pop), C.c_int32, C.POINTER(_Tuning), C.POINTER(C.c_double)],
+    "sr_param_autotune": [_vp, _vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "sr_set_tuning": [_vp, C.POINTER(_Tuning)],
     "sr_compile_expr": [_i32p, _f32p, C.c_int32, C.c_int32, _i32p, _i32p, C.POINTER(C.c_int32)],
     "sr_compile_param_expr": [_i32p, _f32p, _i32p, C.c_int32, C.c_int32, _i32p, _i32p, _vp,
@@ -371,6 +373,20 @@ class Context:
         self._check(self._L.sr_set_tuning(self._h, C.byref(t)))
 
     # ---- subtree cache ----------------------------------------------------
+    def autotune(self, pop: Population, max_sample: int = 256):
+        """Time the launch shapes of the batched interpreter on a sample of pop;
+        keeps the fastest as this context's tuning.  Returns (Tuning, ms)."""
+        t = _Tuning()
+        ms = C.c_double()
+        self._check(self._L.sr_autotune(self._h, C.byref(pop._c()), max_sample, C.byref(t), C.byref(ms)))
+        return Tuning(t.dpt, t.chunk, t.n_queues, t.blocks_per_cu), ms.value
+
+    def param_autotune(self, d_cvals: int):
+        """Datapoints per lane for the forward and adjoint constant-fitting kernels."""
+        f, a = C.c_int32(), C.c_int32()
+        self._check(self._L.sr_param_autotune(self._h, d_cvals, C.byref(f), C.byref(a)))
+        return f.value, a.value
+
     def cache_config(self, capacity=2048, min_size=3, min_freq=10):
         self._check(self._L.sr_cache_config(self._h, capacity, min_size, min_freq))
 

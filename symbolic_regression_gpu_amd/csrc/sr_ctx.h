@@ -128,6 +128,8 @@ struct ParamState {
     bool ready = false;
     int E = 0, max_len = 0;
     int tape_rows = 1;  // most tape rows any program needs
+    int fwd_dpt = 0;    // datapoints per lane of the forward / adjoint kernels
+    int adj_dpt = 0;    // (0: from the context tuning; set by sr_param_autotune)
     int64_t tokens = 0;
     std::vector<int32_t> nslots;
     DevBuf progs, aux, meta, psum, pcnt, pgrad, tape, fd_mse;

@@ -23,11 +23,13 @@ struct ParamPlan {
     int64_t slice, chunk;
 };
 
-static int param_plan(Ctx &c, bool adjoint, ParamPlan &pp)
+static int param_plan(Ctx &c, bool adjoint, ParamPlan &pp, int force_dpt = 0)
 {
     pp.xc = pick_xc(c.nv);
     const int max_dpt = adjoint ? (pp.xc == 32 ? 1 : 2) : (pp.xc == 32 ? 1 : 4);
-    pp.dpt = std::max(1, std::min(c.tune.dpt, max_dpt));
+    const int want = force_dpt ? force_dpt : (adjoint ? c.par.adj_dpt : c.par.fwd_dpt);
+    if (force_dpt && force_dpt > max_dpt) return SR_ERR_UNSUPPORTED;
+    pp.dpt = std::max(1, std::min(want ? want : c.tune.dpt, max_dpt));
     if (pp.dpt == 3) pp.dpt = 2;
     pp.nq = c.tune.n_queues > 1 ? c.tune.n_queues : 1;
     const int64_t tile = 64 * pp.dpt;
@@ -97,6 +99,69 @@ static int param_forward(Ctx &c, const float *cvals, int replicas, float h, floa
         return c.hip_fail(he, "parametric forward");
     if ((he = launch_finalize(a.psum, a.pcnt, pp.P, (int)rows, mse_out, nullptr, st)) != hipSuccess)
         return c.hip_fail(he, "finalize");
+    return SR_OK;
+}
+
+static int param_adjoint(Ctx &c, const float *d_cvals, float *d_mse, float *d_grad, int force_dpt)
+{
+    ParamState &s = c.par;
+    ParamPlan pp;
+    int rc = param_plan(c, true, pp, force_dpt);
+    if (rc) return rc;
+    hipError_t he;
+    const size_t waves = (size_t)pp.grid * (kBlock / kWave);
+    if ((he = s.psum.ensure((size_t)s.E * pp.P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    if ((he = s.pcnt.ensure((size_t)s.E * pp.P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    if ((he = s.pgrad.ensure((size_t)s.E * pp.P * MAX_SLOTS * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    if ((he = s.tape.ensure(waves * (size_t)s.tape_rows * kWave * pp.dpt * 4)) != hipSuccess)
+        return c.hip_fail(he, "hipMalloc tape");
+    if ((he = c.d_counters.ensure(256)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    ParamArgs a = base_args(c, pp, d_cvals);
+    a.psum = s.psum.as<double>();
+    a.pcnt = s.pcnt.as<long long>();
+    a.pgrad = s.pgrad.as<double>();
+    a.tape = s.tape.as<float>();
+    a.tape_stride = s.tape_rows;
+    hipStream_t st = c.stream;
+    if ((he = launch_zero_u32(a.counters, pp.nq, st)) != hipSuccess) return c.hip_fail(he, "zero");
+    if ((he = launch_adjoint(pp.dpt, pp.xc, a, pp.grid, st)) != hipSuccess) return c.hip_fail(he, "adjoint");
+    if ((he = launch_finalize_grad(a.psum, a.pcnt, a.pgrad, pp.P, s.E, d_mse, d_grad, st)) != hipSuccess)
+        return c.hip_fail(he, "finalize grad");
+    return SR_OK;
+}
+
+// Time one adjoint or one forward pass (replicas = 1) of the prepared
+// population at a given datapoints-per-lane (the autotuner's probe).
+int sr_param_time(Ctx &c, bool adjoint, int dpt, const float *d_cvals, float *ms)
+{
+    int rc = param_ready(c);
+    if (rc) return rc;
+    ParamState &s = c.par;
+    hipError_t he;
+    DevBuf tmp;
+    if ((he = tmp.ensure((size_t)s.E * (MAX_SLOTS + 1) * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    float *mse = tmp.as<float>(), *grad = mse + s.E;
+    const int saved_f = s.fwd_dpt, saved_a = s.adj_dpt;
+    for (int rep = 0; rep < 2; ++rep) {
+        hipEventRecord(c.ev[4], c.stream);
+        if (adjoint) {
+            rc = param_adjoint(c, d_cvals, mse, grad, dpt);
+        } else {
+            ParamPlan probe;
+            rc = param_plan(c, false, probe, dpt);
+            if (!rc) {
+                s.fwd_dpt = dpt;
+                rc = param_forward(c, d_cvals, 1, 0.0f, mse);
+                s.fwd_dpt = saved_f;
+            }
+        }
+        hipEventRecord(c.ev[5], c.stream);
+        if (rc) break;
+    }
+    s.adj_dpt = saved_a;
+    if (rc) return rc;
+    if ((he = hipEventSynchronize(c.ev[5])) != hipSuccess) return c.hip_fail(he, "param timing");
+    hipEventElapsedTime(ms, c.ev[4], c.ev[5]);
     return SR_OK;
 }
 
@@ -227,30 +292,7 @@ int sr_param_grad(sr_ctx *x, const float *d_cvals, float *d_mse, float *d_grad)
     hipSetDevice(c.device);
     int rc = param_ready(c);
     if (rc) return rc;
-    ParamState &s = c.par;
-    ParamPlan pp;
-    rc = param_plan(c, true, pp);
-    if (rc) return rc;
-    hipError_t he;
-    const size_t waves = (size_t)pp.grid * (kBlock / kWave);
-    if ((he = s.psum.ensure((size_t)s.E * pp.P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
-    if ((he = s.pcnt.ensure((size_t)s.E * pp.P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
-    if ((he = s.pgrad.ensure((size_t)s.E * pp.P * MAX_SLOTS * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
-    if ((he = s.tape.ensure(waves * (size_t)s.tape_rows * kWave * pp.dpt * 4)) != hipSuccess)
-        return c.hip_fail(he, "hipMalloc tape");
-    if ((he = c.d_counters.ensure(256)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
-    ParamArgs a = base_args(c, pp, d_cvals);
-    a.psum = s.psum.as<double>();
-    a.pcnt = s.pcnt.as<long long>();
-    a.pgrad = s.pgrad.as<double>();
-    a.tape = s.tape.as<float>();
-    a.tape_stride = s.tape_rows;
-    hipStream_t st = c.stream;
-    if ((he = launch_zero_u32(a.counters, pp.nq, st)) != hipSuccess) return c.hip_fail(he, "zero");
-    if ((he = launch_adjoint(pp.dpt, pp.xc, a, pp.grid, st)) != hipSuccess) return c.hip_fail(he, "adjoint");
-    if ((he = launch_finalize_grad(a.psum, a.pcnt, a.pgrad, pp.P, s.E, d_mse, d_grad, st)) != hipSuccess)
-        return c.hip_fail(he, "finalize grad");
-    return SR_OK;
+    return param_adjoint(c, d_cvals, d_mse, d_grad, 0);
 }
 
 int sr_param_info(sr_ctx *x, int32_t *num_exprs, int32_t *nslots_out, int64_t *tokens)

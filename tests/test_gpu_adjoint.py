@@ -144,3 +144,28 @@ def test_autograd_shim_and_adam(ctx):
           f"median MSE {first[ok].median().item():.4g} -> {last[ok].median().item():.4g}")
     assert improved >= 0.8
     assert torch.isfinite(consts).all()
+
+
+def test_param_autotune_keeps_gradients(ctx):
+    nv, N = 4, 8192
+    pop, slot = c5_population(32, 11, nv, min_len=20, max_tokens=80)
+    X = uniform_data(nv, N, 12)
+    X[nv] = np.sin(X[0]) + X[1]
+    ctx.set_data(X)
+    fit = ConstantFit(ctx, to_api(pop), slot)
+    c0 = fit.initial()
+    m1, g1 = fit.adjoint_grad(c0)
+    f1 = fit.mse(c0.detach())
+    fd, ad = ctx.param_autotune(c0.data_ptr())
+    assert fd in (1, 2, 4) and ad in (1, 2)
+    m2, g2 = fit.adjoint_grad(c0)
+    f2 = fit.mse(c0.detach())
+    torch.cuda.synchronize()
+    # only the order of partial sums can move with the launch shape
+    fin = torch.isfinite(m1)
+    assert torch.equal(fin, torch.isfinite(m2))
+    assert torch.allclose(m1[fin], m2[fin], rtol=1e-6) and torch.allclose(f1[fin], f2[fin], rtol=1e-6)
+    a, b = torch.nan_to_num(g1[fin]).double(), torch.nan_to_num(g2[fin]).double()
+    scale = a.abs().amax(dim=1, keepdim=True)
+    assert ((a - b).abs() <= 1e-3 * a.abs() + 1e-5 * scale).all()
+    print(f"param autotune: forward dpt {fd}, adjoint dpt {ad}")

@@ -143,3 +143,25 @@ def test_malformed_population_is_an_error(ctx):
         with pytest.raises(SrError) as ei:
             ctx.eval_mse(bad, mode)
         assert ei.value.code == -3
+
+
+def test_autotune_picks_a_valid_shape_and_keeps_results(ctx):
+    pop = random_population(200, 17, nv=3, max_tokens=80, max_depth=7,
+                            unary=GRAMMAR_UNARY, binary=GRAMMAR_BINARY)
+    N = 65_537
+    X = uniform_data(3, N, 19)
+    ctx.set_data(X)
+    api = to_api(pop)
+    saved = ctx.tuning
+    try:
+        base, vbase, _ = ctx.eval_mse(api, EVAL_BATCHED)
+        best, ms = ctx.autotune(api, max_sample=64)
+        assert best.dpt in (1, 2, 4) and best.n_queues in (1, 8) and best.chunk % (64 * best.dpt) == 0
+        assert ms > 0 and ctx.tuning == best
+        m, v, _ = ctx.eval_mse(api, EVAL_BATCHED)
+        assert np.array_equal(v, vbase) and mse_close(m, base, 1e-12).all()
+        again, ms2 = ctx.autotune(api, max_sample=64)   # remembered per shape
+        assert again == best and ms2 == 0
+        print(f"autotune: {best} ({ms:.3f} ms on the sample)")
+    finally:
+        ctx.tuning = saved
