@@ -237,6 +237,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fit", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the timed batched evaluations of the initial population (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -264,6 +266,11 @@ def main():
     # roofline + plain vs batched on the initial population
     log("[bench] batched evaluation")
     batched = time_evals(ctx, pop0, EVAL_BATCHED, args.reps)
+    if args.roofline_only:
+        print(json.dumps({"kernel_ms": batched["interp_kernel_ms"], "flops_per_launch": flops,
+                          "launches": args.reps + 1, "tuning": [tuned.dpt, tuned.chunk, tuned.n_queues,
+                                                                tuned.blocks_per_cu]}), flush=True)
+        return
     log("[bench] plain one-launch-per-expression evaluation")
     plain = time_evals(ctx, pop0, EVAL_PLAIN, max(2, args.reps // 2))
     log("[bench] subtree-cached evaluation")

@@ -53,8 +53,8 @@ __global__ __launch_bounds__(kBlock) void k_param_fwd(ParamArgs a)
             if (item >= items) break;
             const int rep = item / per_rep;
             const int rem = item - rep * per_rep;
-            const int rank = rem / a.chunks_per_queue;
-            const int chunk = rem - rank * a.chunks_per_queue;
+            const int chunk = rem / a.n_exprs;  // chunk-major (X stays in L2)
+            const int rank = rem - chunk * a.n_exprs;
             const int e = a.order[rank];
             const int pert = a.replicas > 1 ? rep >> 1 : -1;
             const float delta = a.replicas > 1 ? ((rep & 1) ? -a.fd_h : a.fd_h) : 0.0f;
@@ -231,8 +231,8 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(ParamArgs a)
             if (lane == 0) item = (int)atomicAdd(&a.counters[q], 1u);
             item = __builtin_amdgcn_readfirstlane(item);
             if (item >= items) break;
-            const int rank = item / a.chunks_per_queue;
-            const int chunk = item - rank * a.chunks_per_queue;
+            const int chunk = item / a.n_exprs;  // chunk-major (X stays in L2)
+            const int rank = item - chunk * a.n_exprs;
             const int e = a.order[rank];
             const int n = a.prog_len[e];
             stage_param_program(a, e, n, lane, -1, 0.0f, lp, la);

@@ -47,7 +47,8 @@ struct BatchArgs {
 
 // ---------------------------------------------------------------------------
 // Batched persistent interpreter: each wave pulls (expression, datapoint chunk)
-// work items from its XCD's queue; items are ordered longest expression first.
+// work items from its XCD's queue; items run chunk by chunk, longest
+// expression first within a chunk.
 // Lane l of a wave evaluates datapoints tile + d*64 + l for d < DPT, so every
 // X / y / cache access is a coalesced 256-byte wave load.
 // ---------------------------------------------------------------------------
@@ -119,8 +120,11 @@ __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
             if (lane == 0) item = (int)atomicAdd(&a.counters[q], 1u);
             item = __builtin_amdgcn_readfirstlane(item);
             if (item >= items) break;
-            const int rank = item / a.chunks_per_queue;
-            const int chunk = item - rank * a.chunks_per_queue;
+            // chunk-major: the waves of a queue work on one datapoint chunk
+            // at a time, so its X columns stay in the XCD's L2 across
+            // expressions; within a chunk, longest expression first.
+            const int chunk = item / a.n_exprs;
+            const int rank = item - chunk * a.n_exprs;
             const int e = a.order[rank];
             const Ins *prog =
                 a.progs + (a.prog_off ? (int64_t)a.prog_off[e] : (int64_t)e * a.prog_stride);
