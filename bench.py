@@ -251,7 +251,7 @@ def main():
 
     import torch
     from symbolic_regression_gpu_amd import EVAL_BATCHED, EVAL_CACHED, EVAL_PLAIN, Context
-    from symbolic_regression_gpu_amd.api import SelectParams
+    from symbolic_regression_gpu_amd.api import CACHE_GENERATIONAL, CACHE_REFERENCE, SelectParams
     torch.cuda.init()
     ctx = Context(0)
     N = args.N
@@ -307,6 +307,25 @@ def main():
     cached_gen = {"node_evals_per_s": st_cg["logical_node_evals"] / wall_c,
                   "ms_per_generation": 1e3 * wall_c / max(2, args.steps // 4),
                   "executed_over_logical": st_cg["executed_node_evals"] / max(st_cg["logical_node_evals"], 1)}
+
+    # generational cache: every distinct subtree of the current population
+    # keeps its column; a generation executes only the changed nodes
+    log("[bench] generational-cache generations")
+    ctx.cache_policy(CACHE_GENERATIONAL, 0, 4)
+    ctx.ga_set_population(tok, val, ln, cap)
+    ctx.ga_run(sel, EVAL_CACHED, 2, 0, args.seed, False)
+    n_gen = max(4, args.steps // 2)
+    t0 = time.perf_counter()
+    _, _, st_g = ctx.ga_run(sel, EVAL_CACHED, n_gen, 2, args.seed, False)
+    wall_g = time.perf_counter() - t0
+    gen_cache = {"node_evals_per_s": st_g["logical_node_evals"] / wall_g,
+                 "ms_per_generation": 1e3 * wall_g / n_gen,
+                 "generations": n_gen,
+                 "executed_over_logical": st_g["executed_node_evals"] / max(st_g["logical_node_evals"], 1),
+                 "host_detect_ms_per_generation": st_g["detect_time_ms"] / n_gen,
+                 "interp_kernel_ms_per_generation": st_g["interp_kernel_ms"] / n_gen,
+                 "columns": st_g["num_cached"]}
+    ctx.cache_policy(CACHE_REFERENCE, 0, 4)
 
     fit = None
     if not args.no_fit:
@@ -368,6 +387,7 @@ def main():
         "batched_over_plain": batched["node_evals_per_s"] / plain["node_evals_per_s"],
         "cached_eval": cached,
         "cached_evolution": cached_gen,
+        "generational_cache_evolution": gen_cache,
         "constant_fit": fit,
     }
     print(json.dumps(out), flush=True)

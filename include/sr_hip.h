@@ -125,6 +125,43 @@ int sr_eval_predictions(sr_ctx *ctx, const sr_population *pop, float *pred_out);
  * policy).  The cache persists across calls until sr_cache_clear or new data. */
 int sr_cache_config(sr_ctx *ctx, int32_t capacity, int32_t min_size, int32_t min_freq);
 int sr_cache_clear(sr_ctx *ctx);
+/* Cache policy for SR_EVAL_CACHED.
+ *  SR_CACHE_REFERENCE (default): the reference's policy above.
+ *  SR_CACHE_GENERATIONAL, SR_CACHE_GENERATIONAL_REUSED: no counterpart in
+ *    the reference (its cache keeps what it first detected and never evicts,
+ *    cl_gpu_eval.cu:294-329).  The cache follows the population: a subtree
+ *    (size >= min_size) with a column is read instead of evaluated; a column
+ *    is written, inside the launch that evaluates the subtree anyway, for
+ *    every new subtree (GENERATIONAL) or for a subtree met twice -- twice in
+ *    one population or in two consecutive ones (GENERATIONAL_REUSED);
+ *    columns of subtrees absent from the population being evaluated are
+ *    evicted; an expression evaluated before keeps its MSE.  capacity 0
+ *    sizes the columns from free device memory (half of it, at most 131072
+ *    columns of N floats).  MSEs and counts equal SR_EVAL_BATCHED's bit for
+ *    bit (hits are verified against the stored token body).
+ * Any call empties the cache. */
+enum { SR_CACHE_REFERENCE = 0, SR_CACHE_GENERATIONAL = 1, SR_CACHE_GENERATIONAL_REUSED = 2 };
+int sr_cache_policy(sr_ctx *ctx, int32_t policy, int32_t capacity, int32_t min_size);
+
+/* Host planner of the generational cache, without a device (inspection and
+ * tests).  plan: validated streams (as sr_eval_mse) -> launches of skeleton
+ * programs: OP_REF (-2, value = slot) reads a column written by an earlier
+ * launch, OP_STORE (-3, value = slot) evaluates its operand subtree and also
+ * writes that column.  plan_copy (any pointer may be NULL): the programs
+ * packed (num_tokens tokens/values, num_programs offsets/lengths), launch k =
+ * programs [launch_begin[k], launch_begin[k+1]) (num_launches + 1 entries),
+ * root_slot per program (slot holding its whole value, or -1), expr_src per
+ * expression (program giving its MSE, or -2 - slot of a remembered MSE).
+ * commit: MSE and finite count per program, after running the launches. */
+typedef struct sr_memo_planner sr_memo_planner;
+int sr_memo_planner_create(int32_t capacity, int32_t min_size, int32_t policy, sr_memo_planner **out);
+int sr_memo_planner_destroy(sr_memo_planner *p);
+int sr_memo_plan(sr_memo_planner *p, const sr_population *pop, int32_t *num_launches,
+                 int32_t *num_programs, int64_t *num_tokens);
+int sr_memo_plan_copy(const sr_memo_planner *p, int32_t *tokens, double *values, int32_t *offsets,
+                      int32_t *lengths, int32_t *launch_begin, int32_t *root_slot, int32_t *expr_src);
+int sr_memo_commit(sr_memo_planner *p, const float *mse, const int64_t *cnt);
+int sr_memo_remembered(const sr_memo_planner *p, int32_t slot, float *mse, int64_t *cnt);
 /* Flat hashes of the cached subtrees, by slot. */
 int sr_cache_state(sr_ctx *ctx, uint64_t *keys_out, int32_t max_keys, int32_t *num_cached);
 /* Cache marking of the last CACHED evaluation, packed like its population:

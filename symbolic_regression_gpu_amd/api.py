@@ -17,6 +17,7 @@ from dataclasses import dataclass
 import numpy as np
 
 EVAL_PLAIN, EVAL_BATCHED, EVAL_CACHED, EVAL_JIT = 0, 1, 2, 3
+CACHE_REFERENCE, CACHE_GENERATIONAL, CACHE_GENERATIONAL_REUSED = 0, 1, 2
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libsrhip.so")
@@ -116,6 +117,14 @@ _SIGNATURES = {
     "sr_jit_code": [C.POINTER(_Pop), C.c_int32, C.c_int32, _vp, C.c_int64, C.POINTER(C.c_int64)],
     "sr_cache_config": [_vp, C.c_int32, C.c_int32, C.c_int32],
     "sr_cache_clear": [_vp],
+    "sr_cache_policy": [_vp, C.c_int32, C.c_int32, C.c_int32],
+    "sr_memo_planner_create": [C.c_int32, C.c_int32, C.c_int32, C.POINTER(_vp)],
+    "sr_memo_planner_destroy": [_vp],
+    "sr_memo_plan": [_vp, C.POINTER(_Pop), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                     C.POINTER(C.c_int64)],
+    "sr_memo_plan_copy": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "sr_memo_commit": [_vp, _vp, _vp],
+    "sr_memo_remembered": [_vp, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_int64)],
     "sr_cache_state": [_vp, _vp, C.c_int32, C.POINTER(C.c_int32)],
     "sr_cache_last_hints": [_vp, _vp, C.c_int64, C.POINTER(C.c_int64)],
     "sr_detect": [_vp, C.POINTER(_Pop), C.c_int32, C.c_int32, _i32p, _u64p, _i32p, _i32p,
@@ -225,6 +234,56 @@ def jit_code(pop, nv: int, dpt: int = 4) -> bytes:
     if rc != 0:
         raise SrError(rc, "jit code")
     return buf.raw[:n.value]
+
+
+class MemoPlanner:
+    """Host planner of the generational subtree cache (no device needed):
+    turns populations into launches of skeleton programs whose OP_REF (-2)
+    tokens read, and OP_STORE (-3) tokens write, value columns by slot."""
+
+    def __init__(self, capacity: int, min_size: int = 4, policy: int = CACHE_GENERATIONAL):
+        self._L = lib()
+        self._h = _vp()
+        rc = self._L.sr_memo_planner_create(capacity, min_size, policy, C.byref(self._h))
+        if rc != 0:
+            raise SrError(rc, "sr_memo_planner_create")
+        self._nprog = 0
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.sr_memo_planner_destroy(self._h)
+            self._h = None
+
+    def plan(self, pop: "Population") -> dict:
+        nl, npg, nt = C.c_int32(), C.c_int32(), C.c_int64()
+        rc = self._L.sr_memo_plan(self._h, C.byref(pop._c()), C.byref(nl), C.byref(npg), C.byref(nt))
+        if rc != 0:
+            raise SrError(rc, "sr_memo_plan")
+        out = {"tokens": np.empty(nt.value, np.int32), "values": np.empty(nt.value, np.float64),
+               "offsets": np.empty(npg.value, np.int32), "lengths": np.empty(npg.value, np.int32),
+               "launch_begin": np.empty(nl.value + 1, np.int32),
+               "root_slot": np.empty(npg.value, np.int32),
+               "src": np.empty(pop.num_exprs, np.int32)}
+        self._L.sr_memo_plan_copy(self._h, *(out[k].ctypes.data for k in
+                                             ("tokens", "values", "offsets", "lengths", "launch_begin",
+                                              "root_slot", "src")))
+        self._nprog = npg.value
+        return out
+
+    def commit(self, mse, cnt):
+        m = np.ascontiguousarray(mse, np.float32)
+        k = np.ascontiguousarray(cnt, np.int64)
+        assert len(m) == len(k) == self._nprog
+        rc = self._L.sr_memo_commit(self._h, m.ctypes.data, k.ctypes.data)
+        if rc != 0:
+            raise SrError(rc, "sr_memo_commit")
+
+    def remembered(self, slot: int):
+        m, k = C.c_float(), C.c_int64()
+        rc = self._L.sr_memo_remembered(self._h, slot, C.byref(m), C.byref(k))
+        if rc != 0:
+            raise SrError(rc, "sr_memo_remembered")
+        return m.value, k.value
 
 
 class Population:
@@ -389,6 +448,13 @@ class Context:
 
     def cache_config(self, capacity=2048, min_size=3, min_freq=10):
         self._check(self._L.sr_cache_config(self._h, capacity, min_size, min_freq))
+
+    def cache_policy(self, policy, capacity=0, min_size=4):
+        """Select the SR_EVAL_CACHED policy: CACHE_REFERENCE (detect-once, the
+        reference's cl_gpu_eval.cu:294-329) or CACHE_GENERATIONAL (every
+        distinct subtree of the current population kept, stale ones evicted).
+        capacity 0 = sized from free device memory."""
+        self._check(self._L.sr_cache_policy(self._h, int(policy), int(capacity), int(min_size)))
 
     def cache_clear(self):
         self._check(self._L.sr_cache_clear(self._h))

@@ -404,6 +404,7 @@ int sr_cache_clear(sr_ctx *x)
     if (!x) return SR_ERR_INVALID_ARG;
     std::lock_guard<std::recursive_mutex> lk(x->c.mu);
     x->c.cache.clear();
+    x->c.memo.planner.forget();
     return SR_OK;
 }
 

@@ -13,6 +13,7 @@
 #include "../../include/sr_hip.h"
 #include "sr_format.h"
 #include "sr_ga.h"
+#include "sr_memo.h"
 
 namespace sr {
 
@@ -100,6 +101,22 @@ struct SubtreeCache {
     }
 };
 
+// Generational subtree cache (sr_memo.cpp): planner state and the device
+// value columns [capacity][N] its slots index.
+struct MemoState {
+    bool enabled = false;
+    int capacity = 0;  // requested columns; 0 = sized from free HBM
+    int min_size = 4;
+    int policy = SR_CACHE_GENERATIONAL;
+    MemoPlanner planner;
+    MemoPlan plan;
+    std::vector<float> mse;
+    std::vector<int64_t> cnt;
+    int64_t N = 0;
+    uint64_t data_version = 0;
+    DevBuf cols;
+};
+
 // Device-resident GA population (double-buffered, fixed stride cap) and the
 // buffers of its device-side evaluation.
 struct GaState {
@@ -175,6 +192,7 @@ struct Ctx {
     DevBuf d_prog, d_meta, d_psum, d_pcnt, d_mse, d_cnt, d_counters, d_pred;
 
     SubtreeCache cache;
+    MemoState memo;
     GaState ga;
     ParamState par;
     JitState jit;

@@ -4,8 +4,11 @@
 // Token format (opcodes.h:10-40): TOK_CONST = 0 (value = constant),
 // TOK_VAR = -1 (value = variable index), binary ops 1..9, unary ops 10..27.
 // OP_REF = -2 marks a cached-subtree reference in a skeleton
-// (cl_gpu_eval.cu:11; value = cache slot).  Opcode 29 is the clamped exp
-// (evolution_tests.cpp:40) that the reference evaluator does not define.
+// (cl_gpu_eval.cu:11; value = cache slot).  OP_STORE = -3 (internal to the
+// generational cache, never accepted from a caller) is a unary identity whose
+// operand value is also written to cache column 'value'.  Opcode 29 is the
+// clamped exp (evolution_tests.cpp:40) that the reference evaluator does not
+// define.
 //
 // Bytecode ("register program"): the prefix tree is re-ordered into an
 // accumulator program.  Leaves are fused into the instruction that consumes
@@ -28,6 +31,7 @@ namespace sr {
 constexpr int TOK_CONST = 0;
 constexpr int TOK_VAR = -1;
 constexpr int OP_REF = -2;
+constexpr int OP_STORE = -3;
 
 constexpr int OP_ADD = 1, OP_SUB = 2, OP_MUL = 3, OP_DIV = 4, OP_POW = 5, OP_MIN = 6,
               OP_MAX = 7, OP_LOOSE_DIV = 8, OP_LOOSE_POW = 9;
@@ -47,6 +51,10 @@ SR_HD bool is_valid_token(int t)
 {
     return is_leaf(t) || (t >= 1 && t <= 27) || t == OP_EXP_CLAMPED;
 }
+// Bytecode op of the OP_STORE pseudo-token: a unary instruction on the
+// accumulator whose arg is the destination column; the whole code word is
+// always STORE_CODE (its operand is a subtree, never a fused leaf).
+constexpr int OPC_STORE = 31;
 // IEEE add/mul are commutative bit for bit; everything else keeps its order.
 SR_HD bool is_commutative(int t) { return t == OP_ADD || t == OP_MUL; }
 
@@ -70,6 +78,7 @@ constexpr int bc_form_of(int src, int push, int swap, int cls)
 {
     return src | (push ? 8 : 0) | (swap ? 16 : 0) | (cls << 5);
 }
+constexpr int32_t STORE_CODE = OPC_STORE | (SRC_NONE << 8) | (CL_UN << BC_CLS_SHIFT);
 
 struct Ins {
     int32_t code;
@@ -118,12 +127,13 @@ SR_HD int lower_token(int t, float v, int nv, Ins *out)
 }
 
 // Stack depth the raw right-to-left interpreter reaches; -1 when malformed.
-SR_HD int raw_stack_depth(const int32_t *tok, int len)
+// OP_STORE is accepted only with store = true (cache skeletons).
+SR_HD int raw_stack_depth(const int32_t *tok, int len, bool store = false)
 {
     int sp = 0, mx = 0;
     for (int i = len - 1; i >= 0; --i) {
         int t = tok[i];
-        if (!is_valid_token(t)) return -1;
+        if (!is_valid_token(t) && !(store && t == OP_STORE)) return -1;
         if (is_leaf(t)) {
             sp++;
         } else {
@@ -223,6 +233,7 @@ SR_HD int compile_expr(const int32_t *tok, const float *val, int len, int nv, in
             }
             if (!is_binary(t)) {
                 int c = f.node + 1;
+                if (t == OP_STORE && is_leaf(tok[c])) return -1;
                 if (is_leaf(tok[c])) {
                     int src;
                     int32_t arg;
@@ -274,7 +285,8 @@ SR_HD int compile_expr(const int32_t *tok, const float *val, int len, int nv, in
         if (f.stage == 1) {
             if (f.second < 0) {
                 if (leaf_pos) leaf_pos[n] = -1;
-                out[n++] = {t | (SRC_NONE << 8) | (CL_UN << BC_CLS_SHIFT), 0};
+                if (t == OP_STORE) out[n++] = {STORE_CODE, (int32_t)val[f.node]};
+                else out[n++] = {t | (SRC_NONE << 8) | (CL_UN << BC_CLS_SHIFT), 0};
             } else {
                 int src;
                 int32_t arg;

@@ -50,7 +50,7 @@ int build_program(Ctx &c, const sr_population &pop, bool raw, Program &out, int 
             return c.fail(SR_ERR_BAD_EXPR, "expression " + std::to_string(e) + ": length " +
                                                std::to_string(L) + " outside 1.." +
                                                std::to_string(MAX_EXPR_LEN));
-        int depth = raw_stack_depth(tok, L);
+        int depth = raw_stack_depth(tok, L, num_slots > 0);
         if (depth < 0)
             return c.fail(SR_ERR_BAD_EXPR,
                           "expression " + std::to_string(e) + ": malformed prefix token stream");
@@ -61,7 +61,7 @@ int build_program(Ctx &c, const sr_population &pop, bool raw, Program &out, int 
                                                    ": negative variable index");
             // OP_REF only appears in skeletons built by the cache, whose slots
             // all hold a value column; anywhere else it is an invalid token.
-            if (tok[k] == OP_REF && !(vals[k] >= 0.0f && vals[k] < (float)num_slots))
+            if ((tok[k] == OP_REF || tok[k] == OP_STORE) && !(vals[k] >= 0.0f && vals[k] < (float)num_slots))
                 return c.fail(SR_ERR_BAD_EXPR, "expression " + std::to_string(e) +
                                                    ": cache reference outside the cache");
         }
@@ -260,6 +260,7 @@ static int run_plain(Ctx &c, const Program &p, sr_stats *st)
 }
 
 int eval_cached(Ctx &c, const sr_population &pop, sr_stats *st);  // sr_cache.cpp
+int eval_memo(Ctx &c, const sr_population &pop, sr_stats *st, float *mse_out, int64_t *valid_out);
 int eval_jit(Ctx &c, const sr_population &pop, sr_stats *st);  // sr_jit.cpp
 
 }  // namespace sr
@@ -401,6 +402,11 @@ int sr_eval_mse(sr_ctx *x, const sr_population *pop, int mode, float *mse_out, i
 
     if (mode == SR_EVAL_JIT) {
         rc = eval_jit(c, *pop, S);
+    } else if (mode == SR_EVAL_CACHED && c.memo.enabled) {
+        // results come back through the cache (some are remembered MSEs)
+        rc = eval_memo(c, *pop, S, mse_out, valid_out);
+        S->total_eval_time_ms = now_ms() - t0;
+        return rc;
     } else if (mode == SR_EVAL_CACHED) {
         rc = eval_cached(c, *pop, S);
     } else if (mode == SR_EVAL_PLAIN || mode == SR_EVAL_BATCHED) {

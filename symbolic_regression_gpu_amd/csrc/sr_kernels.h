@@ -13,11 +13,15 @@
 
 namespace sr {
 
-// MODE_MSE_REF: MSE of skeletons whose OP_REF leaves read cache columns; the
-// other modes never see OP_REF, so their instruction loop has no memory loads
-// (and the next tile's X loads stay in flight across it).
+// MODE_MSE_REF: MSE of skeletons whose OP_REF leaves read cache columns and
+// whose STORE_CODE instructions write the accumulator to a column (a.out is
+// the column base); the other modes never see either, so their instruction
+// loop has no memory access (and the next tile's X loads stay in flight
+// across it).
 enum Mode { MODE_MSE = 0, MODE_PRED = 1, MODE_COLUMN = 2, MODE_MSE_REF = 3 };
 constexpr bool is_mse_mode(int m) { return m == MODE_MSE || m == MODE_MSE_REF; }
+constexpr bool is_column_mode(int m) { return m == MODE_COLUMN; }
+constexpr bool has_ref(int m) { return m == MODE_MSE_REF; }
 
 constexpr int kBlock = 256;  // 4 waves per workgroup, one per SIMD
 constexpr int kWave = 64;
@@ -138,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
             double sum = 0.0;
             long long cnt = 0;
             float *col_out = nullptr;
-            if constexpr (MODE == MODE_COLUMN) col_out = a.out + (int64_t)a.out_col[e] * N;
+            if constexpr (is_column_mode(MODE)) col_out = a.out + (int64_t)a.out_col[e] * N;
             if constexpr (MODE == MODE_PRED) col_out = a.out + (int64_t)e * N;
 
             auto load_tile = [&](int64_t t) {
@@ -177,7 +181,17 @@ __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
                 };
                 for (int i = 0; i < n; ++i) {
                     const Ins nx = lp[i + 1];
-                    interp_step<MODE == MODE_MSE_REF>(code, arg, __int_as_float(arg), sp, ref, L...);
+                    if (has_ref(MODE) && code == STORE_CODE) {
+                        float *cs = a.out + (int64_t)arg * N;
+                        for_each_lane(
+                            [&](int d, auto &l) {
+                                int64_t dp = t0 + d * kWave + lane;
+                                if (dp < c1) cs[dp] = l.top;
+                            },
+                            L...);
+                    } else {
+                        interp_step<has_ref(MODE)>(code, arg, __int_as_float(arg), sp, ref, L...);
+                    }
                     code = __builtin_amdgcn_readfirstlane(nx.code);
                     arg = __builtin_amdgcn_readfirstlane(nx.arg);
                 }

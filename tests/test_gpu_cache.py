@@ -148,3 +148,135 @@ def test_cache_persists_across_generations(ctx):
         keys, hints = replay[g]
         assert np.array_equal(ctx.cache_keys(), keys), g
         assert np.array_equal(ctx.cache_last_hints(), hints), g
+
+
+# ---- generational policy ---------------------------------------------------
+C4 = dict(num_vars=3, max_tokens=255, max_depth=8, const_min=-2.0, const_max=2.0,
+          unary_ops=GRAMMAR_UNARY, binary_ops=GRAMMAR_BINARY)
+
+
+@pytest.mark.parametrize("capacity,min_size,policy", [(0, 4, 1), (300, 2, 1), (0, 4, 2)])
+def test_generational_cache_matches_batched_over_generations(ctx, capacity, min_size, policy):
+    """Six GA generations evaluated with the generational cache equal the
+    uncached batched MSEs bit for bit, including once the cache is full
+    (capacity 300: the subtrees without a column are evaluated inline)."""
+    from symbolic_regression_gpu_amd.api import CACHE_GENERATIONAL, CACHE_REFERENCE, GaParams, SelectParams
+    N = 20_000
+    X = uniform_data(3, N, 11)
+    X[3] = X[0] * X[1] + np.sin(X[2])
+    ctx.set_data(X)
+    ctx.cache_policy(policy, capacity, min_size)
+    try:
+        seed = 21
+        ctx.ga_init(GaParams.make(**C4), 400, seed)
+        sel = SelectParams(7, 0.1, 0.05)
+        saved = []
+        for g in range(6):
+            m_c, st = ctx.ga_evaluate(EVAL_CACHED)
+            m_b, _ = ctx.ga_evaluate(EVAL_BATCHED)
+            assert np.array_equal(m_c, m_b, equal_nan=True), g
+            if capacity:
+                assert st["num_cached"] <= capacity
+            saved.append(1 - st["executed_node_evals"] / st["logical_node_evals"])
+            if g == 0:
+                # same population again: every column exists already
+                m_c2, st2 = ctx.ga_evaluate(EVAL_CACHED)
+                assert np.array_equal(m_c2, m_b, equal_nan=True)
+                if capacity == 0 and policy == 1:
+                    # only expressions below min_size (no column) run again
+                    assert st2["num_new_subtrees"] == 0
+                    ln = ctx.ga_population()[2]
+                    assert st2["executed_node_evals"] == N * int(ln[ln < min_size].sum())
+            ctx.ga_next(sel, m_b, seed, g)
+        print(f"generational cache policy={policy} capacity={capacity}: node-evals saved per gen "
+              + " ".join(f"{s:.2f}" for s in saved))
+        if capacity == 0:
+            assert min(saved[1:]) > 0.3
+    finally:
+        ctx.cache_policy(CACHE_REFERENCE, 0, 4)
+
+
+def test_generational_cache_c3_and_oracle(ctx):
+    from symbolic_regression_gpu_amd.api import CACHE_GENERATIONAL, CACHE_REFERENCE
+    pop, nv = c3_population(5)
+    N = 100_003
+    X = uniform_data(nv, N, 3)
+    X[nv] = np.cos(X[0]) + X[1]
+    ctx.set_data(X)
+    ctx.cache_policy(CACHE_GENERATIONAL, 0, 3)
+    try:
+        api = to_api(pop)
+        m_b, v_b, _ = ctx.eval_mse(api, EVAL_BATCHED)
+        m_c, v_c, st = ctx.eval_mse(api, EVAL_CACHED)
+        assert np.array_equal(m_c, m_b, equal_nan=True)
+        assert np.array_equal(v_c, v_b)
+        check_population(pop, X, nv, N, m_c, v_c)
+        # one column per distinct subtree body of size >= 3
+        bodies = set()
+        for e in range(pop.E):
+            t, v = pop.expr(e)
+            _, sz = O.merkle(t, v)
+            for i in range(len(t)):
+                if sz[i] >= 3:
+                    bodies.add((tuple(t[i:i + sz[i]]), tuple(v[i:i + sz[i]])))
+        assert st["num_cached"] == len(bodies)
+        assert st["executed_node_evals"] < 0.7 * st["logical_node_evals"]
+        # the same population again: every MSE is remembered, nothing runs
+        m_c1, v_c1, st1 = ctx.eval_mse(api, EVAL_CACHED)
+        assert np.array_equal(m_c1, m_b, equal_nan=True)
+        assert np.array_equal(v_c1, v_b)
+        assert st1["executed_node_evals"] == 0 and st1["num_new_subtrees"] == 0
+        # one expression replaced by a copy of another: no new columns
+        exprs = [pop.expr(e) for e in range(pop.E)]
+        exprs[7] = exprs[0]
+        pop2 = O.Population(exprs)
+        m_c2, _, st2 = ctx.eval_mse(to_api(pop2), EVAL_CACHED)
+        m_b2, _, _ = ctx.eval_mse(to_api(pop2), EVAL_BATCHED)
+        assert np.array_equal(m_c2, m_b2, equal_nan=True)
+        assert st2["num_new_subtrees"] == 0
+        # new data empties the cache; results still match
+        X2 = X.copy()
+        X2[nv] = X[0] * 0.5
+        ctx.set_data(X2)
+        m_c3, _, st3 = ctx.eval_mse(api, EVAL_CACHED)
+        m_b3, _, _ = ctx.eval_mse(api, EVAL_BATCHED)
+        assert np.array_equal(m_c3, m_b3, equal_nan=True)
+        assert st3["num_new_subtrees"] == st["num_new_subtrees"]
+    finally:
+        ctx.cache_policy(CACHE_REFERENCE, 0, 4)
+
+
+def test_generational_cache_edge_cases(ctx):
+    """Mirror-image subtrees that the reference's Merkle combine maps to one
+    hash, duplicates within a population, a 1024-token chain (stores limited
+    by the program length) and expressions below min_size."""
+    from symbolic_regression_gpu_amd.api import CACHE_GENERATIONAL, CACHE_REFERENCE
+    SUB, MUL, NEG, ABS, SIN = 2, 3, 25, 24, 10
+    V = lambda k: (-1, float(k))
+    def expr(*items):
+        t = [i[0] if isinstance(i, tuple) else i for i in items]
+        v = [i[1] if isinstance(i, tuple) else 0.0 for i in items]
+        return np.array(t, np.int32), np.array(v, np.float64)
+    a = expr(SUB, V(2), MUL, V(0), V(2))
+    b = expr(SUB, V(0), MUL, V(2), V(0))
+    ha, _ = O.merkle(*a)
+    hb, _ = O.merkle(*b)
+    assert ha[0] == hb[0]  # the reference's combine collides here
+    chain = expr(*([NEG, ABS, SIN] * 341), V(1))
+    assert len(chain[0]) == 1024
+    exprs = [a, b, a, expr(SIN, V(0)), chain, b, chain]
+    pop = O.Population(exprs)
+    nv, N = 3, 5000
+    X = uniform_data(nv, N, 9)
+    ctx.set_data(X)
+    ctx.cache_policy(CACHE_GENERATIONAL, 0, 4)
+    try:
+        api = to_api(pop)
+        m_b, v_b, _ = ctx.eval_mse(api, EVAL_BATCHED)
+        for _ in range(3):
+            m_c, v_c, _ = ctx.eval_mse(api, EVAL_CACHED)
+            assert np.array_equal(m_c, m_b, equal_nan=True)
+            assert np.array_equal(v_c, v_b)
+        assert m_b[0] != m_b[1]
+    finally:
+        ctx.cache_policy(CACHE_REFERENCE, 0, 4)
