@@ -85,6 +85,9 @@ typedef struct {
     int64_t executed_node_evals; /* token evaluations actually run */
     double interp_kernel_ms;     /* interpreter kernels alone (HIP events on the context stream) */
     int32_t interp_launches;     /* interpreter kernel launches timed in interp_kernel_ms */
+    int64_t executed_op_evals;   /* operator evaluations run (operator tokens * N; 0 = not counted) */
+    int64_t column_reads;        /* cached-column operands read (datapoints) */
+    int64_t column_writes;       /* cached-column values written (datapoints) */
 } sr_stats;
 
 /* Launch shape of the batched interpreter. */

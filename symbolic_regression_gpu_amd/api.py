@@ -52,7 +52,8 @@ class _Stats(C.Structure):
         ("num_new_subtrees", C.c_int32), ("num_cached", C.c_int32),
         ("num_kernel_launches", C.c_int32), ("logical_node_evals", C.c_int64),
         ("executed_node_evals", C.c_int64), ("interp_kernel_ms", C.c_double),
-        ("interp_launches", C.c_int32),
+        ("interp_launches", C.c_int32), ("executed_op_evals", C.c_int64),
+        ("column_reads", C.c_int64), ("column_writes", C.c_int64),
     ]
 
     def as_dict(self):

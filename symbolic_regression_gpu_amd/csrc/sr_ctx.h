@@ -216,8 +216,10 @@ struct Ctx {
 struct Program {
     std::vector<Ins> ins;
     std::vector<int32_t> off, len, order, need;
+    std::vector<int32_t> ref_pf;  // [E][kRefPrefetch] (cache skeletons only)
     int max_need = 0;
     int64_t tokens = 0;
+    int64_t ops = 0, refs = 0, stores = 0;  // operator / OP_REF / OP_STORE tokens
 };
 
 // Validate and compile every expression of pop into bytecode (batched) or

@@ -79,6 +79,13 @@ constexpr int bc_form_of(int src, int push, int swap, int cls)
     return src | (push ? 8 : 0) | (swap ? 16 : 0) | (cls << 5);
 }
 constexpr int32_t STORE_CODE = OPC_STORE | (SRC_NONE << 8) | (CL_UN << BC_CLS_SHIFT);
+// OP_REF operands prefetched a tile ahead: the first kRefPrefetch OP_REF
+// instructions of a cache skeleton carry j + 1 in bits 16..18 (j = their
+// order) and the program's slot list names their columns.  Bits 16.. are
+// otherwise used only by the adjoint's parametric programs.
+constexpr int kRefPrefetch = 4;
+constexpr int BC_PF_SHIFT = 16;
+SR_HD int bc_pf(int code) { return (code >> BC_PF_SHIFT) & 7; }
 
 struct Ins {
     int32_t code;

@@ -350,6 +350,9 @@ int sr_ga_run(sr_ctx *x, const sr_select_params *spp, int mode, int32_t gens, ui
             acc.kernel_time_ms += s1.kernel_time_ms;
             acc.interp_kernel_ms += s1.interp_kernel_ms;
             acc.interp_launches += s1.interp_launches;
+            acc.executed_op_evals += s1.executed_op_evals;
+            acc.column_reads += s1.column_reads;
+            acc.column_writes += s1.column_writes;
             acc.transfer_time_ms += s1.transfer_time_ms;
             acc.jit_compile_time_ms += s1.jit_compile_time_ms;
             acc.num_kernel_launches += s1.num_kernel_launches + 2;

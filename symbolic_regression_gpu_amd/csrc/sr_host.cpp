@@ -39,6 +39,8 @@ int build_program(Ctx &c, const sr_population &pop, bool raw, Program &out, int 
     out.need.assign(E, 0);
     out.max_need = 0;
     out.tokens = 0;
+    out.ops = out.refs = out.stores = 0;
+    out.ref_pf.assign(num_slots > 0 ? (size_t)E * kRefPrefetch : 0, -1);
     std::vector<float> vals;
     std::vector<int> size(MAX_EXPR_LEN), need(MAX_EXPR_LEN);
     std::vector<CompileFrame> frames(MAX_EXPR_LEN + 1);
@@ -67,6 +69,11 @@ int build_program(Ctx &c, const sr_population &pop, bool raw, Program &out, int 
         }
         out.off[e] = (int32_t)out.ins.size();
         out.tokens += L;
+        for (int k = 0; k < L; ++k) {
+            out.ops += tok[k] > 0;
+            out.refs += tok[k] == OP_REF;
+            out.stores += tok[k] == OP_STORE;
+        }
         if (raw) {
             for (int k = 0; k < L; ++k) {
                 Ins ins;
@@ -81,6 +88,16 @@ int build_program(Ctx &c, const sr_population &pop, bool raw, Program &out, int 
                                  tmp.data(), &sn);
             if (n < 0)
                 return c.fail(SR_ERR_BAD_EXPR, "expression " + std::to_string(e) + ": compile failed");
+            if (num_slots > 0) {
+                // the first kRefPrefetch OP_REF operands are loaded a tile ahead
+                int j = 0;
+                for (int k = 0; k < n && j < kRefPrefetch; ++k) {
+                    if (bc_src(tmp[k].code) != SRC_REF) continue;
+                    tmp[k].code |= (j + 1) << BC_PF_SHIFT;
+                    out.ref_pf[(size_t)e * kRefPrefetch + j] = tmp[k].arg;
+                    ++j;
+                }
+            }
             out.ins.insert(out.ins.end(), tmp.begin(), tmp.begin() + n);
             out.len[e] = n;
             out.need[e] = sn;
@@ -169,7 +186,9 @@ int run_batched(Ctx &c, const Program &p, int mode, const float *cache_cols, flo
 
     // Pack program + metadata into pinned staging, one async copy each.
     const size_t prog_bytes = p.ins.size() * sizeof(Ins);
-    const size_t meta_ints = (size_t)E * 4;
+    // a program built without cache slots has no OP_REF: nothing to prefetch
+    const bool pf = has_ref(mode);
+    const size_t meta_ints = (size_t)E * (pf ? 4 + kRefPrefetch : 4);
     if ((he = c.h_prog.ensure(prog_bytes + 16)) != hipSuccess) return c.hip_fail(he, "hipHostMalloc");
     if ((he = c.h_meta.ensure(meta_ints * 4)) != hipSuccess) return c.hip_fail(he, "hipHostMalloc");
     std::memcpy(c.h_prog.p, p.ins.data(), prog_bytes);
@@ -179,6 +198,12 @@ int run_batched(Ctx &c, const Program &p, int mode, const float *cache_cols, flo
     std::memcpy(meta + 2 * E, p.order.data(), E * 4);
     if (out_col_host) std::memcpy(meta + 3 * E, out_col_host, E * 4);
     else std::memset(meta + 3 * E, 0, E * 4);
+    if (pf) {
+        if (p.ref_pf.size() == (size_t)E * kRefPrefetch)
+            std::memcpy(meta + 4 * E, p.ref_pf.data(), (size_t)E * kRefPrefetch * 4);
+        else
+            std::fill(meta + 4 * E, meta + 4 * E + (size_t)E * kRefPrefetch, -1);
+    }
     if ((he = c.d_prog.ensure(prog_bytes + 16)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
     if ((he = c.d_meta.ensure(meta_ints * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
     hipStream_t s = c.stream;
@@ -195,6 +220,7 @@ int run_batched(Ctx &c, const Program &p, int mode, const float *cache_cols, flo
     a.prog_len = a.prog_off + E;
     a.order = a.prog_off + 2 * E;
     a.out_col = a.prog_off + 3 * E;
+    a.ref_pf = pf ? a.prog_off + 4 * E : nullptr;
     a.cache = cache_cols;
     a.out = out_dev;
     rc = enqueue_batched(c, mode, lp, a, E, s, st ? c.ev[4] : nullptr, st ? c.ev[5] : nullptr);
@@ -203,6 +229,9 @@ int run_batched(Ctx &c, const Program &p, int mode, const float *cache_cols, flo
     if (st) {
         st->num_kernel_launches += is_mse_mode(mode) ? 3 : 2;
         st->executed_node_evals += p.tokens * c.N;
+        st->executed_op_evals += p.ops * c.N;
+        st->column_reads += p.refs * c.N;
+        st->column_writes += (p.stores + (is_column_mode(mode) ? E : 0)) * c.N;
         // every caller synchronises right after; waiting here costs nothing
         if ((he = hipEventSynchronize(c.ev[5])) != hipSuccess) return c.hip_fail(he, "interpreter");
         float ms = 0.f;
@@ -250,6 +279,7 @@ static int run_plain(Ctx &c, const Program &p, sr_stats *st)
     if (st) {
         st->num_kernel_launches += E + 1;
         st->executed_node_evals += p.tokens * c.N;
+        st->executed_op_evals += p.ops * c.N;
         if ((he = hipEventSynchronize(c.ev[5])) != hipSuccess) return c.hip_fail(he, "plain interpreter");
         float ms = 0.f;
         hipEventElapsedTime(&ms, c.ev[4], c.ev[5]);

@@ -34,6 +34,7 @@ struct BatchArgs {
     const int32_t *out_col;   // MODE_COLUMN: destination cache slot per expression
     const float *X;
     const float *cache;       // OP_REF source columns (may be null)
+    const int32_t *ref_pf;    // MODE_MSE_REF: [E][kRefPrefetch] prefetched columns, -1 = none
     float *out;               // MODE_PRED: [E][N]; MODE_COLUMN: cache base
     double *psum;             // MODE_MSE: [E][P]
     long long *pcnt;
@@ -117,6 +118,10 @@ __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
     const int items = a.n_exprs * a.chunks_per_queue;
     const int64_t N = a.N;
     constexpr int64_t kTile = (int64_t)kWave * DPT;
+    // OP_REF operands held in registers (current and next tile): 2 * KR * DPT
+    // VGPRs, so fewer at higher DPT.
+    constexpr int KR = has_ref(MODE) ? (DPT == 4 ? 2 : 4) : 0;
+    float rr[KR > 0 ? KR : 1][DPT], rn[KR > 0 ? KR : 1][DPT];
 
     with_lanes<DPT, S, XC>([&](auto &...L) {
         for (;;) {
@@ -134,6 +139,10 @@ __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
                 a.progs + (a.prog_off ? (int64_t)a.prog_off[e] : (int64_t)e * a.prog_stride);
             const int n = a.prog_len[e];
             for (int k = lane; k < n; k += kWave) lp[k] = prog[k];
+            // columns of the first KR OP_REF operands, loaded a tile ahead
+            int pf[KR > 0 ? KR : 1];
+#pragma unroll
+            for (int j = 0; j < KR; ++j) pf[j] = __builtin_amdgcn_readfirstlane(a.ref_pf[(int64_t)e * kRefPrefetch + j]);
             __builtin_amdgcn_wave_barrier();
             int64_t q0 = (int64_t)q * a.slice;
             int64_t q1 = q0 + a.slice < N ? q0 + a.slice : N;
@@ -145,48 +154,68 @@ __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
             if constexpr (is_column_mode(MODE)) col_out = a.out + (int64_t)a.out_col[e] * N;
             if constexpr (MODE == MODE_PRED) col_out = a.out + (int64_t)e * N;
 
+            // Tile addressing: a uniform 64-bit base per column and tile plus
+            // a 32-bit lane offset (clamped to the tile's last valid
+            // datapoint), so every access is one scaled-offset load.
+            auto tile_last = [&](int64_t t) { return (unsigned)((c1 - t < kTile ? c1 - t : kTile) - 1); };
             auto load_tile = [&](int64_t t) {
+                const unsigned last = tile_last(t);
                 for_each_lane(
                     [&](int d, auto &l) {
-                        int64_t dp = t + d * kWave + lane;
-                        int64_t dc = dp < N ? dp : N - 1;
+                        const unsigned i = min((unsigned)(d * kWave + lane), last);
 #pragma unroll
                         for (int k = 0; k < XC; ++k)
-                            if (k < a.nv) l.xn[k] = a.X[(int64_t)k * N + dc];
-                        if constexpr (is_mse_mode(MODE)) l.yn = a.X[(int64_t)a.nv * N + dc];
+                            if (k < a.nv) l.xn[k] = (a.X + (int64_t)k * N + t)[i];
+                        if constexpr (is_mse_mode(MODE)) l.yn = (a.X + (int64_t)a.nv * N + t)[i];
+#pragma unroll
+                        for (int j = 0; j < KR; ++j)
+                            if (pf[j] >= 0) rn[j][d] = (a.cache + (int64_t)pf[j] * N + t)[i];
                     },
                     L...);
             };
             if (c0 < c1) load_tile(c0);
             for (int64_t t0 = c0; t0 < c1; t0 += kTile) {
                 for_each_lane(
-                    [&](int, auto &l) {
+                    [&](int d, auto &l) {
 #pragma unroll
                         for (int k = 0; k < XC; ++k) l.x[k] = l.xn[k];
                         l.y = l.yn;
+#pragma unroll
+                        for (int j = 0; j < KR; ++j) rr[j][d] = rn[j][d];
                     },
                     L...);
                 if (t0 + kTile < c1) load_tile(t0 + kTile);
+                const unsigned last = tile_last(t0);
                 int sp = 0;
                 int code = __builtin_amdgcn_readfirstlane(lp[0].code);
                 int arg = __builtin_amdgcn_readfirstlane(lp[0].arg);
                 auto ref = [&](auto &...ll) {
-                    const float *colr = a.cache + (int64_t)arg * N;
-                    for_each_lane(
-                        [&](int d, auto &l) {
-                            int64_t dp = t0 + d * kWave + lane;
-                            l.o = colr[dp < N ? dp : N - 1];
-                        },
-                        ll...);
+                    const int j = bc_pf(code) - 1;  // uniform
+                    if (j >= 0 && j < KR) {
+                        for_each_lane(
+                            [&](int d, auto &l) {
+                                float v = rr[0][d];
+#pragma unroll
+                                for (int k = 1; k < KR; ++k)
+                                    if (j == k) v = rr[k][d];
+                                l.o = v;
+                            },
+                            ll...);
+                    } else {
+                        const float *colr = a.cache + (int64_t)arg * N + t0;
+                        for_each_lane(
+                            [&](int d, auto &l) { l.o = colr[min((unsigned)(d * kWave + lane), last)]; },
+                            ll...);
+                    }
                 };
                 for (int i = 0; i < n; ++i) {
                     const Ins nx = lp[i + 1];
                     if (has_ref(MODE) && code == STORE_CODE) {
-                        float *cs = a.out + (int64_t)arg * N;
+                        float *cs = a.out + (int64_t)arg * N + t0;
                         for_each_lane(
                             [&](int d, auto &l) {
-                                int64_t dp = t0 + d * kWave + lane;
-                                if (dp < c1) cs[dp] = l.top;
+                                const unsigned i = (unsigned)(d * kWave + lane);
+                                if (i <= last) cs[i] = l.top;
                             },
                             L...);
                     } else {
@@ -198,15 +227,15 @@ __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
                 // Epilogue.
                 for_each_lane(
                     [&](int d, auto &l) {
-                        int64_t dp = t0 + d * kWave + lane;
+                        const unsigned i = (unsigned)(d * kWave + lane);
                         if constexpr (is_mse_mode(MODE)) {
-                            bool ok = dp < c1 && isfinite(l.top);
+                            bool ok = i <= last && isfinite(l.top);
                             float diff = l.top - l.y;
                             float sq = diff * diff;
                             sum += ok ? (double)sq : 0.0;
                             cnt += ok ? 1 : 0;
                         } else {
-                            if (dp < c1) col_out[dp] = l.top;
+                            if (i <= last) (col_out + t0)[i] = l.top;
                         }
                     },
                     L...);
