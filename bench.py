@@ -6,23 +6,27 @@ x {sin, cos, clamped exp, protected log, sqrt|.|}, target a seeded depth-5
 tree (the reference's evaluate -> fitness -> select -> vary step,
 cl_sr_system.cpp:168-333).
 
-One step = one generation on the device: batched persistent evaluation with
-fused MSE, rank, tournament / elitism and mutate / crossover, replayed as one
-hipGraph.  `value` = logical node-evals/s of those generations (sum_e len_e * N
-per generation / wall time); `ms_per_step` = ms per generation.  Beside it, on
-the same initial population and data: the plain one-launch-per-expression
-interpreter, one batched evaluation, the subtree-cached evaluation, and the
-constant-fitting workload (4096 expressions of 50-200 tokens, 2-8 free
-constants, 5 variables, 256k datapoints): adjoint vs central-difference
-gradient evaluations/s.
+One step = one generation: the population is evaluated by the batched
+persistent interpreter over the generational subtree cache (host plan, one or
+two device launches, fused MSE), then ranked, selected and varied on the
+device.  `value` = logical node-evals/s of the timed generations (sum_e len_e
+* N per generation / wall time, host planning and transfers included);
+`ms_per_step` = ms per generation.  The same generations are also run with
+the uncached batched evaluation replayed as one hipGraph per generation (their
+fitness vectors must be identical), and the plain one-launch-per-expression
+interpreter is timed on the initial and the final population.  Beside them:
+the reference-policy subtree cache, and the constant-fitting workload (4096
+expressions of 50-200 tokens, 2-8 free constants, 5 variables, 256k
+datapoints): adjoint vs central-difference gradient evaluations/s.
 
-roofline: the batched interpreter kernel (HIP events around its launch on the
-context stream), counted in FP32 operator-node evaluations (1 flop per
-operator node per datapoint, leaves excluded) against the FP32 vector peak;
-it issues no MFMA.  The HBM side (X|y bytes per launch) is reported too.
-cpu_baseline: the oracle's restatement of evaluate_cpu_mse
-(cl_cpu_eval.cpp:94-168) on a bounded sample of the same population, on this
-host's cores.
+roofline: the skeleton interpreter kernel of the timed generations (HIP events
+around each launch on the context stream), counted in FP32 operator
+evaluations (1 flop per operator token per datapoint, leaves excluded) against
+the FP32 vector peak; it issues no MFMA.  Its HBM side (column reads/writes +
+X|y per launch) is reported too; roofline_batched is the uncached batched
+kernel on the initial population.  cpu_baseline: the oracle's restatement of
+evaluate_cpu_mse (cl_cpu_eval.cpp:94-168) on a bounded sample of the initial
+population, on this host's cores.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 2]
 """
@@ -45,6 +49,7 @@ METRIC = "node-evals/s + ms/generation (1000 exprs × 1M dps), plain vs batched+
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "traffic_k_batched_evolution_1000x1M.json")
+TRAFFIC_FILE_REF = os.path.join(HERE, "profiles", "traffic_k_batched_mseref_cached_evolution.json")
 
 
 def log(*a):
@@ -251,81 +256,85 @@ def main():
 
     import torch
     from symbolic_regression_gpu_amd import EVAL_BATCHED, EVAL_CACHED, EVAL_PLAIN, Context
-    from symbolic_regression_gpu_amd.api import CACHE_GENERATIONAL, CACHE_REFERENCE, SelectParams
+    from symbolic_regression_gpu_amd.api import CACHE_GENERATIONAL_REUSED, CACHE_REFERENCE, SelectParams
     torch.cuda.init()
     ctx = Context(0)
     N = args.N
+    K, W = args.steps, args.warmup
     log(f"[bench] evolution workload: pop 1000, N={N}")
     X, nv, (tok, val, ln, cap) = evolution_setup(ctx, N, args.seed)
     pop0 = to_population(tok, val, ln, cap)
-    flops = operator_nodes(pop0) * N
+    flops0 = operator_nodes(pop0) * N
     log("[bench] autotuning the batched launch shape")
     tuned, tune_ms = ctx.autotune(pop0, max_sample=256)
-    bytes_alg = (nv + 1) * 4 * N
+    xy_bytes = (nv + 1) * 4 * N
 
-    # roofline + plain vs batched on the initial population
-    log("[bench] batched evaluation")
-    batched = time_evals(ctx, pop0, EVAL_BATCHED, args.reps)
+    log("[bench] batched evaluation of the initial population")
+    batched0 = time_evals(ctx, pop0, EVAL_BATCHED, args.reps)
     if args.roofline_only:
-        print(json.dumps({"kernel_ms": batched["interp_kernel_ms"], "flops_per_launch": flops,
+        print(json.dumps({"kernel_ms": batched0["interp_kernel_ms"], "flops_per_launch": flops0,
                           "launches": args.reps + 1, "tuning": [tuned.dpt, tuned.chunk, tuned.n_queues,
                                                                 tuned.blocks_per_cu]}), flush=True)
         return
-    log("[bench] plain one-launch-per-expression evaluation")
-    plain = time_evals(ctx, pop0, EVAL_PLAIN, max(2, args.reps // 2))
-    log("[bench] subtree-cached evaluation")
+    log("[bench] plain one-launch-per-expression evaluation of the initial population")
+    plain0 = time_evals(ctx, pop0, EVAL_PLAIN, max(2, args.reps // 2))
+    log("[bench] reference-policy subtree cache on the initial population")
     ctx.cache_config(2048, 3, 10)
     ctx.cache_clear()
     _, _, st_c1 = ctx.eval_mse(pop0, EVAL_CACHED)
-    cached = time_evals(ctx, pop0, EVAL_CACHED, 2)
-    cached["first_call_new_subtrees"] = st_c1["num_new_subtrees"]
-    cached["coverage"] = st_c1["coverage"]
+    ref_cache = time_evals(ctx, pop0, EVAL_CACHED, 2)
+    ref_cache["first_call_new_subtrees"] = st_c1["num_new_subtrees"]
+    ref_cache["coverage"] = st_c1["coverage"]
 
-    # the timed generations
     sel = SelectParams(tournament=7, p_mutation=0.1, elite_frac=0.05)
+    # (A) device-resident generations, batched evaluation, one hipGraph each
     ctx.ga_set_population(tok, val, ln, cap)
-    log(f"[bench] {args.warmup} warm-up + {args.steps} timed generations (hipGraph)")
-    ctx.ga_run(sel, EVAL_BATCHED, args.warmup, 0, args.seed, True)
-    torch.cuda.synchronize()
+    log(f"[bench] batched generations: {W} warm-up + {K} timed (hipGraph)")
+    ctx.ga_run(sel, EVAL_BATCHED, W, 0, args.seed, True)
     ctx.sync()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    hist, ms_dev, st = ctx.ga_run(sel, EVAL_BATCHED, args.steps, args.warmup, args.seed, True)
+    hist_a, ms_dev, st_a = ctx.ga_run(sel, EVAL_BATCHED, K, W, args.seed, True)
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall_a = time.perf_counter() - t0
+    batched_gen = {"node_evals_per_s": st_a["logical_node_evals"] / wall_a,
+                   "ms_per_generation": 1e3 * wall_a / K,
+                   "device_ms_per_generation": ms_dev / K}
+
+    # (B) the timed steps: the same generations (same seed, same initial
+    # population) with the generational subtree cache; its MSEs equal the
+    # batched ones bit for bit, so selection and offspring are identical.
+    ctx.cache_policy(CACHE_GENERATIONAL_REUSED, 0, 4)
+    ctx.ga_set_population(tok, val, ln, cap)
+    log(f"[bench] cached generations: {W} warm-up + {K} timed")
+    ctx.ga_run(sel, EVAL_CACHED, W, 0, args.seed, False)
+    ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist_b, _, st_b = ctx.ga_run(sel, EVAL_CACHED, K, W, args.seed, False)
     ctx.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    gen_node_evals = st["logical_node_evals"]
-    value = gen_node_evals / wall
-    best = float(np.nanmin(np.where(np.isfinite(hist[-1]), hist[-1], np.nan)))
-
-    # cached evolution (host-assisted detection, device evaluation and GA)
-    log("[bench] cached generations")
-    ctx.cache_clear()
-    ctx.ga_set_population(tok, val, ln, cap)
-    t0 = time.perf_counter()
-    _, _, st_cg = ctx.ga_run(sel, EVAL_CACHED, max(2, args.steps // 4), 0, args.seed, False)
-    wall_c = time.perf_counter() - t0
-    cached_gen = {"node_evals_per_s": st_cg["logical_node_evals"] / wall_c,
-                  "ms_per_generation": 1e3 * wall_c / max(2, args.steps // 4),
-                  "executed_over_logical": st_cg["executed_node_evals"] / max(st_cg["logical_node_evals"], 1)}
-
-    # generational cache: every distinct subtree of the current population
-    # keeps its column; a generation executes only the changed nodes
-    log("[bench] generational-cache generations")
-    ctx.cache_policy(CACHE_GENERATIONAL, 0, 4)
-    ctx.ga_set_population(tok, val, ln, cap)
-    ctx.ga_run(sel, EVAL_CACHED, 2, 0, args.seed, False)
-    n_gen = max(4, args.steps // 2)
-    t0 = time.perf_counter()
-    _, _, st_g = ctx.ga_run(sel, EVAL_CACHED, n_gen, 2, args.seed, False)
-    wall_g = time.perf_counter() - t0
-    gen_cache = {"node_evals_per_s": st_g["logical_node_evals"] / wall_g,
-                 "ms_per_generation": 1e3 * wall_g / n_gen,
-                 "generations": n_gen,
-                 "executed_over_logical": st_g["executed_node_evals"] / max(st_g["logical_node_evals"], 1),
-                 "host_detect_ms_per_generation": st_g["detect_time_ms"] / n_gen,
-                 "interp_kernel_ms_per_generation": st_g["interp_kernel_ms"] / n_gen,
-                 "columns": st_g["num_cached"]}
+    value = st_b["logical_node_evals"] / wall
+    identical = bool(np.array_equal(hist_a, hist_b, equal_nan=True))
+    best = float(np.nanmin(np.where(np.isfinite(hist_b[-1]), hist_b[-1], np.nan)))
+    cached_gen = {"node_evals_per_s": value, "ms_per_generation": 1e3 * wall / K,
+                  "executed_over_logical": st_b["executed_node_evals"] / max(st_b["logical_node_evals"], 1),
+                  "host_plan_ms_per_generation": st_b["detect_time_ms"] / K,
+                  "interp_kernel_ms_per_generation": st_b["interp_kernel_ms"] / K,
+                  "launches_per_generation": st_b["interp_launches"] / K,
+                  "columns_written_per_generation": st_b["column_writes"] / N / K,
+                  "columns_read_per_generation": st_b["column_reads"] / N / K,
+                  "columns_live": st_b["num_cached"],
+                  "fitness_identical_to_batched": identical}
     ctx.cache_policy(CACHE_REFERENCE, 0, 4)
+
+    # (C) plain and batched on the last population of the run
+    popK = to_population(*ctx.ga_population())
+    log("[bench] plain and batched evaluation of the final population")
+    plainK = time_evals(ctx, popK, EVAL_PLAIN, 2)
+    batchedK = time_evals(ctx, popK, EVAL_BATCHED, 3)
 
     fit = None
     if not args.no_fit:
@@ -337,20 +346,32 @@ def main():
         log("[bench] CPU baseline")
         cpu = cpu_baseline(X, nv, pop0.tokens, pop0.values, pop0.offsets, pop0.lengths)
 
-    k_ms = batched["interp_kernel_ms"]
+    # roofline of the dominant kernel of the timed steps: the skeleton
+    # interpreter (k_batched, MODE_MSE_REF), over every launch of the timed
+    # generations (HIP events around each launch on the context stream)
+    launches = max(st_b["interp_launches"], 1)
+    k_ms = st_b["interp_kernel_ms"] / launches
+    flops = st_b["executed_op_evals"] / launches
     achieved = flops / (k_ms * 1e-3) / 1e12
+    hbm_alg = (st_b["column_reads"] + st_b["column_writes"]) * 4 / launches + xy_bytes
     traffic = None
+    if os.path.exists(TRAFFIC_FILE_REF):
+        with open(TRAFFIC_FILE_REF) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    b_ms = batched0["interp_kernel_ms"]
+    b_ach = flops0 / (b_ms * 1e-3) / 1e12
+    b_traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            b_traffic = json.load(f).get("hbm_bytes_per_launch")
     out = {
         "metric": METRIC,
         "value": value,
         "unit": "node-evals/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * wall / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -359,11 +380,13 @@ def main():
         "config": {
             "workload": "evolution loop: population 1000, 1M datapoints, 3 vars, max depth 8, "
                         "<=255 tokens, tournament 7, crossover 0.9 / mutation 0.1, elites 5%, "
-                        "target depth-5 tree; one step = one device generation (hipGraph)",
+                        "target depth-5 tree; one step = one generation: batched persistent "
+                        "evaluation over the generational subtree cache (host plan, device "
+                        "launches), fused MSE, device select/vary",
             "N": N, "population": 1000, "tokens_initial": int(pop0.tokens.size),
-            "node_evals_per_generation": gen_node_evals // max(args.steps, 1),
+            "tokens_final": int(popK.tokens.size),
+            "node_evals_per_generation": st_b["logical_node_evals"] // max(K, 1),
             "best_mse_last_generation": best,
-            "device_ms_per_generation": ms_dev / max(args.steps, 1),
             "tuning": {"dpt": tuned.dpt, "chunk": tuned.chunk, "n_queues": tuned.n_queues,
                        "blocks_per_cu": tuned.blocks_per_cu},
         },
@@ -374,20 +397,34 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": traffic,
-            "kernel": "k_batched (MODE_MSE), one launch = 1000 expressions x 1M datapoints",
+            "kernel": f"k_batched<{tuned.dpt},9,4,MODE_MSE_REF> (cache skeletons), average over the "
+                      "timed generations' launches",
             "kernel_ms": k_ms,
+            "launches": launches,
             "flops_per_launch": flops,
-            "hbm_algorithmic_bytes_per_launch": bytes_alg,
-            "hbm_achieved_GBs": bytes_alg / (k_ms * 1e-3) / 1e9,
-            "hbm_frac": bytes_alg / (k_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "hbm_algorithmic_bytes_per_launch": hbm_alg,
+            "hbm_achieved_GBs": hbm_alg / (k_ms * 1e-3) / 1e9,
+            "hbm_frac": hbm_alg / (k_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+        },
+        "roofline_batched": {
+            "bound": "valu", "achieved": b_ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": b_ach / PEAK_FP32_TFLOPS, "traffic": b_traffic,
+            "kernel": f"k_batched<{tuned.dpt},9,4,MODE_MSE>, one launch = the initial 1000 expressions x 1M",
+            "kernel_ms": b_ms, "flops_per_launch": flops0,
+            "hbm_algorithmic_bytes_per_launch": xy_bytes,
+            "hbm_achieved_GBs": xy_bytes / (b_ms * 1e-3) / 1e9,
         },
         "cpu_baseline": cpu,
-        "plain_per_expression": plain,
-        "batched_eval": batched,
-        "batched_over_plain": batched["node_evals_per_s"] / plain["node_evals_per_s"],
-        "cached_eval": cached,
         "cached_evolution": cached_gen,
-        "generational_cache_evolution": gen_cache,
+        "batched_evolution": batched_gen,
+        "plain_final_population": plainK,
+        "batched_final_population": batchedK,
+        "cached_over_plain": value / plainK["node_evals_per_s"],
+        "cached_over_batched_generations": value / batched_gen["node_evals_per_s"],
+        "plain_initial_population": plain0,
+        "batched_initial_population": batched0,
+        "batched_over_plain_initial": batched0["node_evals_per_s"] / plain0["node_evals_per_s"],
+        "reference_cache_initial_population": ref_cache,
         "constant_fit": fit,
     }
     print(json.dumps(out), flush=True)

@@ -92,7 +92,7 @@ typedef struct {
 
 /* Launch shape of the batched interpreter. */
 typedef struct {
-    int32_t dpt;           /* datapoints per lane: 1, 2, 4 */
+    int32_t dpt;           /* datapoints per lane: 1..4 (3 with <= 4 variables only) */
     int32_t chunk;         /* datapoints per work item (multiple of 64*dpt) */
     int32_t n_queues;      /* 1, or 8 = one queue per XCD */
     int32_t blocks_per_cu; /* persistent workgroups per CU (0 = occupancy limit) */

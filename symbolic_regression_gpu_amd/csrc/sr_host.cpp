@@ -119,8 +119,9 @@ int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp)
     hipError_t he;
     lp.xc = pick_xc(c.nv);
     lp.dpt = c.tune.dpt;
-    if (lp.dpt != 1 && lp.dpt != 2 && lp.dpt != 4) lp.dpt = 4;
+    if (lp.dpt < 1 || lp.dpt > 4) lp.dpt = 4;
     if (lp.xc == 32 && lp.dpt > 2) lp.dpt = 2;
+    if (lp.dpt == 3 && lp.xc != 4) lp.dpt = 2;  // dpt 3 is built for <= 4 variables only
     lp.nq = c.tune.n_queues > 1 ? c.tune.n_queues : 1;
     const int64_t tile = 64 * lp.dpt;
     lp.slice = round_up((c.N + lp.nq - 1) / lp.nq, 512);
@@ -556,8 +557,7 @@ int sr_set_tuning(sr_ctx *x, const sr_tuning *t)
 {
     if (!x || !t) return SR_ERR_INVALID_ARG;
     Ctx &c = x->c;
-    if (t->dpt != 1 && t->dpt != 2 && t->dpt != 4 && t->dpt != 8)
-        return c.fail(SR_ERR_INVALID_ARG, "dpt must be 1, 2, 4 or 8");
+    if (t->dpt < 1 || t->dpt > 4) return c.fail(SR_ERR_INVALID_ARG, "dpt must be 1, 2, 3 or 4");
     if (t->chunk < 64 || (t->n_queues != 1 && t->n_queues != 8) || t->blocks_per_cu < 0)
         return c.fail(SR_ERR_INVALID_ARG, "bad tuning");
     c.tune = *t;

@@ -104,8 +104,13 @@ SR_DEV void interp_step(int code, int arg, float cv, int &sp, RefLoad &&ref, L &
 // read ahead past its end.
 constexpr int kProgLds = MAX_EXPR_LEN + 1;
 
+// Waves per SIMD the register allocation must leave room for: the 3-datapoint
+// variant runs 3 waves per SIMD (<= 168 VGPRs) in every mode.
+constexpr int batched_min_waves(int dpt) { return dpt == 3 ? 3 : 1; }
+
 template <int DPT, int S, int XC, int MODE>
-__global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(batched_min_waves(DPT))))
+void k_batched(BatchArgs a)
 {
     // Each wave copies its expression's program into LDS once per work item
     // (coalesced), then reads instruction i + 1 from LDS while executing
@@ -120,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_batched(BatchArgs a)
     constexpr int64_t kTile = (int64_t)kWave * DPT;
     // OP_REF operands held in registers (current and next tile): 2 * KR * DPT
     // VGPRs, so fewer at higher DPT.
-    constexpr int KR = has_ref(MODE) ? (DPT == 4 ? 2 : 4) : 0;
+    constexpr int KR = has_ref(MODE) ? (DPT >= 3 ? 2 : 4) : 0;
     float rr[KR > 0 ? KR : 1][DPT], rn[KR > 0 ? KR : 1][DPT];
 
     with_lanes<DPT, S, XC>([&](auto &...L) {

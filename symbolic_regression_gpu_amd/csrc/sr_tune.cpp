@@ -76,12 +76,12 @@ int sr_autotune(sr_ctx *x, const sr_population *pop, int32_t max_sample, sr_tuni
     const sr_tuning saved = c.tune;
     sr_tuning best = saved;
     double best_t = 1e300;
-    const int dpts[] = {1, 2, 4};
-    const int chunks[] = {4096, 8192, 16384, 32768};
+    const int dpts[] = {1, 2, 3, 4};
+    const int chunks[] = {3072, 4096, 6144, 8192, 12288, 16384, 24576, 32768};
     const int queues[] = {1, 8};
     const int bpcs[] = {0, 1};
     for (int dpt : dpts) {
-        if (xc == 32 && dpt > 2) continue;
+        if ((xc == 32 && dpt > 2) || (dpt == 3 && xc != 4)) continue;
         for (int ch : chunks) {
             if (ch % (64 * dpt)) continue;
             for (int nq : queues)

@@ -83,7 +83,7 @@ def test_batched_tunings_agree(ctx):
     check_population(pop, X, 3, N, base, vbase)
     saved = ctx.tuning
     try:
-        for dpt in (1, 2, 4):
+        for dpt in (1, 2, 3, 4):
             for nq in (1, 8):
                 for chunk in (512, 16384):
                     ctx.tuning = Tuning(dpt=dpt, chunk=chunk, n_queues=nq, blocks_per_cu=0)
@@ -156,7 +156,7 @@ def test_autotune_picks_a_valid_shape_and_keeps_results(ctx):
     try:
         base, vbase, _ = ctx.eval_mse(api, EVAL_BATCHED)
         best, ms = ctx.autotune(api, max_sample=64)
-        assert best.dpt in (1, 2, 4) and best.n_queues in (1, 8) and best.chunk % (64 * best.dpt) == 0
+        assert best.dpt in (1, 2, 3, 4) and best.n_queues in (1, 8) and best.chunk % (64 * best.dpt) == 0
         assert ms > 0 and ctx.tuning == best
         m, v, _ = ctx.eval_mse(api, EVAL_BATCHED)
         assert np.array_equal(v, vbase) and mse_close(m, base, 1e-12).all()
