@@ -93,9 +93,10 @@ typedef struct {
 /* Launch shape of the batched interpreter. */
 typedef struct {
     int32_t dpt;           /* datapoints per lane: 1..4 (3 with <= 4 variables only) */
-    int32_t chunk;         /* datapoints per work item (multiple of 64*dpt) */
+    int32_t chunk;         /* datapoints per work item (rounded up to a multiple of 768) */
     int32_t n_queues;      /* 1, or 8 = one queue per XCD */
     int32_t blocks_per_cu; /* persistent workgroups per CU (0 = occupancy limit) */
+    int32_t cache_dpt;     /* datapoints per lane of cache-skeleton launches (0 = 3) */
 } sr_tuning;
 
 int sr_ctx_create(int device, sr_ctx **out);

@@ -62,7 +62,7 @@ class _Stats(C.Structure):
 
 class _Tuning(C.Structure):
     _fields_ = [("dpt", C.c_int32), ("chunk", C.c_int32), ("n_queues", C.c_int32),
-                ("blocks_per_cu", C.c_int32)]
+                ("blocks_per_cu", C.c_int32), ("cache_dpt", C.c_int32)]
 
 
 class GaParams(C.Structure):
@@ -349,10 +349,11 @@ class Population:
 
 @dataclass
 class Tuning:
-    dpt: int = 4
-    chunk: int = 16384
+    dpt: int = 3
+    chunk: int = 3072
     n_queues: int = 8
     blocks_per_cu: int = 0
+    cache_dpt: int = 3
 
 
 class Context:
@@ -425,11 +426,11 @@ class Context:
     def tuning(self) -> Tuning:
         t = _Tuning()
         self._check(self._L.sr_get_tuning(self._h, C.byref(t)))
-        return Tuning(t.dpt, t.chunk, t.n_queues, t.blocks_per_cu)
+        return Tuning(t.dpt, t.chunk, t.n_queues, t.blocks_per_cu, t.cache_dpt)
 
     @tuning.setter
     def tuning(self, v: Tuning):
-        t = _Tuning(v.dpt, v.chunk, v.n_queues, v.blocks_per_cu)
+        t = _Tuning(v.dpt, v.chunk, v.n_queues, v.blocks_per_cu, v.cache_dpt)
         self._check(self._L.sr_set_tuning(self._h, C.byref(t)))
 
     # ---- subtree cache ----------------------------------------------------
@@ -439,7 +440,7 @@ class Context:
         t = _Tuning()
         ms = C.c_double()
         self._check(self._L.sr_autotune(self._h, C.byref(pop._c()), max_sample, C.byref(t), C.byref(ms)))
-        return Tuning(t.dpt, t.chunk, t.n_queues, t.blocks_per_cu), ms.value
+        return Tuning(t.dpt, t.chunk, t.n_queues, t.blocks_per_cu, t.cache_dpt), ms.value
 
     def param_autotune(self, d_cvals: int):
         """Datapoints per lane for the forward and adjoint constant-fitting kernels."""

@@ -185,7 +185,7 @@ struct Ctx {
     int64_t N = 0;
     uint64_t data_version = 0;
 
-    sr_tuning tune{4, 16384, 8, 0};
+    sr_tuning tune{3, 3072, 8, 0, 3};
 
     // staging (host pinned) and device buffers for one evaluation
     HostBuf h_prog, h_meta, h_out;

@@ -28,11 +28,13 @@ SR_DEV float qnan() { return __builtin_nanf(""); }
 // small arrays so the compiler keeps each one in VGPRs and indexes it with
 // s_set_gpr_idx (the index -- stack pointer or variable id -- is wave-uniform).
 // ---------------------------------------------------------------------------
-// Variable registers per datapoint: at least 9, so a read at a uniform
-// dynamic index goes through GPR-index mode instead of a compare/select chain
-// over every element (which the backend picks for arrays of <= 8 floats).
+// Variable registers per datapoint.  The backend reads an array of <= 8
+// floats at a uniform dynamic index through a compare/select chain over every
+// element, larger ones through GPR-index mode (one move).  With <= 4
+// variables the 4-element chain (3 selects) costs less than the 5 extra
+// registers per datapoint a 9-element array would take from occupancy.
 template <int XC>
-constexpr int kXRegs = XC > 8 ? XC : 9;
+constexpr int kXRegs = XC > 8 ? XC : (XC == 4 ? 4 : 9);
 
 template <int S, int XC>
 struct Lane {

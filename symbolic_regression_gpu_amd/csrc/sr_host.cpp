@@ -118,14 +118,18 @@ int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp)
 {
     hipError_t he;
     lp.xc = pick_xc(c.nv);
-    lp.dpt = c.tune.dpt;
+    lp.dpt = mode == MODE_MSE_REF ? (c.tune.cache_dpt > 0 ? c.tune.cache_dpt : 3) : c.tune.dpt;
     if (lp.dpt < 1 || lp.dpt > 4) lp.dpt = 4;
+    if (lp.dpt == 3 && lp.xc != 4) lp.dpt = 4;  // dpt 3 is built for <= 4 variables only
     if (lp.xc == 32 && lp.dpt > 2) lp.dpt = 2;
-    if (lp.dpt == 3 && lp.xc != 4) lp.dpt = 2;  // dpt 3 is built for <= 4 variables only
     lp.nq = c.tune.n_queues > 1 ? c.tune.n_queues : 1;
-    const int64_t tile = 64 * lp.dpt;
-    lp.slice = round_up((c.N + lp.nq - 1) / lp.nq, 512);
-    lp.chunk = round_up(std::max<int64_t>(c.tune.chunk, tile), tile);
+    // Chunks are multiples of 768 = lcm(64 * dpt) and queue slices of 1536,
+    // so the datapoints of every (expression, chunk) partial sum, and the
+    // order each lane adds them in (lane + 64 k, k ascending), do not depend
+    // on dpt: MSEs are bit-identical across dpt for equal chunk and queues.
+    constexpr int64_t kChunkQuantum = 768;
+    lp.slice = round_up((c.N + lp.nq - 1) / lp.nq, 2 * kChunkQuantum);
+    lp.chunk = round_up(std::max<int64_t>(c.tune.chunk, kChunkQuantum), kChunkQuantum);
     if (lp.chunk > lp.slice) lp.chunk = lp.slice;
     lp.cpq = (int)((lp.slice + lp.chunk - 1) / lp.chunk);
     lp.P = lp.nq * lp.cpq;
@@ -558,6 +562,7 @@ int sr_set_tuning(sr_ctx *x, const sr_tuning *t)
     if (!x || !t) return SR_ERR_INVALID_ARG;
     Ctx &c = x->c;
     if (t->dpt < 1 || t->dpt > 4) return c.fail(SR_ERR_INVALID_ARG, "dpt must be 1, 2, 3 or 4");
+    if (t->cache_dpt < 0 || t->cache_dpt > 4) return c.fail(SR_ERR_INVALID_ARG, "cache_dpt must be 0..4");
     if (t->chunk < 64 || (t->n_queues != 1 && t->n_queues != 8) || t->blocks_per_cu < 0)
         return c.fail(SR_ERR_INVALID_ARG, "bad tuning");
     c.tune = *t;

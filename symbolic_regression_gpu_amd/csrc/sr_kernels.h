@@ -104,12 +104,14 @@ SR_DEV void interp_step(int code, int arg, float cv, int &sp, RefLoad &&ref, L &
 // read ahead past its end.
 constexpr int kProgLds = MAX_EXPR_LEN + 1;
 
-// Waves per SIMD the register allocation must leave room for: the 3-datapoint
-// variant runs 3 waves per SIMD (<= 168 VGPRs) in every mode.
-constexpr int batched_min_waves(int dpt) { return dpt == 3 ? 3 : 1; }
+// Waves per SIMD the register allocation must leave room for.  Occupancy
+// decides this latency-bound interpreter: the 3-datapoint variant runs 4 waves
+// per SIMD (<= 128 VGPRs, a few spills) for MSE and 3 for cache skeletons;
+// with 4-element variable arrays the 4-datapoint MSE variant fits 3 waves.
+constexpr int batched_min_waves(int dpt, int mode) { return dpt == 3 ? (mode == MODE_MSE ? 4 : 3) : 1; }
 
 template <int DPT, int S, int XC, int MODE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(batched_min_waves(DPT))))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(batched_min_waves(DPT, MODE))))
 void k_batched(BatchArgs a)
 {
     // Each wave copies its expression's program into LDS once per work item

@@ -1,14 +1,16 @@
 // sr_tune.cpp -- autotuner for the launch shape of the batched interpreter
 // and of the constant-fitting kernels.
 //
-// The batched launch has four knobs (sr_tuning): datapoints per lane (DPT:
-// 1, 2, 4 -- more registers per wave, fewer waves per SIMD), datapoints per
-// work item (chunk: LPT granularity vs. per-item program staging), XCD queues
-// (1 or 8) and persistent workgroups per CU (0 = occupancy limit).  The
+// The batched launch has these knobs (sr_tuning): datapoints per lane (DPT
+// 1..4: dispatch amortised over more datapoints vs. more registers per wave,
+// fewer waves per SIMD), datapoints per work item (chunk, multiples of 768:
+// LPT granularity vs. per-item program staging), XCD queues (1 or 8; tuned
+// runs keep 8) and persistent workgroups per CU (0 = occupancy limit).  The
 // tuner times every valid combination on a length-stratified sample of the
 // population (the interpreter kernel alone, HIP events), keeps the fastest,
 // and remembers it per (variable-register class, datapoint count), so later
-// calls with a similar shape reuse the result.
+// calls with a similar shape reuse the result.  cache_dpt (skeleton launches
+// of the subtree caches) is left as set.
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -77,16 +79,17 @@ int sr_autotune(sr_ctx *x, const sr_population *pop, int32_t max_sample, sr_tuni
     sr_tuning best = saved;
     double best_t = 1e300;
     const int dpts[] = {1, 2, 3, 4};
-    const int chunks[] = {3072, 4096, 6144, 8192, 12288, 16384, 24576, 32768};
-    const int queues[] = {1, 8};
+    const int chunks[] = {3072, 6144, 12288, 24576};
+    // one queue per XCD keeps each chunk's X in one L2; a small sample
+    // cannot tell the queue layouts apart, so it is not a tuned knob
+    const int queues[] = {8};
     const int bpcs[] = {0, 1};
     for (int dpt : dpts) {
         if ((xc == 32 && dpt > 2) || (dpt == 3 && xc != 4)) continue;
         for (int ch : chunks) {
-            if (ch % (64 * dpt)) continue;
             for (int nq : queues)
                 for (int bpc : bpcs) {
-                    c.tune = sr_tuning{dpt, ch, nq, bpc};
+                    c.tune = sr_tuning{dpt, ch, nq, bpc, saved.cache_dpt};
                     double t = 0.0;
                     for (int rep = 0; rep < 2; ++rep) {  // first run warms caches
                         sr_stats st{};

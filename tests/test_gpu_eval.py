@@ -82,6 +82,7 @@ def test_batched_tunings_agree(ctx):
     base, vbase, _ = ctx.eval_mse(api, EVAL_BATCHED)
     check_population(pop, X, 3, N, base, vbase)
     saved = ctx.tuning
+    by_shape = {}
     try:
         for dpt in (1, 2, 3, 4):
             for nq in (1, 8):
@@ -91,6 +92,10 @@ def test_batched_tunings_agree(ctx):
                     # same per-point values; only the fp64 partial-sum order moves
                     assert np.array_equal(v, vbase), (dpt, nq, chunk)
                     assert mse_close(m, base, 1e-12).all(), (dpt, nq, chunk)
+                    # chunk boundaries and per-lane summation order do not
+                    # depend on dpt: equal chunk and queues give equal bits
+                    ref = by_shape.setdefault((nq, chunk), m)
+                    assert np.array_equal(m, ref, equal_nan=True), (dpt, nq, chunk)
     finally:
         ctx.tuning = saved
 
@@ -156,7 +161,7 @@ def test_autotune_picks_a_valid_shape_and_keeps_results(ctx):
     try:
         base, vbase, _ = ctx.eval_mse(api, EVAL_BATCHED)
         best, ms = ctx.autotune(api, max_sample=64)
-        assert best.dpt in (1, 2, 3, 4) and best.n_queues in (1, 8) and best.chunk % (64 * best.dpt) == 0
+        assert best.dpt in (1, 2, 3, 4) and best.n_queues in (1, 8) and best.chunk % 768 == 0
         assert ms > 0 and ctx.tuning == best
         m, v, _ = ctx.eval_mse(api, EVAL_BATCHED)
         assert np.array_equal(v, vbase) and mse_close(m, base, 1e-12).all()
