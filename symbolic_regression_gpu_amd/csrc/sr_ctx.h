@@ -187,9 +187,12 @@ struct Ctx {
 
     sr_tuning tune{3, 3072, 8, 0, 3};
 
-    // staging (host pinned) and device buffers for one evaluation
+    // staging (host pinned) and device buffers for one evaluation; the
+    // second program set lets a launch be prepared while one is in flight
     HostBuf h_prog, h_meta, h_out;
     DevBuf d_prog, d_meta, d_psum, d_pcnt, d_mse, d_cnt, d_counters, d_pred;
+    HostBuf h_prog2, h_meta2, h_res[2];
+    DevBuf d_prog2, d_meta2;
 
     SubtreeCache cache;
     MemoState memo;
@@ -199,6 +202,7 @@ struct Ctx {
     // ev[0..3]: transfer / kernel / transfer brackets of one call;
     // ev[4..5]: the interpreter kernel alone.
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev2[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // program set 1
 
     int fail(int code, const std::string &msg)
     {
@@ -247,8 +251,15 @@ int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, 
 // Batched evaluation over a compiled program already described by host
 // vectors; uploads, launches, finalizes.  mode: MODE_MSE/MODE_PRED/MODE_COLUMN.
 // For MODE_MSE writes d_mse/d_cnt on the device (caller copies back).
+// set selects the staging / program buffers and events (0 or 1); with
+// wait = false the interpreter time is not read here: call
+// add_batched_timing once the stream has synchronised.
 int run_batched(Ctx &c, const Program &p, int mode, const float *cache_cols, float *out_dev,
-                const int32_t *out_col_host, sr_stats *st);
+                const int32_t *out_col_host, sr_stats *st, int set = 0, bool wait = true);
+void add_batched_timing(Ctx &c, int set, sr_stats *st);
+// Same checks as build_program without compiling (lengths, stream shape,
+// variable indices; OP_REF / OP_STORE rejected).
+int validate_population(Ctx &c, const sr_population &pop);
 
 }  // namespace sr
 

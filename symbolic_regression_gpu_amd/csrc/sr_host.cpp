@@ -178,8 +178,13 @@ int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, 
 }
 
 int run_batched(Ctx &c, const Program &p, int mode, const float *cache_cols, float *out_dev,
-                const int32_t *out_col_host, sr_stats *st)
+                const int32_t *out_col_host, sr_stats *st, int set, bool wait)
 {
+    HostBuf &h_prog = set ? c.h_prog2 : c.h_prog;
+    HostBuf &h_meta = set ? c.h_meta2 : c.h_meta;
+    DevBuf &d_prog = set ? c.d_prog2 : c.d_prog;
+    DevBuf &d_meta = set ? c.d_meta2 : c.d_meta;
+    hipEvent_t *ev = set ? c.ev2 : c.ev;
     const int E = (int)p.len.size();
     if (E == 0) return SR_OK;
     if (p.max_need > kStackSlots)
@@ -194,10 +199,10 @@ int run_batched(Ctx &c, const Program &p, int mode, const float *cache_cols, flo
     // a program built without cache slots has no OP_REF: nothing to prefetch
     const bool pf = has_ref(mode);
     const size_t meta_ints = (size_t)E * (pf ? 4 + kRefPrefetch : 4);
-    if ((he = c.h_prog.ensure(prog_bytes + 16)) != hipSuccess) return c.hip_fail(he, "hipHostMalloc");
-    if ((he = c.h_meta.ensure(meta_ints * 4)) != hipSuccess) return c.hip_fail(he, "hipHostMalloc");
-    std::memcpy(c.h_prog.p, p.ins.data(), prog_bytes);
-    int32_t *meta = c.h_meta.as<int32_t>();
+    if ((he = h_prog.ensure(prog_bytes + 16)) != hipSuccess) return c.hip_fail(he, "hipHostMalloc");
+    if ((he = h_meta.ensure(meta_ints * 4)) != hipSuccess) return c.hip_fail(he, "hipHostMalloc");
+    std::memcpy(h_prog.p, p.ins.data(), prog_bytes);
+    int32_t *meta = h_meta.as<int32_t>();
     std::memcpy(meta, p.off.data(), E * 4);
     std::memcpy(meta + E, p.len.data(), E * 4);
     std::memcpy(meta + 2 * E, p.order.data(), E * 4);
@@ -209,40 +214,76 @@ int run_batched(Ctx &c, const Program &p, int mode, const float *cache_cols, flo
         else
             std::fill(meta + 4 * E, meta + 4 * E + (size_t)E * kRefPrefetch, -1);
     }
-    if ((he = c.d_prog.ensure(prog_bytes + 16)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
-    if ((he = c.d_meta.ensure(meta_ints * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    if ((he = d_prog.ensure(prog_bytes + 16)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    if ((he = d_meta.ensure(meta_ints * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
     hipStream_t s = c.stream;
-    hipEventRecord(c.ev[0], s);
-    if ((he = hipMemcpyAsync(c.d_prog.p, c.h_prog.p, prog_bytes, hipMemcpyHostToDevice, s)) != hipSuccess)
+    hipEventRecord(ev[0], s);
+    if ((he = hipMemcpyAsync(d_prog.p, h_prog.p, prog_bytes, hipMemcpyHostToDevice, s)) != hipSuccess)
         return c.hip_fail(he, "H2D program");
-    if ((he = hipMemcpyAsync(c.d_meta.p, c.h_meta.p, meta_ints * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
+    if ((he = hipMemcpyAsync(d_meta.p, h_meta.p, meta_ints * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
         return c.hip_fail(he, "H2D metadata");
-    hipEventRecord(c.ev[1], s);
+    hipEventRecord(ev[1], s);
 
     BatchArgs a{};
-    a.progs = c.d_prog.as<Ins>();
-    a.prog_off = c.d_meta.as<int32_t>();
+    a.progs = d_prog.as<Ins>();
+    a.prog_off = d_meta.as<int32_t>();
     a.prog_len = a.prog_off + E;
     a.order = a.prog_off + 2 * E;
     a.out_col = a.prog_off + 3 * E;
     a.ref_pf = pf ? a.prog_off + 4 * E : nullptr;
     a.cache = cache_cols;
     a.out = out_dev;
-    rc = enqueue_batched(c, mode, lp, a, E, s, st ? c.ev[4] : nullptr, st ? c.ev[5] : nullptr);
+    rc = enqueue_batched(c, mode, lp, a, E, s, st ? ev[4] : nullptr, st ? ev[5] : nullptr);
     if (rc) return rc;
-    hipEventRecord(c.ev[2], s);
+    hipEventRecord(ev[2], s);
     if (st) {
         st->num_kernel_launches += is_mse_mode(mode) ? 3 : 2;
         st->executed_node_evals += p.tokens * c.N;
         st->executed_op_evals += p.ops * c.N;
         st->column_reads += p.refs * c.N;
         st->column_writes += (p.stores + (is_column_mode(mode) ? E : 0)) * c.N;
-        // every caller synchronises right after; waiting here costs nothing
-        if ((he = hipEventSynchronize(c.ev[5])) != hipSuccess) return c.hip_fail(he, "interpreter");
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, c.ev[4], c.ev[5]);
-        st->interp_kernel_ms += ms;
-        st->interp_launches += 1;
+        if (wait) {
+            // every caller synchronises right after; waiting here costs nothing
+            if ((he = hipEventSynchronize(ev[5])) != hipSuccess) return c.hip_fail(he, "interpreter");
+            add_batched_timing(c, set, st);
+        }
+    }
+    return SR_OK;
+}
+
+void add_batched_timing(Ctx &c, int set, sr_stats *st)
+{
+    hipEvent_t *ev = set ? c.ev2 : c.ev;
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, ev[4], ev[5]);
+    st->interp_kernel_ms += ms;
+    st->interp_launches += 1;
+}
+
+int validate_population(Ctx &c, const sr_population &pop)
+{
+    for (int e = 0; e < pop.num_exprs; ++e) {
+        const int32_t L = pop.lengths[e];
+        const int64_t o = pop.offsets[e];
+        const int32_t *tok = pop.tokens + o;
+        if (o < 0 || L <= 0 || L > MAX_EXPR_LEN)
+            return c.fail(SR_ERR_BAD_EXPR, "expression " + std::to_string(e) + ": length " +
+                                               std::to_string(L) + " outside 1.." +
+                                               std::to_string(MAX_EXPR_LEN));
+        if (raw_stack_depth(tok, L) < 0)
+            return c.fail(SR_ERR_BAD_EXPR,
+                          "expression " + std::to_string(e) + ": malformed prefix token stream");
+        for (int k = 0; k < L; ++k) {
+            if (tok[k] == OP_REF)
+                return c.fail(SR_ERR_BAD_EXPR, "expression " + std::to_string(e) +
+                                                   ": cache reference outside the cache");
+            if (tok[k] == TOK_VAR) {
+                const float v = pop.values ? (float)pop.values[o + k] : pop.values_f32[o + k];
+                if (!(v >= 0.0f))
+                    return c.fail(SR_ERR_BAD_EXPR, "expression " + std::to_string(e) +
+                                                       ": negative variable index");
+            }
+        }
     }
     return SR_OK;
 }
@@ -329,6 +370,7 @@ int sr_ctx_create(int device, sr_ctx **out)
     }
     c.stream = c.own_stream;
     for (auto &e : c.ev) hipEventCreate(&e);
+    for (auto &e : c.ev2) hipEventCreate(&e);
     *out = x;
     return SR_OK;
 }
@@ -340,6 +382,8 @@ int sr_ctx_destroy(sr_ctx *x)
     hipSetDevice(c.device);
     hipStreamSynchronize(c.stream);
     for (auto &e : c.ev)
+        if (e) hipEventDestroy(e);
+    for (auto &e : c.ev2)
         if (e) hipEventDestroy(e);
     if (c.own_stream) hipStreamDestroy(c.own_stream);
     delete x;

@@ -177,7 +177,7 @@ int MemoPlanner::alloc(uint64_t key, const int32_t *tok, const double *val, int 
     return s;
 }
 
-void MemoPlanner::plan(const sr_population &pop, MemoPlan &out)
+void MemoPlanner::plan(const sr_population &pop, MemoPlan &out, const std::function<void(int)> &launch_ready)
 {
     const int E = pop.num_exprs;
     const int ms = min_size_;
@@ -352,6 +352,7 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out)
     if (out.off.size() > 0) {
         out.launch_begin.push_back((int32_t)out.off.size());
         ++epoch_;  // launch 1's columns are readable in launch 2
+        if (launch_ready) launch_ready(out.num_launches() - 1);
     }
 
     // Launch 2: the expressions.
@@ -380,8 +381,11 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out)
         const int rs = emit(p, pop.lengths[e], root, store2);
         if (rs >= 0) root_prog.put((uint64_t)rs, out.src[e]);
     }
-    if ((int)out.off.size() > first2) out.launch_begin.push_back((int32_t)out.off.size());
     if (reused) std::swap(prev_bits_, cur_bits_);
+    if ((int)out.off.size() > first2) {
+        out.launch_begin.push_back((int32_t)out.off.size());
+        if (launch_ready) launch_ready(out.num_launches() - 1);
+    }
 }
 
 void MemoPlanner::commit(const MemoPlan &p, const float *mse, const int64_t *cnt)
@@ -430,45 +434,75 @@ int eval_memo(Ctx &c, const sr_population &pop, sr_stats *st, float *mse_out, in
         m.N = c.N;
     }
     auto t0 = std::chrono::steady_clock::now();
-    Program check;  // validation of the caller's streams
-    int rc = build_program(c, pop, true, check);
+    int rc = validate_population(c, pop);
     if (rc) return rc;
     MemoPlan &plan = m.plan;
-    m.planner.plan(pop, plan);
-    const double host_ms =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-
-    const int P = plan.num_programs();
-    m.mse.resize(P);
-    m.cnt.resize(P);
-    float kernel_ms = 0.f, xfer_ms = 0.f, t = 0.f;
-    for (int k = 0; k < plan.num_launches() && rc == SR_OK; ++k) {
+    // Each launch is built and enqueued as soon as the planner has emitted
+    // it, so launch 1 runs on the device while launch 2 is planned.  The two
+    // launches use separate program buffers and result staging; the partial
+    // buffers are sized for both before the first one is in flight.
+    int launched = 0, res_b[2] = {0, 0}, res_n[2] = {0, 0};
+    float *const cols = m.cols.as<float>();
+    auto launch = [&](int k) {
+        if (rc || launched >= 2) {
+            if (!rc) rc = c.fail(SR_ERR_INVALID_ARG, "more than two cache launches");
+            return;
+        }
+        const int set = launched;
         const int b = plan.launch_begin[k], n = plan.launch_begin[k + 1] - b;
         sr_population bp{n, plan.tok.data(), plan.val.data(), nullptr, plan.off.data() + b, plan.len.data() + b};
         Program pr;
         rc = build_program(c, bp, false, pr, m.planner.capacity());
-        if (rc == SR_OK)
-            rc = run_batched(c, pr, MODE_MSE_REF, m.cols.as<float>(), m.cols.as<float>(), nullptr, st);
-        if (rc) break;
-        he = hipMemcpyAsync(m.mse.data() + b, c.d_mse.p, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream);
-        if (he == hipSuccess)
-            he = hipMemcpyAsync(m.cnt.data() + b, c.d_cnt.p, (size_t)n * 8, hipMemcpyDeviceToHost, c.stream);
-        if (he == hipSuccess) he = hipEventRecord(c.ev[3], c.stream);
-        if (he == hipSuccess) he = hipStreamSynchronize(c.stream);
-        if (he != hipSuccess) {
-            rc = c.hip_fail(he, "generational cache evaluation");
-            break;
+        if (rc == SR_OK && set == 0) {
+            LaunchPlan lp;
+            rc = plan_batched(c, MODE_MSE_REF, std::max(n, pop.num_exprs), lp);
         }
-        hipEventElapsedTime(&t, c.ev[0], c.ev[1]);
-        xfer_ms += t;
-        hipEventElapsedTime(&t, c.ev[1], c.ev[2]);
-        kernel_ms += t;
-        hipEventElapsedTime(&t, c.ev[2], c.ev[3]);
-        xfer_ms += t;
-    }
+        if (rc == SR_OK) rc = run_batched(c, pr, MODE_MSE_REF, cols, cols, nullptr, st, set, false);
+        if (rc) return;
+        HostBuf &hr = c.h_res[set];
+        const size_t mbytes = ((size_t)n * 4 + 7) / 8 * 8;
+        hipError_t e2 = hr.ensure(mbytes + (size_t)n * 8);
+        if (e2 == hipSuccess)
+            e2 = hipMemcpyAsync(hr.p, c.d_mse.p, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream);
+        if (e2 == hipSuccess)
+            e2 = hipMemcpyAsync(hr.as<char>() + mbytes, c.d_cnt.p, (size_t)n * 8, hipMemcpyDeviceToHost,
+                                c.stream);
+        if (e2 == hipSuccess) e2 = hipEventRecord((set ? c.ev2 : c.ev)[3], c.stream);
+        if (e2 != hipSuccess) {
+            rc = c.hip_fail(e2, "generational cache evaluation");
+            return;
+        }
+        res_b[set] = b;
+        res_n[set] = n;
+        ++launched;
+    };
+    m.planner.plan(pop, plan, launch);
+    const double host_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (rc == SR_OK && launched > 0 && (he = hipStreamSynchronize(c.stream)) != hipSuccess)
+        rc = c.hip_fail(he, "generational cache evaluation");
     if (rc) {
+        if (launched > 0) (void)hipStreamSynchronize(c.stream);
         m.planner.forget();  // columns of this plan's slots may be unwritten
         return rc;
+    }
+    const int P = plan.num_programs();
+    m.mse.resize(P);
+    m.cnt.resize(P);
+    float kernel_ms = 0.f, xfer_ms = 0.f, t = 0.f;
+    for (int set = 0; set < launched; ++set) {
+        const HostBuf &hr = c.h_res[set];
+        const size_t mbytes = ((size_t)res_n[set] * 4 + 7) / 8 * 8;
+        std::memcpy(m.mse.data() + res_b[set], hr.p, (size_t)res_n[set] * 4);
+        std::memcpy(m.cnt.data() + res_b[set], hr.as<char>() + mbytes, (size_t)res_n[set] * 8);
+        hipEvent_t *ev = set ? c.ev2 : c.ev;
+        if (st) add_batched_timing(c, set, st);
+        hipEventElapsedTime(&t, ev[0], ev[1]);
+        xfer_ms += t;
+        hipEventElapsedTime(&t, ev[1], ev[2]);
+        kernel_ms += t;
+        hipEventElapsedTime(&t, ev[2], ev[3]);
+        xfer_ms += t;
     }
     m.planner.commit(plan, m.mse.data(), m.cnt.data());
     for (int e = 0; e < pop.num_exprs; ++e) {

@@ -18,6 +18,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include "../../include/sr_hip.h"
@@ -65,8 +66,10 @@ public:
     void forget();
     // Plan one population (validated prefix streams).  Slots handed out here
     // are written by the planned launches, which the caller must run in
-    // order (or call forget()).
-    void plan(const sr_population &pop, MemoPlan &out);
+    // order (or call forget()).  launch_ready(k), when given, is called as
+    // soon as launch k's programs are complete (launch 0 before launch 1 is
+    // planned, so the caller can start it on the device meanwhile).
+    void plan(const sr_population &pop, MemoPlan &out, const std::function<void(int)> &launch_ready = {});
     // MSE and finite count of every program of the last plan.
     void commit(const MemoPlan &p, const float *mse, const int64_t *cnt);
     // MSE and finite count remembered in slot s (a plan's src = -2 - s).
