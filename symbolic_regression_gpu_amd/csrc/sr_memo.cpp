@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <thread>
 
 #include "sr_ctx.h"
 #include "sr_kernels.h"
@@ -206,28 +207,53 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out, const std::funct
     sz_.resize(end);
     uint64_t *h = h_.data();
     int32_t *sz = sz_.data();
-    for (int e = 0; e < E; ++e)
-        strong_hash(tok + pop.offsets[e], vals + pop.offsets[e], pop.lengths[e], h + pop.offsets[e],
-                    sz + pop.offsets[e]);
 
-    // Mark the slots this population uses; evict the rest.
+    // Hash every position and mark the slots this population uses (threads
+    // over contiguous expression ranges; the slot map is only read), then
+    // evict the rest.
     ++gen_;
     const bool reused = policy_ == SR_CACHE_GENERATIONAL_REUSED;
     // keys met in this population, as a 2^21-bit filter (a false hit only
     // stores a column that was not needed)
     std::vector<uint64_t> &cur_keys = cur_bits_;
     if (reused) cur_keys.assign(kKeyBits / 64, 0);
-    for (int e = 0; e < E; ++e) {
-        const int64_t a = pop.offsets[e];
-        for (int64_t q = a; q < a + pop.lengths[e]; ++q) {
-            if (sz[q] < ms) continue;
-            const int s = slot_of_.find(h[q]);
-            if (s >= 0) seen_gen_[s] = gen_;
-            if (reused) {
-                const uint64_t b = h[q] >> (64 - kKeyBitsLog2);
-                cur_keys[b >> 6] |= 1ull << (b & 63);
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int nt = std::max(1, std::min({8, hw, E / 128}));
+    std::vector<std::vector<int32_t>> hits(nt);
+    std::vector<std::vector<uint64_t>> bits(reused && nt > 1 ? nt : 0);
+    auto hash_mark = [&](int t) {
+        const int e0 = (int)((int64_t)E * t / nt), e1 = (int)((int64_t)E * (t + 1) / nt);
+        uint64_t *bt = nullptr;
+        if (reused) {
+            if (nt > 1) bits[t].assign(kKeyBits / 64, 0);
+            bt = nt > 1 ? bits[t].data() : cur_keys.data();
+        }
+        for (int e = e0; e < e1; ++e) {
+            const int64_t a = pop.offsets[e];
+            strong_hash(tok + a, vals + a, pop.lengths[e], h + a, sz + a);
+            for (int64_t q = a; q < a + pop.lengths[e]; ++q) {
+                if (sz[q] < ms) continue;
+                const int s = slot_of_.find(h[q]);
+                if (s >= 0) hits[t].push_back(s);
+                if (bt) {
+                    const uint64_t b = h[q] >> (64 - kKeyBitsLog2);
+                    bt[b >> 6] |= 1ull << (b & 63);
+                }
             }
         }
+    };
+    if (nt == 1) {
+        hash_mark(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; ++t) pool.emplace_back(hash_mark, t);
+        hash_mark(0);
+        for (auto &th : pool) th.join();
+    }
+    for (int t = 0; t < nt; ++t) {
+        for (int32_t s : hits[t]) seen_gen_[s] = gen_;
+        if (reused && nt > 1)
+            for (size_t w = 0; w < cur_keys.size(); ++w) cur_keys[w] |= bits[t][w];
     }
     bool evicted = false;
     for (int s = 0; s < used_; ++s) {
