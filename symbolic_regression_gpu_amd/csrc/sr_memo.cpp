@@ -335,6 +335,14 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out, const std::funct
     FlatMap &root_prog = root_prog_;  // root slot -> program index
     root_prog.reset(1024);
 
+    // Store policy for a column-less subtree met once: every one, or (REUSED)
+    // one whose key the previous population held.
+    auto store_once = [&](int64_t q) {
+        if (!reused) return true;
+        if (prev_bits_.empty()) return false;
+        const uint64_t b = h[q] >> (64 - kKeyBitsLog2);
+        return ((prev_bits_[b >> 6] >> (b & 63)) & 1) != 0;
+    };
     // Launch 1: column-less subtrees met at least twice among the executed
     // positions, largest first; a program also stores the new subtrees nested
     // in it, which are then skipped.
@@ -371,7 +379,9 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out, const std::funct
             if (full()) break;
             const int64_t q = cd.second;
             if (lookup(q) != -1) continue;  // stored by a larger program already
-            const int rs = emit(q, sz[q], -1, [](int64_t) { return true; });
+            // the program's root is stored (that is its purpose); nested
+            // new subtrees follow the policy
+            const int rs = emit(q, sz[q], -1, [&](int64_t r) { return r == q || store_once(r); });
             if (rs >= 0) root_prog.put((uint64_t)rs, (int32_t)out.off.size() - 1);
         }
     }
@@ -383,12 +393,6 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out, const std::funct
 
     // Launch 2: the expressions.
     const int first2 = (int)out.off.size();
-    auto store2 = [&](int64_t q) {
-        if (!reused) return true;
-        if (prev_bits_.empty()) return false;
-        const uint64_t b = h[q] >> (64 - kKeyBitsLog2);
-        return ((prev_bits_[b >> 6] >> (b & 63)) & 1) != 0;
-    };
     for (int e = 0; e < E; ++e) {
         if (out.src[e] <= -2) continue;
         const int64_t p = pop.offsets[e];
@@ -404,7 +408,7 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out, const std::funct
             }
         }
         out.src[e] = (int32_t)out.off.size();
-        const int rs = emit(p, pop.lengths[e], root, store2);
+        const int rs = emit(p, pop.lengths[e], root, store_once);
         if (rs >= 0) root_prog.put((uint64_t)rs, out.src[e]);
     }
     if (reused) std::swap(prev_bits_, cur_bits_);
