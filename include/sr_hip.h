@@ -45,7 +45,8 @@ typedef enum {
     SR_ERR_CAPACITY = -5,  /* a fixed-capacity buffer would overflow */
     SR_ERR_JIT = -6,       /* hipRTC compile / load failure */
     SR_ERR_UNSUPPORTED = -7,
-    SR_ERR_NO_DEVICE = -8
+    SR_ERR_NO_DEVICE = -8,
+    SR_ERR_IO = -9         /* file missing or unreadable / unwritable */
 } sr_status;
 
 typedef enum {
@@ -304,6 +305,32 @@ int sr_tree_splice(sr_ctx *ctx, int32_t n, int32_t cap, const int32_t *ltok, con
                    const int32_t *rtok, const float *rval, const int32_t *rlen, int32_t n_right,
                    const int32_t *rsel, const int32_t *ridx, int32_t *otok, float *oval,
                    int32_t *olen);
+
+/* ---- on-disk formats (host only, no device needed) -----------------------
+ * Digest (gen_k.txt) as parse_input_info reads it (utils.cpp:122-233): the
+ * expression count, then per expression "num_vars num_dps num_tokens", the
+ * tokens, the values and the data-file path on its own line; shared_data = 1
+ * when there are several expressions and all name the same file, variable and
+ * datapoint count (utils.cpp:202-223).  Numbers parse as the reference's
+ * stream reads do; a truncated or malformed digest is SR_ERR_BAD_EXPR. */
+typedef struct sr_digest sr_digest;
+int sr_digest_read(const char *path, sr_digest **out);
+int sr_digest_free(sr_digest *d);
+int sr_digest_shape(const sr_digest *d, int32_t *num_exprs, int64_t *num_tokens, int32_t *shared_data);
+/* Packed like sr_population (offsets into tokens / values); any pointer may
+ * be NULL. */
+int sr_digest_population(const sr_digest *d, int32_t *tokens, double *values, int32_t *offsets,
+                         int32_t *lengths, int32_t *num_vars, int32_t *num_dps);
+int sr_digest_data_path(const sr_digest *d, int32_t expr, char *out, int64_t cap, int64_t *len);
+/* The same format, values printed with 17 significant digits (exact). */
+int sr_digest_write(const char *path, const sr_population *pop, const int32_t *num_vars,
+                    const int32_t *num_dps, const char *const *data_paths);
+/* Data file as load_data_file reads it (utils.cpp:235-300): num_dps rows of
+ * num_vars + 1 numbers (x..., y) parsed with strtod; out is column-major
+ * double [num_vars+1][num_dps] (the reference's vars[i][j]); parsed = numbers
+ * actually converted (a short file leaves zeros, as in the reference). */
+int sr_data_file_read(const char *path, int32_t num_vars, int64_t num_dps, double *out, int64_t *parsed);
+int sr_data_file_write(const char *path, int32_t num_vars, int64_t num_dps, const double *cols);
 
 #ifdef __cplusplus
 }

@@ -124,6 +124,10 @@ def ref():
         R.ref_eval_point32.argtypes = [_i32p, _f64p, C.c_int, _f32p, C.c_int]
         R.ref_mse32.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, _f32p, C.c_int, C.c_int64,
                                 _f32p]
+        R.ref_parse_digest.restype = C.c_int
+        R.ref_parse_digest.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 5 + \
+            [C.c_char_p, C.POINTER(C.c_int)]
+        R.ref_load_data.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_void_p]
         R.ref_hash_flat.restype = C.c_uint64
         R.ref_hash_flat.argtypes = [_i32p, _f64p, C.c_int]
         R.ref_detect.restype = C.c_int

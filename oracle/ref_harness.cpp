@@ -213,4 +213,42 @@ int ref_ga_op(int op, int which, int seed, const sro_ga_params *p, const int *at
     return sro_ga_crossover(p, at, av, alen, bt, bv, blen, &r, tok, val);
 }
 
+// parse_input_info (utils.cpp:122-233) of a digest file.  Returns the
+// expression count (0 on the reference's own failure) and fills the arrays
+// when they are large enough: per expression nv / ndps / ntok, packed
+// tokens and values, paths at path_stride bytes each, the shared flag.
+int ref_parse_digest(const char *path, int cap_exprs, int cap_tokens, int path_stride, int *nv, int *ndps,
+                     int *ntok, int *tokens, double *values, char *paths, int *shared)
+{
+    InputInfo info = parse_input_info(path);
+    const int E = info.num_exprs;
+    if (E > 0 && E <= cap_exprs) {
+        int t = 0;
+        for (int e = 0; e < E; ++e) {
+            nv[e] = info.num_vars[e];
+            ndps[e] = info.num_dps[e];
+            ntok[e] = info.num_tokens[e];
+            for (int k = 0; k < ntok[e] && t < cap_tokens; ++k, ++t) {
+                tokens[t] = info.tokens[e][k];
+                values[t] = info.values[e][k];
+            }
+            std::snprintf(paths + (size_t)e * path_stride, path_stride, "%s", info.data_filenames[e].c_str());
+        }
+        *shared = info.has_shared_data ? 1 : 0;
+    }
+    if (E > 0) free_input_info(info);
+    return E;
+}
+
+// load_data_file (utils.cpp:235-300): out = column-major [nv+1][ndps].
+void ref_load_data(const char *path, int nv, int ndps, double *out)
+{
+    double **v = load_data_file(path, nv, ndps);
+    for (int i = 0; i <= nv; ++i) {
+        for (int j = 0; j < ndps; ++j) out[(size_t)i * ndps + j] = v[i][j];
+        free(v[i]);
+    }
+    free(v);
+}
+
 }  // extern "C"

@@ -119,6 +119,14 @@ _SIGNATURES = {
     "sr_cache_config": [_vp, C.c_int32, C.c_int32, C.c_int32],
     "sr_cache_clear": [_vp],
     "sr_cache_policy": [_vp, C.c_int32, C.c_int32, C.c_int32],
+    "sr_digest_read": [C.c_char_p, C.POINTER(_vp)],
+    "sr_digest_free": [_vp],
+    "sr_digest_shape": [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int32)],
+    "sr_digest_population": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "sr_digest_data_path": [_vp, C.c_int32, _vp, C.c_int64, C.POINTER(C.c_int64)],
+    "sr_digest_write": [C.c_char_p, C.POINTER(_Pop), _vp, _vp, _vp],
+    "sr_data_file_read": [C.c_char_p, C.c_int32, C.c_int64, _vp, C.POINTER(C.c_int64)],
+    "sr_data_file_write": [C.c_char_p, C.c_int32, C.c_int64, _vp],
     "sr_memo_planner_create": [C.c_int32, C.c_int32, C.c_int32, C.POINTER(_vp)],
     "sr_memo_planner_destroy": [_vp],
     "sr_memo_plan": [_vp, C.POINTER(_Pop), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -235,6 +243,78 @@ def jit_code(pop, nv: int, dpt: int = 4) -> bytes:
     if rc != 0:
         raise SrError(rc, "jit code")
     return buf.raw[:n.value]
+
+
+@dataclass
+class Digest:
+    """A digest file (gen_k.txt) as parse_input_info reads it."""
+    population: "Population"
+    num_vars: np.ndarray
+    num_dps: np.ndarray
+    data_paths: list
+    shared: bool
+
+
+def read_digest(path: str) -> Digest:
+    """Parse a digest file (utils.cpp:122-233 format); raises SrError."""
+    L = lib()
+    h = _vp()
+    rc = L.sr_digest_read(os.fsencode(path), C.byref(h))
+    if rc != 0:
+        raise SrError(rc, f"sr_digest_read({path})")
+    try:
+        E, T, sh = C.c_int32(), C.c_int64(), C.c_int32()
+        L.sr_digest_shape(h, C.byref(E), C.byref(T), C.byref(sh))
+        tok = np.empty(T.value, np.int32)
+        val = np.empty(T.value, np.float64)
+        off = np.empty(E.value, np.int32)
+        ln = np.empty(E.value, np.int32)
+        nv = np.empty(E.value, np.int32)
+        nd = np.empty(E.value, np.int32)
+        L.sr_digest_population(h, tok.ctypes.data, val.ctypes.data, off.ctypes.data, ln.ctypes.data,
+                               nv.ctypes.data, nd.ctypes.data)
+        paths = []
+        for e in range(E.value):
+            n = C.c_int64()
+            L.sr_digest_data_path(h, e, None, 0, C.byref(n))
+            buf = C.create_string_buffer(n.value + 1)
+            L.sr_digest_data_path(h, e, buf, n.value + 1, C.byref(n))
+            paths.append(buf.value.decode())
+        return Digest(Population(tok, val, off, ln), nv, nd, paths, bool(sh.value))
+    finally:
+        L.sr_digest_free(h)
+
+
+def write_digest(path: str, pop: "Population", num_vars, num_dps, data_paths):
+    """Write pop in the digest format (values with 17 significant digits)."""
+    E = pop.num_exprs
+    nv = np.ascontiguousarray(np.broadcast_to(num_vars, (E,)), np.int32)
+    nd = np.ascontiguousarray(np.broadcast_to(num_dps, (E,)), np.int32)
+    paths = [data_paths] * E if isinstance(data_paths, str) else list(data_paths)
+    arr = (C.c_char_p * E)(*[p.encode() for p in paths])
+    rc = lib().sr_digest_write(os.fsencode(path), C.byref(pop._c()), nv.ctypes.data, nd.ctypes.data,
+                               C.cast(arr, _vp))
+    if rc != 0:
+        raise SrError(rc, f"sr_digest_write({path})")
+
+
+def read_data_file(path: str, num_vars: int, num_dps: int):
+    """A data file as load_data_file reads it (utils.cpp:235-300):
+    (float64 [num_vars+1, num_dps] column-major X|y, numbers converted)."""
+    out = np.empty((num_vars + 1) * num_dps, np.float64)
+    n = C.c_int64()
+    rc = lib().sr_data_file_read(os.fsencode(path), num_vars, num_dps, out.ctypes.data, C.byref(n))
+    if rc != 0:
+        raise SrError(rc, f"sr_data_file_read({path})")
+    return out.reshape(num_vars + 1, num_dps), n.value
+
+
+def write_data_file(path: str, cols):
+    """Write X|y columns [nv+1, N] as N rows of nv+1 numbers."""
+    c = np.ascontiguousarray(cols, np.float64)
+    rc = lib().sr_data_file_write(os.fsencode(path), c.shape[0] - 1, c.shape[1], c.ctypes.data)
+    if rc != 0:
+        raise SrError(rc, f"sr_data_file_write({path})")
 
 
 class MemoPlanner:

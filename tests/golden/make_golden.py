@@ -13,6 +13,11 @@ Each fixture holds inputs and the reference's outputs on seeded data:
                flat hashes of detected subtrees, hint map
   ga.npz       generate_random_expression / mutate_expression /
                crossover_expressions (utils.cpp:1156-1272) under set_seed(k)
+  formats.npz  parse_input_info (utils.cpp:122-233) and load_data_file
+               (utils.cpp:235-300) on the files under formats/: the
+               reference's example digest and data files (data/examples/,
+               copied as fixtures) and two synthetic files this script writes
+               (odd spacing, exponents, signed zeros, CRLF rows)
 """
 import ctypes as C
 import os
@@ -39,8 +44,71 @@ def population(n, seed, nv, max_tokens, max_depth):
                          for i in range(n)])
 
 
+def write_synthetic_formats():
+    """Digest and data files in the reference's formats with awkward but valid
+    number spellings; written once here and committed beside the outputs."""
+    fdir = os.path.join(HERE, "formats")
+    rng = np.random.default_rng(77)
+    with open(os.path.join(fdir, "synth_digest.txt"), "w") as f:
+        f.write("3\n")
+        rows = [
+            (2, 64, [1, -1, 10, -1], ["0", "1", "0", "0.0"]),
+            (2, 64, [3, 0, 2, -1, 0], ["0", "-0.0", "0", "1", "1.5e-300"]),
+            (2, 64, [8, 0, 0], ["0", "3.141592653589793238", "-2.5E+7"]),
+        ]
+        for nv, ndps, tok, val in rows:
+            f.write(f"{nv} {ndps} {len(tok)}\n")
+            f.write(" ".join(map(str, tok)) + "\n")
+            f.write("  ".join(val) + "\n")
+            f.write("data/synth_data.txt\n")
+    with open(os.path.join(fdir, "synth_data.txt"), "w", newline="") as f:
+        for j in range(64):
+            x = rng.uniform(-3, 3, size=3)
+            cells = [f"{x[0]:.17g}", f"{x[1]:.6e}", f"{x[2]:.3f}"]
+            f.write("\t".join(cells) + ("\r\n" if j % 3 == 0 else "\n"))
+
+
+def ref_digest(R, path, cap_e=64, cap_t=4096, stride=256):
+    nv = np.zeros(cap_e, np.int32)
+    nd = np.zeros(cap_e, np.int32)
+    nt = np.zeros(cap_e, np.int32)
+    tok = np.zeros(cap_t, np.int32)
+    val = np.zeros(cap_t, np.float64)
+    paths = C.create_string_buffer(cap_e * stride)
+    shared = C.c_int()
+    E = R.ref_parse_digest(path.encode(), cap_e, cap_t, stride, nv.ctypes.data, nd.ctypes.data,
+                           nt.ctypes.data, tok.ctypes.data, val.ctypes.data, paths, C.byref(shared))
+    T = int(nt[:E].sum())
+    names = [paths.raw[e * stride:(e + 1) * stride].split(b"\0")[0].decode() for e in range(E)]
+    return {"nv": nv[:E], "ndps": nd[:E], "ntok": nt[:E], "tok": tok[:T], "val": val[:T],
+            "paths": np.array(names), "shared": shared.value}
+
+
+def ref_data(R, path, nv, ndps):
+    out = np.zeros((nv + 1) * ndps, np.float64)
+    R.ref_load_data(path.encode(), nv, ndps, out.ctypes.data)
+    return out.reshape(nv + 1, ndps)
+
+
+def make_formats(R):
+    fdir = os.path.join(HERE, "formats")
+    write_synthetic_formats()
+    out = {}
+    for tag, name in (("sample", "sample_input.txt"), ("synth", "synth_digest.txt")):
+        d = ref_digest(R, os.path.join(fdir, name))
+        out.update({f"{tag}_{k}": v for k, v in d.items()})
+    for tag, name, nv, ndps in (("s1", "sample_1", 1, 1000), ("s2", "sample_2", 3, 1000),
+                                ("sd", "synth_data.txt", 2, 64), ("short", "synth_data.txt", 2, 70)):
+        out[f"data_{tag}"] = ref_data(R, os.path.join(fdir, name), nv, ndps)
+    np.savez(os.path.join(HERE, "formats.npz"), **out)
+
+
 def main():
     R = O.ref()
+    if sys.argv[1:] == ["formats"]:
+        make_formats(R)
+        print("formats fixture written to", HERE)
+        return
     rng = np.random.default_rng(2024)
     # ---- op table -------------------------------------------------------
     special = np.array([0.0, -0.0, 1.0, -1.0, 0.5, -0.5, 2.0, 1e-9, -1e-9, 5e-10, -5e-10, 1e-10,
@@ -154,6 +222,7 @@ def main():
              log_hi=np.concatenate([r[8] for r in rows]), log_res=np.concatenate([r[9] for r in rows]),
              p_tok=parents.tok, p_val=parents.val, p_off=parents.off, p_len=parents.len,
              params=np.array([3, 48, 6], np.int32), consts=np.array([-2.0, 2.0], np.float32))
+    make_formats(R)
     print("golden fixtures written to", HERE)
 
 
