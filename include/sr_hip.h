@@ -306,6 +306,25 @@ int sr_tree_splice(sr_ctx *ctx, int32_t n, int32_t cap, const int32_t *ltok, con
                    const int32_t *rsel, const int32_t *ridx, int32_t *otok, float *oval,
                    int32_t *olen);
 
+/* ---- evogp population layout (defs.h:30-107, gpu_kernel.h:15-58) ---------
+ * Per individual, max_len slots of value (float: constant, variable index or
+ * opcode), node type (int16: VAR 0, CONST 1, UFUNC 2, BFUNC 3; bit 7, the
+ * output flag, is ignored) and subtree size (int16; size[0] = tree length),
+ * in prefix order.  Device pointers; the context's stream, synchronous.
+ * evogp -> prefix restates gpu_simple_jinha_with_evolve.cu:846-866 (a
+ * function node's value is its opcode); an individual whose size[0] is
+ * outside 1..min(max_len, cap) gets length 0.  prefix -> evogp computes the
+ * node types from token arity and the subtree sizes; a stream longer than
+ * max_len or malformed gets size[0] = 0.  The GA forms read / write the
+ * context's device population (sr_ga_init sets its stride). */
+int sr_evogp_to_prefix(sr_ctx *ctx, int32_t pop, int32_t max_len, const float *d_value, const int16_t *d_type,
+                       const int16_t *d_size, int32_t cap, int32_t *d_tok, float *d_val, int32_t *d_len);
+int sr_prefix_to_evogp(sr_ctx *ctx, int32_t pop, int32_t cap, const int32_t *d_tok, const float *d_val,
+                       const int32_t *d_len, int32_t max_len, float *d_value, int16_t *d_type, int16_t *d_size);
+int sr_ga_import_evogp(sr_ctx *ctx, int32_t pop, int32_t max_len, const float *d_value, const int16_t *d_type,
+                       const int16_t *d_size);
+int sr_ga_export_evogp(sr_ctx *ctx, int32_t max_len, float *d_value, int16_t *d_type, int16_t *d_size);
+
 /* ---- on-disk formats (host only, no device needed) -----------------------
  * Digest (gen_k.txt) as parse_input_info reads it (utils.cpp:122-233): the
  * expression count, then per expression "num_vars num_dps num_tokens", the

@@ -101,6 +101,9 @@ def lib():
         L.sro_ga_init_population.argtypes = [C.POINTER(GaParams), C.c_int, C.c_int, C.c_uint64,
                                              _i32p, _f32p, _i32p]
         L.sro_fitness_order.argtypes = [_f32p, C.c_int, _i32p]
+        _vp = C.c_void_p
+        L.sro_evogp_to_prefix.argtypes = [C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp]
+        L.sro_prefix_to_evogp.argtypes = [C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp]
         L.sro_ga_subtree_size.argtypes = [_i32p, C.c_int, C.c_int]
         _bind_ga_hooks(L)
         _lib = L
@@ -393,6 +396,32 @@ def _bind_ga_hooks(L):
                                 _i32p, _f32p]
     L.sro_ga_crossover.argtypes = [C.POINTER(GaParams), _i32p, _f32p, C.c_int, _i32p, _f32p, C.c_int,
                                    C.POINTER(RngHooks), _i32p, _f32p]
+
+
+def evogp_to_prefix(value, type_, size, max_len, cap):
+    """evogp arrays [pop * max_len] -> (tok, val [pop * cap], len [pop])."""
+    pop = len(value) // max_len
+    tok = np.zeros(pop * cap, np.int32)
+    val = np.zeros(pop * cap, np.float32)
+    ln = np.zeros(pop, np.int32)
+    lib().sro_evogp_to_prefix(pop, max_len, np.ascontiguousarray(value, np.float32).ctypes.data,
+                              np.ascontiguousarray(type_, np.int16).ctypes.data,
+                              np.ascontiguousarray(size, np.int16).ctypes.data, cap, tok.ctypes.data,
+                              val.ctypes.data, ln.ctypes.data)
+    return tok, val, ln
+
+
+def prefix_to_evogp(tok, val, ln, cap, max_len):
+    """Stride-cap prefix population -> evogp arrays [pop * max_len]."""
+    pop = len(ln)
+    value = np.zeros(pop * max_len, np.float32)
+    type_ = np.zeros(pop * max_len, np.int16)
+    size = np.zeros(pop * max_len, np.int16)
+    lib().sro_prefix_to_evogp(pop, cap, np.ascontiguousarray(tok, np.int32).ctypes.data,
+                              np.ascontiguousarray(val, np.float32).ctypes.data,
+                              np.ascontiguousarray(ln, np.int32).ctypes.data, max_len, value.ctypes.data,
+                              type_.ctypes.data, size.ctypes.data)
+    return value, type_, size
 
 
 def fitness_order(fit):

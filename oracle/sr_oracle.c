@@ -1405,3 +1405,72 @@ int sro_ga_init_population(const sro_ga_params *p, int pop, int cap, uint64_t se
     }
     return rc;
 }
+
+/* ---- evogp layout --------------------------------------------------------- */
+void sro_evogp_to_prefix(int pop, int max_len, const float *value, const short *type, const short *size,
+                         int cap, int *tok, float *val, int *len)
+{
+    for (int i = 0; i < pop; ++i) {
+        const float *vi = value + (size_t)i * max_len;
+        const short *ti = type + (size_t)i * max_len;
+        const int n = size[(size_t)i * max_len];
+        int *to = tok + (size_t)i * cap;
+        float *vo = val + (size_t)i * cap;
+        if (n <= 0 || n > max_len || n > cap) {
+            len[i] = 0;
+            continue;
+        }
+        for (int t = 0; t < n; ++t) {
+            const int kind = ti[t] & 0x7f;
+            if (kind == 1) {        /* CONST */
+                to[t] = 0;
+                vo[t] = vi[t];
+            } else if (kind == 0) { /* VAR */
+                to[t] = -1;
+                vo[t] = vi[t];
+            } else {                /* function: the opcode is the value */
+                to[t] = (int)vi[t];
+                vo[t] = 0.0f;
+            }
+        }
+        len[i] = n;
+    }
+}
+
+/* End of the subtree starting at t (one past it), or -1 past the stream. */
+static int evogp_extent(const int *tok, int n, int t)
+{
+    int need = 1, k = t;
+    while (need > 0) {
+        if (k >= n) return -1;
+        const int x = tok[k++];
+        need += (x <= 0 ? 0 : (x < 10 ? 2 : 1)) - 1;
+    }
+    return k;
+}
+
+void sro_prefix_to_evogp(int pop, int cap, const int *tok, const float *val, const int *len, int max_len,
+                         float *value, short *type, short *size)
+{
+    for (int i = 0; i < pop; ++i) {
+        const int *ti = tok + (size_t)i * cap;
+        const float *vi = val + (size_t)i * cap;
+        float *vo = value + (size_t)i * max_len;
+        short *to = type + (size_t)i * max_len;
+        short *so = size + (size_t)i * max_len;
+        const int n = len[i] < cap ? len[i] : cap;
+        int bad = n <= 0 || n > max_len || n > 32767;
+        for (int t = 0; t < n && !bad; ++t)
+            bad = !(ti[t] == 0 || ti[t] == -1 || (ti[t] >= 1 && ti[t] <= 27) || ti[t] == 29);
+        if (bad || evogp_extent(ti, n, 0) != n) {
+            so[0] = 0;
+            continue;
+        }
+        for (int t = 0; t < n; ++t) {
+            const int x = ti[t];
+            to[t] = (short)(x == 0 ? 1 : x == -1 ? 0 : (x < 10 && x > 0) ? 3 : 2);
+            vo[t] = (x == 0 || x == -1) ? vi[t] : (float)x;
+            so[t] = (short)(evogp_extent(ti, n, t) - t);
+        }
+    }
+}

@@ -222,6 +222,17 @@ int sro_ga_next_generation(const sro_ga_params *p, const sro_select_params *sp, 
 int sro_ga_init_population(const sro_ga_params *p, int pop, int cap, uint64_t seed,
                            int *out_tok, float *out_val, int *out_len);
 
+/* evogp layout (defs.h:30-107): per individual max_len slots of value,
+ * node type (VAR 0, CONST 1, UFUNC 2, BFUNC 3; bit 7 ignored) and subtree
+ * size.  To prefix: gpu_simple_jinha_with_evolve.cu:846-866 (a function
+ * node's value is its opcode); size[0] outside 1..min(max_len, cap) -> length
+ * 0.  From prefix: types from token arity, sizes by brute-force subtree
+ * extent; an over-long or malformed stream -> size[0] = 0. */
+void sro_evogp_to_prefix(int pop, int max_len, const float *value, const short *type, const short *size,
+                         int cap, int *tok, float *val, int *len);
+void sro_prefix_to_evogp(int pop, int cap, const int *tok, const float *val, const int *len, int max_len,
+                         float *value, short *type, short *size);
+
 #ifdef __cplusplus
 }
 #endif

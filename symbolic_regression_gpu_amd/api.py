@@ -127,6 +127,10 @@ _SIGNATURES = {
     "sr_digest_write": [C.c_char_p, C.POINTER(_Pop), _vp, _vp, _vp],
     "sr_data_file_read": [C.c_char_p, C.c_int32, C.c_int64, _vp, C.POINTER(C.c_int64)],
     "sr_data_file_write": [C.c_char_p, C.c_int32, C.c_int64, _vp],
+    "sr_evogp_to_prefix": [_vp, C.c_int32, C.c_int32, _vp, _vp, _vp, C.c_int32, _vp, _vp, _vp],
+    "sr_prefix_to_evogp": [_vp, C.c_int32, C.c_int32, _vp, _vp, _vp, C.c_int32, _vp, _vp, _vp],
+    "sr_ga_import_evogp": [_vp, C.c_int32, C.c_int32, _vp, _vp, _vp],
+    "sr_ga_export_evogp": [_vp, C.c_int32, _vp, _vp, _vp],
     "sr_memo_planner_create": [C.c_int32, C.c_int32, C.c_int32, C.POINTER(_vp)],
     "sr_memo_planner_destroy": [_vp],
     "sr_memo_plan": [_vp, C.POINTER(_Pop), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -616,6 +620,51 @@ class Context:
         self._check(self._L.sr_ga_run(self._h, C.byref(sel), mode, gens, gen0, seed,
                                       int(use_graph), hist.ctypes.data, C.byref(ms), C.byref(st)))
         return hist[:gens * pop].reshape(gens, pop), ms.value, st.as_dict()
+
+    # ---- evogp population layout (device tensors) ---------------------------
+    def evogp_to_prefix(self, value, type_, size, cap: int):
+        """evogp arrays (CUDA tensors [pop, max_len]: float32 value, int16 type,
+        int16 subtree size) -> (tok int32 [pop, cap], val float32, len int32)."""
+        import torch
+        pop, max_len = value.shape
+        tok = torch.zeros((pop, cap), dtype=torch.int32, device=value.device)
+        val = torch.zeros((pop, cap), dtype=torch.float32, device=value.device)
+        ln = torch.zeros(pop, dtype=torch.int32, device=value.device)
+        torch.cuda.synchronize(value.device)
+        self._check(self._L.sr_evogp_to_prefix(self._h, pop, max_len, value.data_ptr(), type_.data_ptr(),
+                                               size.data_ptr(), cap, tok.data_ptr(), val.data_ptr(),
+                                               ln.data_ptr()))
+        return tok, val, ln
+
+    def prefix_to_evogp(self, tok, val, ln, max_len: int):
+        """Stride population (CUDA tensors [pop, cap]) -> evogp arrays."""
+        import torch
+        pop, cap = tok.shape
+        value = torch.zeros((pop, max_len), dtype=torch.float32, device=tok.device)
+        type_ = torch.zeros((pop, max_len), dtype=torch.int16, device=tok.device)
+        size = torch.zeros((pop, max_len), dtype=torch.int16, device=tok.device)
+        torch.cuda.synchronize(tok.device)
+        self._check(self._L.sr_prefix_to_evogp(self._h, pop, cap, tok.data_ptr(), val.data_ptr(), ln.data_ptr(),
+                                               max_len, value.data_ptr(), type_.data_ptr(), size.data_ptr()))
+        return value, type_, size
+
+    def ga_import_evogp(self, value, type_, size):
+        import torch
+        torch.cuda.synchronize(value.device)
+        pop, max_len = value.shape
+        self._check(self._L.sr_ga_import_evogp(self._h, pop, max_len, value.data_ptr(), type_.data_ptr(),
+                                               size.data_ptr()))
+
+    def ga_export_evogp(self, max_len: int):
+        import torch
+        pop, _ = self._ga_shape()
+        value = torch.zeros((pop, max_len), dtype=torch.float32, device="cuda")
+        type_ = torch.zeros((pop, max_len), dtype=torch.int16, device="cuda")
+        size = torch.zeros((pop, max_len), dtype=torch.int16, device="cuda")
+        torch.cuda.synchronize()
+        self._check(self._L.sr_ga_export_evogp(self._h, max_len, value.data_ptr(), type_.data_ptr(),
+                                               size.data_ptr()))
+        return value, type_, size
 
     # ---- constant fitting (device pointers) ---------------------------------
     def param_prepare(self, pop: Population, const_slot):

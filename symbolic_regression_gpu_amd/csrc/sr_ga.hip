@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "sr_ga.h"
+#include "sr_evogp.h"
 #include "sr_ga_launch.h"
 
 namespace sr {
@@ -259,6 +260,43 @@ hipError_t launch_splice(int n, int cap, const int32_t *ltok, const float *lval,
 {
     hipLaunchKernelGGL(k_splice, dim3(blocks(n, 64)), dim3(64), 0, s, n, cap, ltok, lval, llen, lsel,
                        lidx, rtok, rval, rlen, rsel, ridx, otok, oval, olen);
+    return hipGetLastError();
+}
+
+// evogp layout <-> prefix streams, one thread per individual.
+__global__ void k_evogp_to_prefix(int pop, int max_len, const float *value, const int16_t *type,
+                                  const int16_t *size, int cap, int32_t *tok, float *val, int32_t *len)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pop) return;
+    const size_t a = (size_t)i * max_len, b = (size_t)i * cap;
+    len[i] = evogp_to_prefix_one(value + a, type + a, size + a, max_len, cap, tok + b, val + b);
+}
+
+__global__ void k_prefix_to_evogp(int pop, int cap, const int32_t *tok, const float *val, const int32_t *len,
+                                  int max_len, float *value, int16_t *type, int16_t *size)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pop) return;
+    const size_t a = (size_t)i * cap, b = (size_t)i * max_len;
+    const int n = len[i] < cap ? len[i] : cap;
+    prefix_to_evogp_one(tok + a, val + a, n, max_len, value + b, type + b, size + b);
+}
+
+hipError_t launch_evogp_to_prefix(int pop, int max_len, const float *value, const int16_t *type,
+                                  const int16_t *size, int cap, int32_t *tok, float *val, int32_t *len,
+                                  hipStream_t s)
+{
+    hipLaunchKernelGGL(k_evogp_to_prefix, dim3(blocks(pop, 64)), dim3(64), 0, s, pop, max_len, value, type,
+                       size, cap, tok, val, len);
+    return hipGetLastError();
+}
+
+hipError_t launch_prefix_to_evogp(int pop, int cap, const int32_t *tok, const float *val, const int32_t *len,
+                                  int max_len, float *value, int16_t *type, int16_t *size, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_prefix_to_evogp, dim3(blocks(pop, 64)), dim3(64), 0, s, pop, cap, tok, val, len,
+                       max_len, value, type, size);
     return hipGetLastError();
 }
 

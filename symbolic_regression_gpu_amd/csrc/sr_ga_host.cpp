@@ -523,3 +523,80 @@ extern "C" int sr_tree_splice(sr_ctx *x, int32_t n, int32_t cap, const int32_t *
     if (he != hipSuccess) return c.hip_fail(he, "splice");
     return SR_OK;
 }
+
+// ---- evogp population layout (defs.h:30-107) ---------------------------------
+extern "C" int sr_evogp_to_prefix(sr_ctx *x, int32_t pop, int32_t max_len, const float *d_value,
+                                  const int16_t *d_type, const int16_t *d_size, int32_t cap, int32_t *d_tok,
+                                  float *d_val, int32_t *d_len)
+{
+    if (!x || pop < 0 || max_len <= 0 || cap <= 0 || (pop > 0 && (!d_value || !d_type || !d_size || !d_tok ||
+                                                                  !d_val || !d_len)))
+        return SR_ERR_INVALID_ARG;
+    Ctx &c = x->c;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    hipSetDevice(c.device);
+    if (pop == 0) return SR_OK;
+    hipError_t he = launch_evogp_to_prefix(pop, max_len, d_value, d_type, d_size, cap, d_tok, d_val, d_len, c.stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c.stream);
+    if (he != hipSuccess) return c.hip_fail(he, "evogp to prefix");
+    return SR_OK;
+}
+
+extern "C" int sr_prefix_to_evogp(sr_ctx *x, int32_t pop, int32_t cap, const int32_t *d_tok, const float *d_val,
+                                  const int32_t *d_len, int32_t max_len, float *d_value, int16_t *d_type,
+                                  int16_t *d_size)
+{
+    if (!x || pop < 0 || max_len <= 0 || cap <= 0 || (pop > 0 && (!d_value || !d_type || !d_size || !d_tok ||
+                                                                  !d_val || !d_len)))
+        return SR_ERR_INVALID_ARG;
+    Ctx &c = x->c;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    hipSetDevice(c.device);
+    if (pop == 0) return SR_OK;
+    hipError_t he = launch_prefix_to_evogp(pop, cap, d_tok, d_val, d_len, max_len, d_value, d_type, d_size, c.stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c.stream);
+    if (he != hipSuccess) return c.hip_fail(he, "prefix to evogp");
+    return SR_OK;
+}
+
+extern "C" int sr_ga_import_evogp(sr_ctx *x, int32_t pop, int32_t max_len, const float *d_value,
+                                  const int16_t *d_type, const int16_t *d_size)
+{
+    if (!x || pop <= 0 || max_len <= 0 || !d_value || !d_type || !d_size) return SR_ERR_INVALID_ARG;
+    Ctx &c = x->c;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    hipSetDevice(c.device);
+    int rc = ga_ready(c);
+    if (rc) return rc;
+    const int cap = c.ga.p.max_tokens + c.ga.p.max_depth + 2;
+    rc = ga_alloc(c, pop, cap);
+    if (rc) return rc;
+    GaState &g = c.ga;
+    hipError_t he = launch_evogp_to_prefix(pop, max_len, d_value, d_type, d_size, cap, g.tok[g.cur].as<int32_t>(),
+                                           g.val[g.cur].as<float>(), g.len[g.cur].as<int32_t>(), c.stream);
+    std::vector<int32_t> len(pop);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(len.data(), g.len[g.cur].p, (size_t)pop * 4, hipMemcpyDeviceToHost, c.stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c.stream);
+    if (he != hipSuccess) return c.hip_fail(he, "evogp import");
+    for (int i = 0; i < pop; ++i)
+        if (len[i] < 1)
+            return c.fail(SR_ERR_BAD_EXPR, "individual " + std::to_string(i) + ": evogp tree size outside 1..cap");
+    return SR_OK;
+}
+
+extern "C" int sr_ga_export_evogp(sr_ctx *x, int32_t max_len, float *d_value, int16_t *d_type, int16_t *d_size)
+{
+    if (!x || max_len <= 0 || !d_value || !d_type || !d_size) return SR_ERR_INVALID_ARG;
+    Ctx &c = x->c;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    hipSetDevice(c.device);
+    int rc = ga_ready(c);
+    if (rc) return rc;
+    GaState &g = c.ga;
+    hipError_t he = launch_prefix_to_evogp(g.pop, g.cap, g.tok[g.cur].as<int32_t>(), g.val[g.cur].as<float>(),
+                                           g.len[g.cur].as<int32_t>(), max_len, d_value, d_type, d_size, c.stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c.stream);
+    if (he != hipSuccess) return c.hip_fail(he, "evogp export");
+    return SR_OK;
+}

@@ -24,6 +24,11 @@ hipError_t launch_record(const void *src, int pop, void *hist, const uint32_t *g
                          int max_gens, hipStream_t s);
 hipError_t launch_u32_add(uint32_t *p, uint32_t v, hipStream_t s);
 hipError_t launch_u32_set(uint32_t *p, uint32_t v, hipStream_t s);
+hipError_t launch_evogp_to_prefix(int pop, int max_len, const float *value, const int16_t *type,
+                                  const int16_t *size, int cap, int32_t *tok, float *val, int32_t *len,
+                                  hipStream_t s);
+hipError_t launch_prefix_to_evogp(int pop, int cap, const int32_t *tok, const float *val, const int32_t *len,
+                                  int max_len, float *value, int16_t *type, int16_t *size, hipStream_t s);
 
 }  // namespace sr
 
