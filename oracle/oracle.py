@@ -86,8 +86,14 @@ def lib():
         L.sro_detect.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, C.c_int, C.c_int,
                                  C.POINTER(DetectResult)]
         L.sro_detect_free.argtypes = [C.POINTER(DetectResult)]
-        L.sro_mark_cached.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, _u64p, C.c_int,
-                                      C.c_int, _i32p]
+        L.sro_cache_keys_create.restype = C.c_void_p
+        L.sro_cache_keys_create.argtypes = [C.c_int]
+        L.sro_cache_keys_free.argtypes = [C.c_void_p]
+        L.sro_cache_keys_get.restype = C.c_int
+        L.sro_cache_keys_get.argtypes = [C.c_void_p, C.c_void_p]
+        L.sro_cache_update.restype = C.c_int
+        L.sro_cache_update.argtypes = [C.POINTER(DetectResult), _i32p, _f64p, _i32p, C.c_void_p, _i32p, _i32p]
+        L.sro_mark_cached.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, C.c_void_p, C.c_int, _i32p]
         L.sro_skeleton.restype = C.c_int
         L.sro_skeleton.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, _i32p, _i32p, _f64p,
                                    _i32p, _i32p]
@@ -105,6 +111,16 @@ def lib():
         L.sro_evogp_to_prefix.argtypes = [C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp]
         L.sro_prefix_to_evogp.argtypes = [C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp]
         L.sro_ga_subtree_size.argtypes = [_i32p, C.c_int, C.c_int]
+        L.sro_subtree_predict64.restype = C.c_int
+        L.sro_subtree_predict64.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, _f64p, C.c_int, C.c_int64,
+                                            C.c_int, _f64p, _vp]
+        L.sro_subtree_state_create.restype = _vp
+        L.sro_subtree_state_free.argtypes = [_vp]
+        L.sro_subtree_state_size.restype = C.c_int
+        L.sro_subtree_state_size.argtypes = [_vp]
+        L.sro_subtree_state_predict64.restype = C.c_int
+        L.sro_subtree_state_predict64.argtypes = [_vp, _i32p, _f64p, _i32p, _i32p, C.c_int, _f64p, C.c_int,
+                                                  C.c_int64, C.c_int, _f64p, _vp]
         _bind_ga_hooks(L)
         _lib = L
     return _lib
@@ -140,6 +156,13 @@ def ref():
         R.ref_ga_op_log.argtypes = [C.c_int, C.c_int, C.POINTER(GaParams), _i32p, _f32p, C.c_int,
                                     _i32p, _f32p, C.c_int, _i32p, _f32p, _i32p, _f32p, _f32p, _i32p,
                                     C.c_int, C.POINTER(C.c_int)]
+        R.ref_subtree_predict64.argtypes = [_i32p, _f64p, _i32p, _i32p, C.c_int, _f64p, C.c_int, C.c_int,
+                                            _f64p]
+        R.ref_subtree_cache_create.restype = C.c_void_p
+        R.ref_subtree_cache_free.argtypes = [C.c_void_p]
+        R.ref_subtree_stateful_predict64.restype = C.c_int
+        R.ref_subtree_stateful_predict64.argtypes = [C.c_void_p, _i32p, _f64p, _i32p, _i32p, C.c_int, _f64p,
+                                                     C.c_int, C.c_int, _f64p]
         R.ref_ga_op.restype = C.c_int
         R.ref_ga_op.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(GaParams), _i32p, _f32p,
                                 C.c_int, _i32p, _f32p, C.c_int, _i32p, _f32p]
@@ -290,12 +313,44 @@ def _span(pop, e, start, size):
     return pop.tok[a:a + size], pop.val[a:a + size]
 
 
-def mark_cached(pop: Population, keys, min_size=3):
-    keys = np.ascontiguousarray(keys, np.uint64)
-    hints = np.empty(int(pop.len.sum()), dtype=np.int32)
-    lib().sro_mark_cached(pop.tok, pop.val, pop.off, pop.len, pop.E,
-                          keys if len(keys) else np.zeros(1, np.uint64), len(keys), min_size, hints)
-    return hints
+class CacheKeys:
+    """Slots of the cross-generation subtree cache (cl_gpu_eval.cu:384-397):
+    flat hash + body per slot, bodies compared on lookup."""
+
+    def __init__(self, capacity):
+        self._k = lib().sro_cache_keys_create(int(capacity))
+
+    def update(self, pop: Population, min_size=3, min_freq=10):
+        """Detect, then give new subtrees slots; returns (new_sub, new_slot)."""
+        det = DetectResult()
+        L = lib()
+        L.sro_detect(pop.tok, pop.val, pop.off, pop.len, pop.E, min_size, min_freq, C.byref(det))
+        try:
+            new_sub = np.zeros(max(det.num_subs, 1), np.int32)
+            new_slot = np.zeros(max(det.num_subs, 1), np.int32)
+            n = L.sro_cache_update(C.byref(det), pop.tok, pop.val, pop.off, self._k, new_sub, new_slot)
+        finally:
+            L.sro_detect_free(C.byref(det))
+        return new_sub[:n], new_slot[:n]
+
+    @property
+    def keys(self):
+        n = lib().sro_cache_keys_get(self._k, None)
+        out = np.zeros(max(n, 1), np.uint64)
+        lib().sro_cache_keys_get(self._k, out.ctypes.data)
+        return out[:n]
+
+    def mark(self, pop: Population, min_size=3):
+        hints = np.empty(max(int(pop.len.sum()), 1), dtype=np.int32)
+        lib().sro_mark_cached(pop.tok, pop.val, pop.off, pop.len, pop.E, self._k, min_size, hints)
+        return hints[:int(pop.len.sum())]
+
+    def close(self):
+        if self._k:
+            lib().sro_cache_keys_free(self._k)
+            self._k = None
+
+    __del__ = close
 
 
 def skeleton(pop: Population, hints):
@@ -422,6 +477,50 @@ def prefix_to_evogp(tok, val, ln, cap, max_len):
                               np.ascontiguousarray(ln, np.int32).ctypes.data, max_len, value.ctypes.data,
                               type_.ctypes.data, size.ctypes.data)
     return value, type_, size
+
+
+def _x64(X, nv, N):
+    X = np.ascontiguousarray(X, np.float64).reshape(-1)
+    assert X.size == (nv + 1) * N
+    return X
+
+
+def subtree_predict64(pop: Population, X, nv, N, threads=8, want_hints=False):
+    """CPU subtree evaluator, FP64 (cpu_subtree.cpp:135-340).  X: [(nv+1), N]
+    (widened to double).  Returns (pred [E, N], number of detected subtrees
+    [, hint map])."""
+    pred = np.empty((pop.E, N), np.float64)
+    hints = np.empty(max(len(pop.tok), 1), np.int32) if want_hints else None
+    S = lib().sro_subtree_predict64(pop.tok, pop.val, pop.off, pop.len, pop.E, _x64(X, nv, N), nv, N, threads,
+                                    pred, None if hints is None else hints.ctypes.data)
+    return (pred, S, hints) if want_hints else (pred, S)
+
+
+class SubtreeState:
+    """Stateful CPU subtree evaluator, FP64 (cpu_subtree.cpp:445-501): a cache
+    of subtree columns kept across calls on one dataset."""
+
+    def __init__(self):
+        self._s = lib().sro_subtree_state_create()
+
+    def predict64(self, pop: Population, X, nv, N, threads=8):
+        """-> (pred [E, N], new cache entries, hint map packed like pop.tok)."""
+        pred = np.empty((pop.E, N), np.float64)
+        hints = np.empty(max(len(pop.tok), 1), np.int32)
+        new = lib().sro_subtree_state_predict64(self._s, pop.tok, pop.val, pop.off, pop.len, pop.E,
+                                                _x64(X, nv, N), nv, N, threads, pred, hints.ctypes.data)
+        return pred, new, hints
+
+    @property
+    def size(self):
+        return lib().sro_subtree_state_size(self._s)
+
+    def close(self):
+        if self._s:
+            lib().sro_subtree_state_free(self._s)
+            self._s = None
+
+    __del__ = close
 
 
 def fitness_order(fit):

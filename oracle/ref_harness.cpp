@@ -8,8 +8,12 @@
 //   utils/detect.cpp          compute_hash / hash_node / detect_common_subtrees
 //   eval/cpu_simple_single.cpp eval_op (fp64 twin)           [linked separately]
 //   utils/utils.cpp           GA operators + set_seed        [linked separately]
+//   eval/cpu_subtree.cpp      eval_cpu_common_subtree_batch / eval_cpu_stateful_batch
+//                             (built with -DUSE_CPU_SUBTREE) [linked separately]
 #include "cleaned/cl_cpu_eval.cpp"
 #include "utils/detect.cpp"
+#define USE_CPU_SUBTREE
+#include "eval/evaluator.h"
 
 #include <cstdint>
 #include <random>
@@ -249,6 +253,70 @@ void ref_load_data(const char *path, int nv, int ndps, double *out)
         free(v[i]);
     }
     free(v);
+}
+
+// ---- CPU subtree evaluator (cpu_subtree.cpp:135-501) on one shared dataset --
+// X column-major [(nv+1)][N]; pred [E][N].
+struct RefBatch {
+    InputInfo info{};
+    std::vector<int> nv, nd, nt;
+    std::vector<int *> tp;
+    std::vector<double *> vp;
+    std::vector<double *> cols;
+    std::vector<double **> vars;
+    std::vector<double *> preds;
+};
+
+static void ref_batch(RefBatch &b, const int *tok, const double *val, const int *off, const int *len, int E,
+                      const double *X, int nv, int N, double *pred)
+{
+    b.nv.assign(E, nv);
+    b.nd.assign(E, N);
+    b.nt.assign(len, len + E);
+    b.tp.resize(E);
+    b.vp.resize(E);
+    b.preds.resize(E);
+    b.cols.resize(nv + 1);
+    for (int i = 0; i <= nv; ++i) b.cols[i] = const_cast<double *>(X) + (size_t)i * N;
+    b.vars.assign(E, b.cols.data());
+    int maxt = 0;
+    for (int e = 0; e < E; ++e) {
+        b.tp[e] = const_cast<int *>(tok + off[e]);
+        b.vp[e] = const_cast<double *>(val + off[e]);
+        b.preds[e] = pred + (size_t)e * N;
+        maxt = std::max(maxt, len[e]);
+    }
+    b.info.num_exprs = E;
+    b.info.num_vars = b.nv.data();
+    b.info.num_dps = b.nd.data();
+    b.info.num_tokens = b.nt.data();
+    b.info.tokens = b.tp.data();
+    b.info.values = b.vp.data();
+    b.info.max_tokens = maxt;
+    b.info.max_num_dps = N;
+    b.info.max_num_features = nv + 1;
+    b.info.has_shared_data = true;
+}
+
+void ref_subtree_predict64(const int *tok, const double *val, const int *off, const int *len, int E,
+                           const double *X, int nv, int N, double *pred)
+{
+    RefBatch b;
+    ref_batch(b, tok, val, off, len, E, X, nv, N, pred);
+    eval_cpu_common_subtree_batch(b.info, b.vars.data(), b.preds.data(), nullptr);
+}
+
+void *ref_subtree_cache_create() { return new SubtreeCache(); }
+void ref_subtree_cache_free(void *c) { delete static_cast<SubtreeCache *>(c); }
+
+int ref_subtree_stateful_predict64(void *c, const int *tok, const double *val, const int *off, const int *len,
+                                   int E, const double *X, int nv, int N, double *pred)
+{
+    RefBatch b;
+    ref_batch(b, tok, val, off, len, E, X, nv, N, pred);
+    SubtreeCache &cache = *static_cast<SubtreeCache *>(c);
+    eval_cpu_stateful_batch(b.info, b.vars.data(), b.preds.data(), cache, nullptr);
+    return (int)cache.results.size();
 }
 
 }  // extern "C"

@@ -918,27 +918,77 @@ void sro_detect_free(sro_detect_result *res)
     memset(res, 0, sizeof *res);
 }
 
-static int find_key(const uint64_t *keys, int n, uint64_t h)
+struct sro_cache_keys {
+    int n, capacity;
+    uint64_t *key;
+    int *off, *len; /* body of slot s: tok/val[off[s] .. off[s] + len[s]) */
+    int *tok;
+    double *val;
+    int ntok, captok;
+};
+
+sro_cache_keys *sro_cache_keys_create(int capacity)
 {
-    for (int i = 0; i < n; i++)
-        if (keys[i] == h) return i;
+    sro_cache_keys *k = (sro_cache_keys *)calloc(1, sizeof(sro_cache_keys));
+    k->capacity = capacity > 0 ? capacity : 0;
+    const int c = k->capacity > 0 ? k->capacity : 1;
+    k->key = (uint64_t *)malloc(sizeof(uint64_t) * c);
+    k->off = (int *)malloc(sizeof(int) * c);
+    k->len = (int *)malloc(sizeof(int) * c);
+    return k;
+}
+
+void sro_cache_keys_free(sro_cache_keys *k)
+{
+    if (!k) return;
+    free(k->key);
+    free(k->off);
+    free(k->len);
+    free(k->tok);
+    free(k->val);
+    free(k);
+}
+
+int sro_cache_keys_get(const sro_cache_keys *k, uint64_t *keys_out)
+{
+    if (keys_out) memcpy(keys_out, k->key, sizeof(uint64_t) * (size_t)k->n);
+    return k->n;
+}
+
+/* Slot whose flat hash AND body equal this subtree's, or -1. */
+static int find_key(const sro_cache_keys *k, uint64_t h, const int *t, const double *v, int size)
+{
+    for (int i = 0; i < k->n; i++)
+        if (k->key[i] == h && k->len[i] == size && memcmp(k->tok + k->off[i], t, sizeof(int) * size) == 0 &&
+            memcmp(k->val + k->off[i], v, sizeof(double) * size) == 0)
+            return i;
     return -1;
 }
 
 /* cl_gpu_eval.cu:384-397 */
 int sro_cache_update(const sro_detect_result *det, const int *tok, const double *val,
-                     const int *off, uint64_t *keys, int *num_cached, int capacity,
-                     int *new_sub, int *new_slot)
+                     const int *off, sro_cache_keys *k, int *new_sub, int *new_slot)
 {
     int nn = 0;
     for (int s = 0; s < det->num_subs; s++) {
         const int *t = tok + off[det->sub_expr[s]] + det->sub_start[s];
         const double *v = val + off[det->sub_expr[s]] + det->sub_start[s];
-        uint64_t h = sro_hash_flat(t, v, det->sub_size[s]);
-        if (find_key(keys, *num_cached, h) >= 0) continue;
-        if (*num_cached >= capacity) continue;
-        int slot = (*num_cached)++;
-        keys[slot] = h;
+        const int size = det->sub_size[s];
+        uint64_t h = sro_hash_flat(t, v, size);
+        if (find_key(k, h, t, v, size) >= 0) continue;
+        if (k->n >= k->capacity) continue;
+        int slot = k->n++;
+        k->key[slot] = h;
+        if (k->ntok + size > k->captok) {
+            k->captok = 2 * (k->ntok + size);
+            k->tok = (int *)realloc(k->tok, sizeof(int) * (size_t)k->captok);
+            k->val = (double *)realloc(k->val, sizeof(double) * (size_t)k->captok);
+        }
+        k->off[slot] = k->ntok;
+        k->len[slot] = size;
+        memcpy(k->tok + k->ntok, t, sizeof(int) * size);
+        memcpy(k->val + k->ntok, v, sizeof(double) * size);
+        k->ntok += size;
         new_sub[nn] = s;
         new_slot[nn] = slot;
         nn++;
@@ -948,7 +998,7 @@ int sro_cache_update(const sro_detect_result *det, const int *tok, const double 
 
 /* cl_gpu_eval.cu:279-292 (size with arity 2 for tokens < 10, 1 otherwise). */
 void sro_mark_cached(const int *tok, const double *val, const int *off, const int *len,
-                     int E, const uint64_t *keys, int num_cached, int min_size, int *hints)
+                     int E, const sro_cache_keys *k, int min_size, int *hints)
 {
     for (int e = 0; e < E; e++) {
         const int *t = tok + off[e];
@@ -960,7 +1010,8 @@ void sro_mark_cached(const int *tok, const double *val, const int *off, const in
             if (h[pos] != -1) continue;
             int size = sro_subtree_size(t, pos, L);
             if (size < min_size) continue;
-            int slot = find_key(keys, num_cached, sro_hash_flat(t + pos, v + pos, size));
+            if (pos + size > L) continue;
+            int slot = find_key(k, sro_hash_flat(t + pos, v + pos, size), t + pos, v + pos, size);
             if (slot < 0) continue;
             int overlap = 0;
             for (int k = 0; k < size; k++)
@@ -1026,6 +1077,421 @@ void sro_predict32_cached(const int *tok, const double *val, int len, const floa
         }
         out[dp] = stk[0];
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* CPU subtree evaluator, FP64 (cpu_subtree.cpp:83-501)                       */
+/* ------------------------------------------------------------------------ */
+
+#define SUB_STACK 1100 /* > MAX_EXPR_LEN / 2 + 1: the deepest right-to-left stack */
+
+/* eval_tree_cpu (cpu_subtree.cpp:83-132): right-to-left scan; with hints, -2
+ * skips a token inside a reused subtree and h >= 0 pushes column h at dp.
+ * x[i] = X[i*N + dp] for i <= nv (the reference's per-point copy). A stack
+ * underflow (malformed stream) gives NaN instead of reading outside. */
+static double sub_eval_point(const int *tok, const double *val, const int *hints, int len, const double *x,
+                             int nx, double *const *cols, int64_t dp)
+{
+    double st[SUB_STACK];
+    int sp = 0;
+    for (int i = len - 1; i >= 0; i--) {
+        if (hints) {
+            if (hints[i] == -2) continue;
+            if (hints[i] >= 0) {
+                st[sp++] = cols[hints[i]][dp];
+                continue;
+            }
+        }
+        const int t = tok[i];
+        if (t > 0) {
+            const int bin = t < 10;
+            if (sp < 1 + bin) return NAN;
+            double a = st[--sp], b = 0.0;
+            if (bin) b = st[--sp];
+            st[sp++] = sro_op64(t, a, b);
+        } else if (t == SRO_TOK_CONST) {
+            st[sp++] = val[i];
+        } else if (t == SRO_TOK_VAR) {
+            const int v = (int)val[i];
+            st[sp++] = (v >= 0 && v < nx) ? x[v] : NAN;
+        }
+        if (sp >= SUB_STACK) return NAN;
+    }
+    return sp > 0 ? st[0] : NAN;
+}
+
+typedef struct {
+    int phase; /* 0: fill columns, datapoints round robin; 1: expressions round robin */
+    int worker, threads;
+    /* columns to fill: n_fill subtrees (tokens, values, sizes), written to fill_col */
+    int n_fill;
+    const int *const *fill_tok;
+    const double *const *fill_val;
+    const int *fill_len;
+    double *const *fill_col;
+    /* expressions */
+    const int *tok;
+    const double *val;
+    const int *off;
+    const int *len;
+    const int *hints; /* packed like tok, or NULL */
+    int E;
+    double *const *cols;
+    const double *X;
+    int nv;
+    int64_t N;
+    double *pred;
+} sub_job;
+
+static void *sub_worker(void *arg)
+{
+    sub_job *j = (sub_job *)arg;
+    double x[64];
+    const int nx = j->nv + 1 < 64 ? j->nv + 1 : 64;
+    if (j->phase == 0) {
+        /* cpu_subtree.cpp:191-215: every new subtree's column, datapoints
+         * dealt round robin to the workers */
+        for (int64_t dp = j->worker; dp < j->N; dp += j->threads) {
+            for (int i = 0; i < nx; i++) x[i] = j->X[(int64_t)i * j->N + dp];
+            for (int s = 0; s < j->n_fill; s++)
+                j->fill_col[s][dp] = sub_eval_point(j->fill_tok[s], j->fill_val[s], NULL, j->fill_len[s], x, nx,
+                                                    NULL, 0);
+        }
+    } else {
+        /* cpu_subtree.cpp:241-270: expressions dealt round robin */
+        for (int e = j->worker; e < j->E; e += j->threads) {
+            const int *t = j->tok + j->off[e];
+            const double *v = j->val + j->off[e];
+            const int *h = j->hints ? j->hints + j->off[e] : NULL;
+            double *out = j->pred + (int64_t)e * j->N;
+            for (int64_t dp = 0; dp < j->N; dp++) {
+                for (int i = 0; i < nx; i++) x[i] = j->X[(int64_t)i * j->N + dp];
+                out[dp] = sub_eval_point(t, v, h, j->len[e], x, nx, j->cols, dp);
+            }
+        }
+    }
+    return NULL;
+}
+
+static void sub_run(sub_job proto, int threads)
+{
+    if (threads < 1) threads = 1;
+    if (threads > 64) threads = 64;
+    pthread_t th[64];
+    sub_job jobs[64];
+    for (int w = 0; w < threads; w++) {
+        jobs[w] = proto;
+        jobs[w].worker = w;
+        jobs[w].threads = threads;
+        pthread_create(&th[w], NULL, sub_worker, &jobs[w]);
+    }
+    for (int w = 0; w < threads; w++) pthread_join(th[w], NULL);
+}
+
+int sro_subtree_predict64(const int *tok, const double *val, const int *off, const int *len, int E,
+                          const double *X, int nv, int64_t N, int threads, double *pred, int *hints_out)
+{
+    /* cpu_subtree.cpp:152-167: min_size 3, min_freq 5 (2 below 10 expressions) */
+    sro_detect_result det;
+    sro_detect(tok, val, off, len, E, 3, E < 10 ? 2 : 5, &det);
+    const int S = det.num_subs;
+    double **cols = NULL;
+    const int **ft = NULL;
+    const double **fv = NULL;
+    if (S > 0) {
+        cols = (double **)malloc(sizeof(double *) * S);
+        ft = (const int **)malloc(sizeof(int *) * S);
+        fv = (const double **)malloc(sizeof(double *) * S);
+        for (int s = 0; s < S; s++) {
+            cols[s] = (double *)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1));
+            ft[s] = tok + off[det.sub_expr[s]] + det.sub_start[s];
+            fv[s] = val + off[det.sub_expr[s]] + det.sub_start[s];
+        }
+        sub_job fill = {0};
+        fill.phase = 0;
+        fill.n_fill = S;
+        fill.fill_tok = ft;
+        fill.fill_val = fv;
+        fill.fill_len = det.sub_size;
+        fill.fill_col = cols;
+        fill.X = X;
+        fill.nv = nv;
+        fill.N = N;
+        sub_run(fill, threads);
+    }
+    sub_job ev = {0};
+    ev.phase = 1;
+    ev.tok = tok;
+    ev.val = val;
+    ev.off = off;
+    ev.len = len;
+    ev.hints = S > 0 ? det.hints : NULL; /* cpu_subtree.cpp:262-265 */
+    ev.E = E;
+    ev.cols = cols;
+    ev.X = X;
+    ev.nv = nv;
+    ev.N = N;
+    ev.pred = pred;
+    sub_run(ev, threads);
+    if (hints_out) {
+        int64_t T = 0;
+        for (int e = 0; e < E; e++) T = off[e] + len[e] > T ? off[e] + len[e] : T;
+        memcpy(hints_out, det.hints, sizeof(int) * (size_t)T);
+    }
+    for (int s = 0; s < S; s++) free(cols[s]);
+    free(cols);
+    free(ft);
+    free(fv);
+    sro_detect_free(&det);
+    return S;
+}
+
+/* ---- stateful form: a cache of subtree columns kept across calls ---------- */
+
+struct sro_subtree_state {
+    int n, cap;
+    uint64_t *key;
+    int *size;
+    int **tok;
+    double **val;
+    double **col;
+    int64_t N;
+    int *slot; /* open-addressing index over key, 2*cap entries, -1 = empty */
+    int nslot;
+};
+
+sro_subtree_state *sro_subtree_state_create(void)
+{
+    return (sro_subtree_state *)calloc(1, sizeof(sro_subtree_state));
+}
+
+static void state_clear(sro_subtree_state *s)
+{
+    for (int i = 0; i < s->n; i++) {
+        free(s->tok[i]);
+        free(s->val[i]);
+        free(s->col[i]);
+    }
+    free(s->key);
+    free(s->size);
+    free(s->tok);
+    free(s->val);
+    free(s->col);
+    free(s->slot);
+    memset(s, 0, sizeof(*s));
+}
+
+void sro_subtree_state_free(sro_subtree_state *s)
+{
+    if (!s) return;
+    state_clear(s);
+    free(s);
+}
+
+int sro_subtree_state_size(const sro_subtree_state *s) { return s ? s->n : 0; }
+
+static int same_body(const int *ta, const double *va, const int *tb, const double *vb, int size)
+{
+    return memcmp(ta, tb, sizeof(int) * size) == 0 && memcmp(va, vb, sizeof(double) * size) == 0;
+}
+
+static int state_find(const sro_subtree_state *s, uint64_t h, const int *t, const double *v, int size)
+{
+    if (!s->nslot) return -1;
+    for (uint64_t i = h & (uint64_t)(s->nslot - 1);; i = (i + 1) & (uint64_t)(s->nslot - 1)) {
+        const int id = s->slot[i];
+        if (id < 0) return -1;
+        if (s->key[id] == h && s->size[id] == size && same_body(s->tok[id], s->val[id], t, v, size)) return id;
+    }
+}
+
+static void state_index(sro_subtree_state *s, int id)
+{
+    for (uint64_t i = s->key[id] & (uint64_t)(s->nslot - 1);; i = (i + 1) & (uint64_t)(s->nslot - 1))
+        if (s->slot[i] < 0) {
+            s->slot[i] = id;
+            return;
+        }
+}
+
+static int state_add(sro_subtree_state *s, uint64_t h, const int *t, const double *v, int size)
+{
+    if (s->n == s->cap) {
+        const int cap = s->cap ? 2 * s->cap : 256;
+        s->key = (uint64_t *)realloc(s->key, sizeof(uint64_t) * cap);
+        s->size = (int *)realloc(s->size, sizeof(int) * cap);
+        s->tok = (int **)realloc(s->tok, sizeof(int *) * cap);
+        s->val = (double **)realloc(s->val, sizeof(double *) * cap);
+        s->col = (double **)realloc(s->col, sizeof(double *) * cap);
+        s->cap = cap;
+        free(s->slot);
+        s->nslot = 4 * cap;
+        s->slot = (int *)malloc(sizeof(int) * s->nslot);
+        for (int i = 0; i < s->nslot; i++) s->slot[i] = -1;
+        for (int i = 0; i < s->n; i++) state_index(s, i);
+    }
+    const int id = s->n++;
+    s->key[id] = h;
+    s->size[id] = size;
+    s->tok[id] = (int *)malloc(sizeof(int) * size);
+    s->val[id] = (double *)malloc(sizeof(double) * size);
+    memcpy(s->tok[id], t, sizeof(int) * size);
+    memcpy(s->val[id], v, sizeof(double) * size);
+    s->col[id] = NULL;
+    state_index(s, id);
+    return id;
+}
+
+typedef struct {
+    uint64_t h;
+    int e, j, size;
+    int cls; /* equal-body class within its hash run */
+} sub_cand;
+
+static int sub_cand_cmp(const void *a, const void *b)
+{
+    const sub_cand *x = (const sub_cand *)a, *y = (const sub_cand *)b;
+    if (x->h != y->h) return x->h < y->h ? -1 : 1;
+    if (x->cls != y->cls) return x->cls < y->cls ? -1 : 1;
+    if (x->e != y->e) return x->e < y->e ? -1 : 1;
+    return (x->j > y->j) - (x->j < y->j);
+}
+
+typedef struct {
+    int first, count, size, id;
+    uint64_t h;
+} sub_group;
+
+static int sub_group_cmp(const void *a, const void *b)
+{
+    const sub_group *x = (const sub_group *)a, *y = (const sub_group *)b;
+    if (x->size != y->size) return x->size > y->size ? -1 : 1;
+    return (x->h > y->h) - (x->h < y->h);
+}
+
+int sro_subtree_state_predict64(sro_subtree_state *s, const int *tok, const double *val, const int *off,
+                                const int *len, int E, const double *X, int nv, int64_t N, int threads,
+                                double *pred, int *hints_out)
+{
+    if (s->N != N) state_clear(s), s->N = N; /* a cache belongs to one dataset */
+    /* cpu_subtree.cpp:462-476 -> detect.cpp:391-560 with min_size 2 and
+     * min_freq 5 (2 below 10 expressions); candidates grouped by the flat
+     * hash of the subtree itself and then by equal tokens and values (the
+     * flat hash collides on high-bit value differences). */
+    const int min_size = 2, min_freq = E < 10 ? 2 : 5;
+    int64_t T = 0;
+    for (int e = 0; e < E; e++) T = off[e] + len[e] > T ? off[e] + len[e] : T;
+    int *hints = hints_out ? hints_out : (int *)malloc(sizeof(int) * (size_t)(T > 0 ? T : 1));
+    for (int e = 0; e < E; e++)
+        for (int j = 0; j < len[e]; j++) hints[off[e] + j] = -1;
+    int64_t nc = 0, capc = 1024;
+    sub_cand *cand = (sub_cand *)malloc(sizeof(sub_cand) * capc);
+    for (int e = 0; e < E; e++) {
+        const int *t = tok + off[e];
+        const double *v = val + off[e];
+        for (int j = 0; j < len[e]; j++) {
+            const int size = sro_subtree_size(t, j, len[e]);
+            if (size < min_size) continue;
+            if (nc == capc) cand = (sub_cand *)realloc(cand, sizeof(sub_cand) * (capc *= 2));
+            cand[nc].h = sro_hash_flat(t + j, v + j, size);
+            cand[nc].e = e;
+            cand[nc].j = j;
+            cand[nc].size = size;
+            cand[nc].cls = 0;
+            nc++;
+        }
+    }
+    qsort(cand, (size_t)nc, sizeof(sub_cand), sub_cand_cmp);
+    /* split each hash run into equal-body classes, numbered by first occurrence */
+    for (int64_t a = 0; a < nc;) {
+        int64_t b = a + 1;
+        while (b < nc && cand[b].h == cand[a].h) b++;
+        int ncls = 0;
+        for (int64_t c = a; c < b; c++) {
+            const int *tc = tok + off[cand[c].e] + cand[c].j;
+            const double *vc = val + off[cand[c].e] + cand[c].j;
+            int k = -1;
+            for (int64_t r = a; r < c && k < 0; r++) {
+                if (cand[r].size == cand[c].size &&
+                    same_body(tok + off[cand[r].e] + cand[r].j, val + off[cand[r].e] + cand[r].j, tc, vc,
+                              cand[c].size))
+                    k = cand[r].cls;
+            }
+            cand[c].cls = k >= 0 ? k : ncls++;
+        }
+        if (ncls > 1) qsort(cand + a, (size_t)(b - a), sizeof(sub_cand), sub_cand_cmp);
+        a = b;
+    }
+    /* groups in ascending hash order (a std::map's order): cached ones are
+     * hits, new ones of frequency >= min_freq join the cache in that order */
+    int ng = 0, capg = 256, old_n = s->n;
+    sub_group *g = (sub_group *)malloc(sizeof(sub_group) * capg);
+    for (int64_t a = 0; a < nc;) {
+        int64_t b = a + 1;
+        while (b < nc && cand[b].h == cand[a].h && cand[b].cls == cand[a].cls) b++;
+        const int *ta = tok + off[cand[a].e] + cand[a].j;
+        const double *va = val + off[cand[a].e] + cand[a].j;
+        int id = state_find(s, cand[a].h, ta, va, cand[a].size);
+        if (id < 0 && b - a >= min_freq)
+            id = state_add(s, cand[a].h, tok + off[cand[a].e] + cand[a].j, val + off[cand[a].e] + cand[a].j,
+                           cand[a].size);
+        if (id >= 0) {
+            if (ng == capg) g = (sub_group *)realloc(g, sizeof(sub_group) * (capg *= 2));
+            g[ng].first = (int)a;
+            g[ng].count = (int)(b - a);
+            g[ng].size = cand[a].size;
+            g[ng].id = id;
+            g[ng].h = cand[a].h;
+            ng++;
+        }
+        a = b;
+    }
+    /* larger subtrees first; an occurrence overlapping a marked one is left
+     * to the larger subtree (detect.cpp:522-555) */
+    qsort(g, (size_t)ng, sizeof(sub_group), sub_group_cmp);
+    for (int k = 0; k < ng; k++)
+        for (int c = g[k].first; c < g[k].first + g[k].count; c++) {
+            int *h = hints + off[cand[c].e] + cand[c].j;
+            int overlap = 0;
+            for (int q = 0; q < cand[c].size && !overlap; q++) overlap = h[q] != -1;
+            if (overlap) continue;
+            h[0] = g[k].id;
+            for (int q = 1; q < cand[c].size; q++) h[q] = -2;
+        }
+    free(g);
+    free(cand);
+    /* cpu_subtree.cpp:478-495: columns of the new entries, then every expression */
+    const int nn = s->n - old_n;
+    if (nn > 0) {
+        for (int i = old_n; i < s->n; i++) s->col[i] = (double *)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1));
+        sub_job fill = {0};
+        fill.phase = 0;
+        fill.n_fill = nn;
+        fill.fill_tok = (const int *const *)(s->tok + old_n);
+        fill.fill_val = (const double *const *)(s->val + old_n);
+        fill.fill_len = s->size + old_n;
+        fill.fill_col = s->col + old_n;
+        fill.X = X;
+        fill.nv = nv;
+        fill.N = N;
+        sub_run(fill, threads);
+    }
+    sub_job ev = {0};
+    ev.phase = 1;
+    ev.tok = tok;
+    ev.val = val;
+    ev.off = off;
+    ev.len = len;
+    ev.hints = s->n > 0 ? hints : NULL;
+    ev.E = E;
+    ev.cols = s->col;
+    ev.X = X;
+    ev.nv = nv;
+    ev.N = N;
+    ev.pred = pred;
+    sub_run(ev, threads);
+    if (!hints_out) free(hints);
+    return nn;
 }
 
 /* ------------------------------------------------------------------------ */

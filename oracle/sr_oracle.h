@@ -18,10 +18,20 @@
  *   sro_detect
  *       pinned: detect_common_subtrees (detect.cpp:120-387) compiled in place,
  *       compared as hash sets / occurrence sets / hint maps modulo ids.
- *   sro_mark_cached / sro_skeleton
+ *   sro_mark_cached / sro_skeleton / sro_cache_update
  *       pinned by brute force (O(n^2) subtree comparison) and invariants; the
- *       reference implementation (cl_gpu_eval.cu:294-329,433-459) is CUDA
- *       source that cannot be compiled here.
+ *       reference implementation (cl_gpu_eval.cu:294-329,384-397,433-459) is
+ *       CUDA source that cannot be compiled here.  Slots compare bodies as
+ *       well as flat hashes (the reference's flat-hash-only keys collide).
+ *   sro_subtree_predict64
+ *       pinned: eval_cpu_common_subtree_batch (cpu_subtree.cpp:135-340)
+ *       compiled in place, FP64 predictions bit for bit; equal to
+ *       sro_predict64 bit for bit.
+ *   sro_subtree_state_*
+ *       pinned by invariants: equal to sro_predict64 bit for bit over
+ *       generations, new-entry counts equal a brute-force count.  The
+ *       reference's eval_cpu_stateful_batch keys subtrees by the wrong span
+ *       (see below), so it is not a golden source.
  *   sro_philox4x32_10
  *       pinned: Random123 published known-answer vectors.
  *   sro_ga_generate / sro_ga_mutate / sro_ga_crossover
@@ -137,16 +147,24 @@ void sro_detect(const int *tok, const double *val, const int *off, const int *le
                 int min_size, int min_freq, sro_detect_result *res);
 void sro_detect_free(sro_detect_result *res);
 
-/* Persistent cache keyed by flat hash (cl_gpu_eval.cu:384-397).  The caller
- * owns keys[capacity]; *num_cached is updated.  Returns number of new slots and
- * writes the detected-subtree index of each new slot into new_sub[]. */
+/* Persistent cache keyed by flat hash (cl_gpu_eval.cu:384-397), at most
+ * `capacity` slots.  The reference keys by the flat hash alone; that hash
+ * keeps a difference confined to the high bits of a value in the high bits
+ * of the hash, so subtrees such as x0*x0 and x2*x2 (values 0.0 and 2.0)
+ * collide and would share a column.  Here each slot also keeps its subtree's
+ * tokens and values and a lookup compares them.  sro_cache_update returns the
+ * number of new slots and writes the detected-subtree index of each new slot
+ * into new_sub[] and the slot into new_slot[]. */
+typedef struct sro_cache_keys sro_cache_keys;
+sro_cache_keys *sro_cache_keys_create(int capacity);
+void sro_cache_keys_free(sro_cache_keys *k);
+int sro_cache_keys_get(const sro_cache_keys *k, uint64_t *keys_out); /* returns the slot count */
 int sro_cache_update(const sro_detect_result *det, const int *tok, const double *val,
-                     const int *off, uint64_t *keys, int *num_cached, int capacity,
-                     int *new_sub, int *new_slot);
+                     const int *off, sro_cache_keys *k, int *new_sub, int *new_slot);
 /* mark_cached_subtrees_from_state (cl_gpu_eval.cu:294-329): left-to-right scan,
  * first cached subtree of size >= min_size wins.  hints packed like tok. */
 void sro_mark_cached(const int *tok, const double *val, const int *off, const int *len,
-                     int E, const uint64_t *keys, int num_cached, int min_size, int *hints);
+                     int E, const sro_cache_keys *k, int min_size, int *hints);
 /* Skeleton build (cl_gpu_eval.cu:433-459): cached subtree -> (OP_REF, slot).
  * Output arrays sized >= total tokens; returns total skeleton tokens. */
 int sro_skeleton(const int *tok, const double *val, const int *off, const int *len, int E,
@@ -154,6 +172,39 @@ int sro_skeleton(const int *tok, const double *val, const int *off, const int *l
 /* Evaluate a skeleton point: OP_REF pushes cache[slot*N + dp]. */
 void sro_predict32_cached(const int *tok, const double *val, int len, const float *X, int nv,
                           int64_t N, const float *cache, float *out);
+
+/* ---- CPU subtree evaluator, FP64 (cpu_subtree.cpp:83-501) ----
+ * X: column-major doubles [(nv+1) * N] (y in column nv, as the reference's
+ * per-expression vars[0..nv]); pred [E * N].  threads <= 0 -> 1; results do
+ * not depend on the thread count.
+ * sro_subtree_predict64 restates eval_cpu_common_subtree_batch
+ * (cpu_subtree.cpp:135-340): sro_detect with min_size 3 and min_freq 5 (2
+ * below 10 expressions), one FP64 column per detected subtree, then every
+ * expression evaluated right to left with the hint map (-2 skip, h >= 0 push
+ * column h).  Returns the number of detected subtrees; hints_out (nullable)
+ * receives the hint map packed like tok. */
+int sro_subtree_predict64(const int *tok, const double *val, const int *off, const int *len, int E,
+                          const double *X, int nv, int64_t N, int threads, double *pred, int *hints_out);
+/* The stateful form (eval_cpu_stateful_batch, cpu_subtree.cpp:445-501 with
+ * detect_and_update_cache, detect.cpp:391-560): subtrees of size >= 2 are
+ * grouped by the flat hash of their own tokens; a group already cached is a
+ * hit, a new group occurring >= min_freq times (5, or 2 below 10 expressions)
+ * joins the cache (ids in ascending hash order); groups are marked largest
+ * first, an occurrence overlapping a marked span is skipped; new columns are
+ * filled, then every expression is evaluated with the hints.  The reference
+ * hashes the first `size` tokens of the EXPRESSION (compute_hash(tokens[i],
+ * values[i], size), detect.cpp:429-431) instead of the subtree at j, so its
+ * groups mix different subtrees and its predictions are wrong; this
+ * restatement keys each subtree by its own hash.  A state belongs to one
+ * dataset: a call with another N clears it.  Returns the number of new cache
+ * entries. */
+typedef struct sro_subtree_state sro_subtree_state;
+sro_subtree_state *sro_subtree_state_create(void);
+void sro_subtree_state_free(sro_subtree_state *s);
+int sro_subtree_state_size(const sro_subtree_state *s);
+int sro_subtree_state_predict64(sro_subtree_state *s, const int *tok, const double *val, const int *off,
+                                const int *len, int E, const double *X, int nv, int64_t N, int threads,
+                                double *pred, int *hints_out);
 
 /* ---- counter-based RNG ---- */
 void sro_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

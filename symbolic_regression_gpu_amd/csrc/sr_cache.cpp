@@ -6,7 +6,10 @@
 //   1. detect common subtrees (size >= min_size, >= min_freq exact
 //      occurrences) with Merkle hashes over (token, double value);
 //   2. give every detected subtree not yet cached a slot, keyed by its flat
-//      FNV hash (cl_gpu_eval.cu:384-397), while slots remain;
+//      FNV hash (cl_gpu_eval.cu:384-397), while slots remain; unlike the
+//      reference, a slot also holds the subtree's tokens and values and a
+//      lookup compares them, because the flat hash collides on subtrees whose
+//      values differ only in high bits (variables 0 and 2: 0.0 and 2.0);
 //   3. mark, left to right, the first cached subtree of size >= min_size at
 //      each position (cl_gpu_eval.cu:294-329) and build skeletons in which a
 //      cached subtree is one OP_REF token (cl_gpu_eval.cu:433-459);
@@ -209,6 +212,20 @@ static void detect(const HostPop &p, int min_size, int min_freq, std::vector<Det
     }
 }
 
+// Slot holding exactly this subtree, or -1.
+static int find_slot(const SubtreeCache &cache, uint64_t h, const int32_t *t, const double *v, int size)
+{
+    auto r = cache.slot_of.equal_range(h);
+    for (auto it = r.first; it != r.second; ++it) {
+        const int s = it->second;
+        if (cache.body_len[s] == size &&
+            std::memcmp(cache.body_tok.data() + cache.body_off[s], t, sizeof(int32_t) * size) == 0 &&
+            std::memcmp(cache.body_val.data() + cache.body_off[s], v, sizeof(double) * size) == 0)
+            return s;
+    }
+    return -1;
+}
+
 // mark_cached_subtrees_from_state: hints[pos] = slot / -2 / -1.
 static void mark_cached(const HostPop &p, const SubtreeCache &cache, int min_size,
                         std::vector<int32_t> &hints)
@@ -228,9 +245,10 @@ static void mark_cached(const HostPop &p, const SubtreeCache &cache, int min_siz
             if (h[pos] != -1) continue;
             const int size = sz[pos];
             if (size < min_size) continue;
-            auto it = cache.slot_of.find(flat_hash(t + pos, v + pos, size));
-            if (it == cache.slot_of.end()) continue;
-            h[pos] = it->second;
+            if (pos + size > n) continue;
+            const int slot = find_slot(cache, flat_hash(t + pos, v + pos, size), t + pos, v + pos, size);
+            if (slot < 0) continue;
+            h[pos] = slot;
             for (int k = 1; k < size && pos + k < n; ++k) h[pos + k] = -2;
         }
     }
@@ -265,11 +283,17 @@ int eval_cached(Ctx &c, const sr_population &pop, sr_stats *st)
     // New slots for detected subtrees not cached yet.
     std::vector<int> new_sub, new_slot;
     for (size_t s = 0; s < det.size(); ++s) {
-        if (cache.slot_of.count(det[s].flat)) continue;
+        const int32_t *bt = hp.tok + hp.off[det[s].expr] + det[s].start;
+        const double *bv = hp.val.data() + hp.off[det[s].expr] + det[s].start;
+        if (find_slot(cache, det[s].flat, bt, bv, det[s].size) >= 0) continue;
         if ((int)cache.keys.size() >= cache.capacity) continue;
         int slot = (int)cache.keys.size();
         cache.keys.push_back(det[s].flat);
         cache.slot_of.emplace(det[s].flat, slot);
+        cache.body_off.push_back((int32_t)cache.body_tok.size());
+        cache.body_len.push_back(det[s].size);
+        cache.body_tok.insert(cache.body_tok.end(), bt, bt + det[s].size);
+        cache.body_val.insert(cache.body_val.end(), bv, bv + det[s].size);
         new_sub.push_back((int)s);
         new_slot.push_back(slot);
     }

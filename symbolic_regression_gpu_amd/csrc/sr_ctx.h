@@ -85,8 +85,14 @@ struct HostBuf {
 // Persistent subtree cache (cl_gpu_common.h:26-143 GPUSubtreeStateContext):
 // flat-hash -> slot, value columns [capacity][N] on the device.
 struct SubtreeCache {
-    std::unordered_map<uint64_t, int> slot_of;
+    // Slots by flat FNV hash.  That hash keeps a difference confined to the
+    // high bits of a value in the high bits (x0*x0 and x2*x2 collide), so a
+    // lookup also compares the slot's body: body_tok / body_val from
+    // body_off[slot], body_len[slot] tokens.
+    std::unordered_multimap<uint64_t, int> slot_of;
     std::vector<uint64_t> keys;  // keys[slot]
+    std::vector<int32_t> body_off, body_len, body_tok;
+    std::vector<double> body_val;
     int capacity = 2048;         // cl_gpu_common.h:13 MAX_CACHED_SUBTREES
     int min_size = 3;            // detection parameters of cl_gpu_eval.cu:371-373
     int min_freq = 10;
@@ -98,6 +104,10 @@ struct SubtreeCache {
     {
         slot_of.clear();
         keys.clear();
+        body_off.clear();
+        body_len.clear();
+        body_tok.clear();
+        body_val.clear();
     }
 };
 

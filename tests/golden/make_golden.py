@@ -13,6 +13,9 @@ Each fixture holds inputs and the reference's outputs on seeded data:
                flat hashes of detected subtrees, hint map
   ga.npz       generate_random_expression / mutate_expression /
                crossover_expressions (utils.cpp:1156-1272) under set_seed(k)
+  subtree.npz  eval_cpu_common_subtree_batch and eval_cpu_stateful_batch
+               (cpu_subtree.cpp:135-501), FP64 predictions, on two planted
+               populations sharing subtrees (python tests/golden/make_golden.py subtree)
   formats.npz  parse_input_info (utils.cpp:122-233) and load_data_file
                (utils.cpp:235-300) on the files under formats/: the
                reference's example digest and data files (data/examples/,
@@ -103,11 +106,37 @@ def make_formats(R):
     np.savez(os.path.join(HERE, "formats.npz"), **out)
 
 
+def make_subtree(R):
+    from tests.helpers import planted_population
+    nv, N = 3, 96
+    rng = np.random.default_rng(77)
+    X = rng.uniform(-3, 4, size=(nv + 1, N)).astype(np.float32).astype(np.float64)
+    out = {"X": X}
+    cache = R.ref_subtree_cache_create()
+    try:
+        for g, seed in enumerate((5, 6)):
+            pop = planted_population(60, seed, nv=nv, pool_size=8, unary=list(range(10, 28)))
+            pred = np.empty((pop.E, N), np.float64)
+            R.ref_subtree_predict64(pop.tok, pop.val, pop.off, pop.len, pop.E, X.ravel(), nv, N, pred)
+            spred = np.empty((pop.E, N), np.float64)
+            n = R.ref_subtree_stateful_predict64(cache, pop.tok, pop.val, pop.off, pop.len, pop.E, X.ravel(), nv,
+                                                 N, spred)
+            out.update({f"tok{g}": pop.tok, f"val{g}": pop.val, f"off{g}": pop.off, f"len{g}": pop.len,
+                        f"pred{g}": pred, f"stateful{g}": spred, f"cache_size{g}": np.int32(n)})
+    finally:
+        R.ref_subtree_cache_free(cache)
+    np.savez(os.path.join(HERE, "subtree.npz"), **out)
+
+
 def main():
     R = O.ref()
     if sys.argv[1:] == ["formats"]:
         make_formats(R)
         print("formats fixture written to", HERE)
+        return
+    if sys.argv[1:] == ["subtree"]:
+        make_subtree(R)
+        print("subtree fixture written to", HERE)
         return
     rng = np.random.default_rng(2024)
     # ---- op table -------------------------------------------------------
@@ -223,6 +252,7 @@ def main():
              p_tok=parents.tok, p_val=parents.val, p_off=parents.off, p_len=parents.len,
              params=np.array([3, 48, 6], np.int32), consts=np.array([-2.0, 2.0], np.float32))
     make_formats(R)
+    make_subtree(R)
     print("golden fixtures written to", HERE)
 
 

@@ -38,6 +38,53 @@ def random_population(n, seed, nv=3, max_tokens=60, max_depth=6, unary=None, bin
     return O.Population(exprs)
 
 
+def planted_population(n, seed, nv=3, pool_size=10, unary=None, binary=None):
+    """n expressions sharing subtrees: about half join two members of a pool
+    of random subtrees (2..15 tokens) under a binary operator, a third wrap a
+    pool member and a fresh expression, the rest are fresh; every pool member
+    recurs many times."""
+    rng = np.random.default_rng(seed)
+    ub, bb = unary or ALL_UNARY, binary or ALL_BINARY
+    base = random_population(pool_size + n, seed, nv=nv, max_tokens=15, max_depth=4, unary=ub, binary=bb,
+                             min_len=2)
+    pool = [base.expr(i) for i in range(pool_size)]
+    exprs = []
+    for i in range(n):
+        r = rng.random()
+        a = pool[rng.integers(pool_size)]
+        if r < 0.5:
+            b = pool[rng.integers(pool_size)]
+        elif r < 0.8:
+            b = base.expr(pool_size + i)
+        else:
+            exprs.append(base.expr(pool_size + i))
+            continue
+        op = bb[rng.integers(len(bb))]
+        if rng.random() < 0.5:
+            a, b = b, a
+        t = np.concatenate([[op], a[0], b[0]]).astype(np.int32)
+        v = np.concatenate([[0.0], a[1], b[1]])
+        if rng.random() < 0.3:
+            t = np.concatenate([[ub[rng.integers(len(ub))]], t]).astype(np.int32)
+            v = np.concatenate([[0.0], v])
+        exprs.append((t, v))
+    return O.Population(exprs)
+
+
+def collision_population():
+    """x0*x0 and x2*x2 (VAR values 0.0 and 2.0), six times each as
+    sin(square) + c: the flat FNV hash of the two subtrees is equal, so a cache
+    keyed by that hash alone would hand one the other's column."""
+    sq0 = ([3, -1, -1], [0.0, 0.0, 0.0])
+    sq2 = ([3, -1, -1], [0.0, 2.0, 2.0])
+    exprs = []
+    for k in range(6):
+        for sq in (sq0, sq2):
+            exprs.append((np.array([1, 10] + sq[0] + [0], np.int32),
+                          np.array([0.0, 0.0] + sq[1] + [0.25 * k + 0.5], np.float64)))
+    return O.Population(exprs)
+
+
 def to_api(pop):
     from symbolic_regression_gpu_amd import Population
     return Population(pop.tok, pop.val, pop.off, pop.len)

@@ -12,7 +12,8 @@ import pytest
 
 from oracle import oracle as O
 from symbolic_regression_gpu_amd import EVAL_BATCHED, EVAL_CACHED
-from tests.helpers import (GRAMMAR_BINARY, GRAMMAR_UNARY, check_population, random_population,
+from tests.helpers import (EPS32, GRAMMAR_BINARY, GRAMMAR_UNARY, check_population, collision_population,
+                           mse_error_bound, random_population,
                            to_api, uniform_data)
 
 pytestmark = pytest.mark.gpu
@@ -55,29 +56,12 @@ def c3_population(seed=3):
 
 
 def oracle_cache_replay(pops, capacity, min_size=3, min_freq=10):
-    keys = np.zeros(capacity, np.uint64)
-    n = np.zeros(1, np.int32)
+    ck = O.CacheKeys(capacity)
     out = []
-    import ctypes as C
-    L = O.lib()
     for pop in pops:
-        det = O.DetectResult()
-        L.sro_detect(pop.tok, pop.val, pop.off, pop.len, pop.E, min_size, min_freq, C.byref(det))
-        new_sub = np.zeros(max(det.num_subs, 1), np.int32)
-        new_slot = np.zeros(max(det.num_subs, 1), np.int32)
-        nc = C.c_int(int(n[0]))
-        L.sro_cache_update.argtypes = [C.POINTER(O.DetectResult), np.ctypeslib.ndpointer(np.int32),
-                                       np.ctypeslib.ndpointer(np.float64),
-                                       np.ctypeslib.ndpointer(np.int32),
-                                       np.ctypeslib.ndpointer(np.uint64), C.POINTER(C.c_int), C.c_int,
-                                       np.ctypeslib.ndpointer(np.int32),
-                                       np.ctypeslib.ndpointer(np.int32)]
-        L.sro_cache_update(C.byref(det), pop.tok, pop.val, pop.off, keys, C.byref(nc), capacity,
-                           new_sub, new_slot)
-        L.sro_detect_free(C.byref(det))
-        n[0] = nc.value
-        hints = O.mark_cached(pop, keys[:nc.value], min_size)
-        out.append((keys[:nc.value].copy(), hints))
+        ck.update(pop, min_size, min_freq)
+        out.append((ck.keys.copy(), ck.mark(pop, min_size)))
+    ck.close()
     return out
 
 
@@ -279,4 +263,72 @@ def test_generational_cache_edge_cases(ctx):
             assert np.array_equal(v_c, v_b)
         assert m_b[0] != m_b[1]
     finally:
+        ctx.cache_policy(CACHE_REFERENCE, 0, 4)
+
+
+def test_reference_policy_flat_hash_collision(ctx):
+    pop = collision_population()
+    nv, N = 3, 4099
+    X = uniform_data(nv, N, 4)
+    ctx.set_data(X)
+    ctx.cache_config(64, 3, 5)
+    ctx.cache_clear()
+    api = to_api(pop)
+    m_b, v_b, _ = ctx.eval_mse(api, EVAL_BATCHED)
+    for _ in range(2):
+        m_c, v_c, st = ctx.eval_mse(api, EVAL_CACHED)
+        assert np.array_equal(m_c, m_b, equal_nan=True)
+        assert np.array_equal(v_c, v_b)
+    keys, hints = oracle_cache_replay([pop], 64, 3, 5)[0]
+    assert np.array_equal(ctx.cache_keys(), keys) and len(keys) == 4 and len(set(keys.tolist())) == 3
+    assert np.array_equal(ctx.cache_last_hints(), hints)
+    assert m_b[0] != m_b[1]
+    ctx.cache_config(2048, 3, 10)
+    ctx.cache_clear()
+
+
+def test_generational_cache_vs_fp64_subtree_oracle(ctx):
+    """The cached path over GA generations against the stateful FP64 CPU
+    subtree evaluator (cpu_subtree.cpp:445-501, its own column cache carried
+    across the same generations).  Every expression whose relative MSE error
+    bound (first-order running bound of an fp32 evaluation, 2 ulp per
+    transcendental) is <= 1e-6 must lie within 1e-5 of that FP64 MSE; every
+    other one with a finite bound within twice its bound."""
+    from symbolic_regression_gpu_amd.api import CACHE_GENERATIONAL, CACHE_REFERENCE, GaParams, SelectParams
+    nv, N = 3, 4096
+    X = uniform_data(nv, N, 12)
+    X[nv] = np.sin(X[0]) * X[1] + X[2]
+    X64 = X.astype(np.float64)
+    ctx.set_data(X)
+    ctx.cache_policy(CACHE_GENERATIONAL, 0, 3)
+    st64 = O.SubtreeState()
+    try:
+        seed = 31
+        ctx.ga_init(GaParams.make(**C4), 300, seed)
+        sel = SelectParams(7, 0.1, 0.05)
+        checked = 0
+        for g in range(4):
+            m_c, st = ctx.ga_evaluate(EVAL_CACHED)
+            tok, val, ln, cap = ctx.ga_population()
+            pop = O.Population([(tok[i * cap:i * cap + ln[i]], val[i * cap:i * cap + ln[i]].astype(np.float64))
+                                for i in range(len(ln))])
+            pred, new, _ = st64.predict64(pop, X64, nv, N)
+            assert new > 0 or g > 0
+            y = X64[nv]
+            for e in range(pop.E):
+                t, v = pop.expr(e)
+                c32 = O.predict32(t, v, X, nv, N)
+                v64, b64 = O.predict64_bound(t, v, X, nv, N)
+                rb = mse_error_bound(v64, b64, c32, y)
+                fin = np.isfinite(c32)
+                if not np.isfinite(rb) or not np.isfinite(m_c[e]):
+                    continue
+                m64 = float(np.mean((pred[e][fin] - y[fin]) ** 2))
+                tol = 1e-5 if rb <= 1e-6 else 2 * rb + 4 * EPS32
+                checked += rb <= 1e-6
+                assert abs(float(m_c[e]) - m64) <= tol * m64, (g, e, m_c[e], m64, rb)
+            ctx.ga_next(sel, m_c, seed, g)
+        assert checked > 0.15 * 4 * 300
+    finally:
+        st64.close()
         ctx.cache_policy(CACHE_REFERENCE, 0, 4)
