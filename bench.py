@@ -256,7 +256,7 @@ def main():
 
     import torch
     from symbolic_regression_gpu_amd import EVAL_BATCHED, EVAL_CACHED, EVAL_PLAIN, Context
-    from symbolic_regression_gpu_amd.api import CACHE_GENERATIONAL_REUSED, CACHE_REFERENCE, SelectParams
+    from symbolic_regression_gpu_amd.api import CACHE_GENERATIONAL_SHARED, CACHE_REFERENCE, SelectParams
     torch.cuda.init()
     ctx = Context(0)
     N = args.N
@@ -305,7 +305,7 @@ def main():
     # (B) the timed steps: the same generations (same seed, same initial
     # population) with the generational subtree cache; its MSEs equal the
     # batched ones bit for bit, so selection and offspring are identical.
-    ctx.cache_policy(CACHE_GENERATIONAL_REUSED, 0, 4)
+    ctx.cache_policy(CACHE_GENERATIONAL_SHARED, 0, 3)
     ctx.ga_set_population(tok, val, ln, cap)
     log(f"[bench] cached generations: {W} warm-up + {K} timed")
     ctx.ga_run(sel, EVAL_CACHED, W, 0, args.seed, False)

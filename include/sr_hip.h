@@ -132,20 +132,30 @@ int sr_cache_config(sr_ctx *ctx, int32_t capacity, int32_t min_size, int32_t min
 int sr_cache_clear(sr_ctx *ctx);
 /* Cache policy for SR_EVAL_CACHED.
  *  SR_CACHE_REFERENCE (default): the reference's policy above.
- *  SR_CACHE_GENERATIONAL, SR_CACHE_GENERATIONAL_REUSED: no counterpart in
+ *  SR_CACHE_GENERATIONAL, SR_CACHE_GENERATIONAL_REUSED,
+ *  SR_CACHE_GENERATIONAL_SHARED: no counterpart in
  *    the reference (its cache keeps what it first detected and never evicts,
  *    cl_gpu_eval.cu:294-329).  The cache follows the population: a subtree
  *    (size >= min_size) with a column is read instead of evaluated; a column
  *    is written, inside the launch that evaluates the subtree anyway, for
- *    every new subtree (GENERATIONAL) or for a subtree met twice -- twice in
- *    one population or in two consecutive ones (GENERATIONAL_REUSED);
+ *    every new subtree (GENERATIONAL), for a subtree met twice -- twice in
+ *    one population or in two consecutive ones (GENERATIONAL_REUSED), or
+ *    for a subtree met twice in one population and for a subtree nested in
+ *    one of those that the previous population also held
+ *    (GENERATIONAL_SHARED: a subtree met once is rarely met again, so its
+ *    column is seldom read);
  *    columns of subtrees absent from the population being evaluated are
  *    evicted; an expression evaluated before keeps its MSE.  capacity 0
  *    sizes the columns from free device memory (half of it, at most 131072
  *    columns of N floats).  MSEs and counts equal SR_EVAL_BATCHED's bit for
  *    bit (hits are verified against the stored token body).
  * Any call empties the cache. */
-enum { SR_CACHE_REFERENCE = 0, SR_CACHE_GENERATIONAL = 1, SR_CACHE_GENERATIONAL_REUSED = 2 };
+enum {
+    SR_CACHE_REFERENCE = 0,
+    SR_CACHE_GENERATIONAL = 1,
+    SR_CACHE_GENERATIONAL_REUSED = 2,
+    SR_CACHE_GENERATIONAL_SHARED = 3
+};
 int sr_cache_policy(sr_ctx *ctx, int32_t policy, int32_t capacity, int32_t min_size);
 
 /* Host planner of the generational cache, without a device (inspection and

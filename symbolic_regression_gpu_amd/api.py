@@ -17,7 +17,7 @@ from dataclasses import dataclass
 import numpy as np
 
 EVAL_PLAIN, EVAL_BATCHED, EVAL_CACHED, EVAL_JIT = 0, 1, 2, 3
-CACHE_REFERENCE, CACHE_GENERATIONAL, CACHE_GENERATIONAL_REUSED = 0, 1, 2
+CACHE_REFERENCE, CACHE_GENERATIONAL, CACHE_GENERATIONAL_REUSED, CACHE_GENERATIONAL_SHARED = 0, 1, 2, 3
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libsrhip.so")
@@ -537,9 +537,13 @@ class Context:
 
     def cache_policy(self, policy, capacity=0, min_size=4):
         """Select the SR_EVAL_CACHED policy: CACHE_REFERENCE (detect-once, the
-        reference's cl_gpu_eval.cu:294-329) or CACHE_GENERATIONAL (every
-        distinct subtree of the current population kept, stale ones evicted).
-        capacity 0 = sized from free device memory."""
+        reference's cl_gpu_eval.cu:294-329) or one of the generational
+        policies (columns follow the population, stale ones evicted), which
+        differ in which subtrees get a column: CACHE_GENERATIONAL every new
+        one, CACHE_GENERATIONAL_REUSED one met twice (in one population or in
+        two consecutive ones), CACHE_GENERATIONAL_SHARED one met twice in the
+        population plus one nested in those that the previous population
+        held.  capacity 0 = sized from free device memory."""
         self._check(self._L.sr_cache_policy(self._h, int(policy), int(capacity), int(min_size)))
 
     def cache_clear(self):

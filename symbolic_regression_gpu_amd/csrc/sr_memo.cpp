@@ -212,7 +212,7 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out, const std::funct
     // over contiguous expression ranges; the slot map is only read), then
     // evict the rest.
     ++gen_;
-    const bool reused = policy_ == SR_CACHE_GENERATIONAL_REUSED;
+    const bool reused = policy_ == SR_CACHE_GENERATIONAL_REUSED || policy_ == SR_CACHE_GENERATIONAL_SHARED;
     // keys met in this population, as a 2^21-bit filter (a false hit only
     // stores a column that was not needed)
     std::vector<uint64_t> &cur_keys = cur_bits_;
@@ -408,7 +408,10 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out, const std::funct
             }
         }
         out.src[e] = (int32_t)out.off.size();
-        const int rs = emit(p, pop.lengths[e], root, store_once);
+        // SHARED: an expression stores nothing (a subtree met once in the
+        // population is rarely met in the next one)
+        const int rs = emit(p, pop.lengths[e], root,
+                            [&](int64_t r) { return policy_ != SR_CACHE_GENERATIONAL_SHARED && store_once(r); });
         if (rs >= 0) root_prog.put((uint64_t)rs, out.src[e]);
     }
     if (reused) std::swap(prev_bits_, cur_bits_);
@@ -579,7 +582,7 @@ int sr_cache_policy(sr_ctx *x, int32_t policy, int32_t capacity, int32_t min_siz
     Ctx &c = x->c;
     std::lock_guard<std::recursive_mutex> lk(c.mu);
     if (policy != SR_CACHE_REFERENCE && policy != SR_CACHE_GENERATIONAL &&
-        policy != SR_CACHE_GENERATIONAL_REUSED)
+        policy != SR_CACHE_GENERATIONAL_REUSED && policy != SR_CACHE_GENERATIONAL_SHARED)
         return c.fail(SR_ERR_INVALID_ARG, "unknown cache policy");
     if (capacity < 0 || min_size < 2) return c.fail(SR_ERR_INVALID_ARG, "cache capacity < 0 or min_size < 2");
     MemoState &m = c.memo;
@@ -595,7 +598,8 @@ int sr_cache_policy(sr_ctx *x, int32_t policy, int32_t capacity, int32_t min_siz
 int sr_memo_planner_create(int32_t capacity, int32_t min_size, int32_t policy, sr_memo_planner **out)
 {
     if (!out || capacity <= 0 || min_size < 2 ||
-        (policy != SR_CACHE_GENERATIONAL && policy != SR_CACHE_GENERATIONAL_REUSED))
+        (policy != SR_CACHE_GENERATIONAL && policy != SR_CACHE_GENERATIONAL_REUSED &&
+         policy != SR_CACHE_GENERATIONAL_SHARED))
         return SR_ERR_INVALID_ARG;
     auto *p = new sr_memo_planner;
     p->p.configure(capacity, min_size, policy);

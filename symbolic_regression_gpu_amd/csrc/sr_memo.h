@@ -58,9 +58,12 @@ struct MemoPlan {
 
 class MemoPlanner {
 public:
-    // store_policy: SR_CACHE_GENERATIONAL (store every new subtree) or
+    // store_policy: SR_CACHE_GENERATIONAL (store every new subtree),
     // SR_CACHE_GENERATIONAL_REUSED (store a subtree met twice: twice among a
-    // population's executed positions, or in two consecutive populations).
+    // population's executed positions, or in two consecutive populations) or
+    // SR_CACHE_GENERATIONAL_SHARED (store a subtree met twice in the
+    // population, and one nested in such a subtree that the previous
+    // population held).
     void configure(int capacity, int min_size, int store_policy);
     // Drop every slot (their columns are no longer valid).
     void forget();

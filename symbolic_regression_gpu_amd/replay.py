@@ -30,7 +30,7 @@ import time
 
 import numpy as np
 
-from .api import (CACHE_GENERATIONAL_REUSED, EVAL_BATCHED, EVAL_CACHED, EVAL_PLAIN, Context,
+from .api import (CACHE_GENERATIONAL_SHARED, EVAL_BATCHED, EVAL_CACHED, EVAL_PLAIN, Context,
                   Population, SrError, read_data_file, read_digest, write_data_file, write_digest)
 
 CSV_HEADER = ("Gen,BestMSE,MedianMSE,TotalTokens,CoveredTokens,TotalTimeMs,DetectTimeMs,"
@@ -195,7 +195,7 @@ def main(argv=None):
     args = ap.parse_args(argv)
     ctx = Context(0)
     if args.mode == "cached":
-        ctx.cache_policy(CACHE_GENERATIONAL_REUSED, 0, 4)
+        ctx.cache_policy(CACHE_GENERATIONAL_SHARED, 0, 3)
     replay(args.data_dir, args.start, args.end, MODES[args.mode], ctx, args.csv, args.data_root)
     return 0
 

@@ -139,7 +139,7 @@ C4 = dict(num_vars=3, max_tokens=255, max_depth=8, const_min=-2.0, const_max=2.0
           unary_ops=GRAMMAR_UNARY, binary_ops=GRAMMAR_BINARY)
 
 
-@pytest.mark.parametrize("capacity,min_size,policy", [(0, 4, 1), (300, 2, 1), (0, 4, 2)])
+@pytest.mark.parametrize("capacity,min_size,policy", [(0, 4, 1), (300, 2, 1), (0, 4, 2), (0, 3, 3), (200, 3, 3)])
 def test_generational_cache_matches_batched_over_generations(ctx, capacity, min_size, policy):
     """Six GA generations evaluated with the generational cache equal the
     uncached batched MSEs bit for bit, including once the cache is full

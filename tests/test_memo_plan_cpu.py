@@ -13,7 +13,8 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from symbolic_regression_gpu_amd.api import (CACHE_GENERATIONAL, CACHE_GENERATIONAL_REUSED, MemoPlanner,
+from symbolic_regression_gpu_amd.api import (CACHE_GENERATIONAL, CACHE_GENERATIONAL_REUSED,
+                                             CACHE_GENERATIONAL_SHARED, MemoPlanner,
                                              Population)
 from tests.helpers import GRAMMAR_BINARY, GRAMMAR_UNARY
 
@@ -120,7 +121,7 @@ def run_generations(planner, pops, capacity):
     return stats
 
 
-@pytest.mark.parametrize("policy", [CACHE_GENERATIONAL, CACHE_GENERATIONAL_REUSED])
+@pytest.mark.parametrize("policy", [CACHE_GENERATIONAL, CACHE_GENERATIONAL_REUSED, CACHE_GENERATIONAL_SHARED])
 @pytest.mark.parametrize("capacity,min_size", [(1 << 16, 4), (1 << 16, 2), (150, 3)])
 def test_plans_rebuild_every_expression(policy, capacity, min_size):
     planner = MemoPlanner(capacity, min_size, policy)
