@@ -208,9 +208,25 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out, const std::funct
     uint64_t *h = h_.data();
     int32_t *sz = sz_.data();
 
-    // Hash every position and mark the slots this population uses (threads
-    // over contiguous expression ranges; the slot map is only read), then
-    // evict the rest.
+    // >= 0: slot with a matching body; -1: no slot; -2: the key belongs to
+    // another body (that subtree is never cached).  Reads planner state only.
+    auto lookup = [&](int64_t q) -> int {
+        const int s = slot_of_.find(h[q]);
+        if (s < 0) return -1;
+        const int n = sz[q];
+        if ((int)body_tok_[s].size() != n) return -2;
+        if (std::memcmp(body_tok_[s].data(), tok + q, (size_t)n * 4) != 0) return -2;
+        const double *bv = body_val_[s].data();
+        for (int k = 0; k < n; ++k)
+            if (dbits(bv[k]) != dbits(vals[q + k])) return -2;
+        return s;
+    };
+
+    // Hash every position and mark the slots this population uses, settle
+    // expressions by remembered MSEs and list the positions the others will
+    // execute without a column (threads over contiguous expression ranges;
+    // planner state is only read), then evict the slots not met.  A slot met
+    // here survives the eviction, so these lookups stay valid after it.
     ++gen_;
     const bool reused = policy_ == SR_CACHE_GENERATIONAL_REUSED || policy_ == SR_CACHE_GENERATIONAL_SHARED;
     // keys met in this population, as a 2^21-bit filter (a false hit only
@@ -220,6 +236,7 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out, const std::funct
     const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
     const int nt = std::max(1, std::min({8, hw, E / 128}));
     std::vector<std::vector<int32_t>> hits(nt);
+    std::vector<std::vector<int64_t>> miss(nt);  // executed positions without a column
     std::vector<std::vector<uint64_t>> bits(reused && nt > 1 ? nt : 0);
     auto hash_mark = [&](int t) {
         const int e0 = (int)((int64_t)E * t / nt), e1 = (int)((int64_t)E * (t + 1) / nt);
@@ -229,9 +246,9 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out, const std::funct
             bt = nt > 1 ? bits[t].data() : cur_keys.data();
         }
         for (int e = e0; e < e1; ++e) {
-            const int64_t a = pop.offsets[e];
+            const int64_t a = pop.offsets[e], a1 = a + pop.lengths[e];
             strong_hash(tok + a, vals + a, pop.lengths[e], h + a, sz + a);
-            for (int64_t q = a; q < a + pop.lengths[e]; ++q) {
+            for (int64_t q = a; q < a1; ++q) {
                 if (sz[q] < ms) continue;
                 const int s = slot_of_.find(h[q]);
                 if (s >= 0) hits[t].push_back(s);
@@ -239,6 +256,21 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out, const std::funct
                     const uint64_t b = h[q] >> (64 - kKeyBitsLog2);
                     bt[b >> 6] |= 1ull << (b & 63);
                 }
+            }
+            if (sz[a] >= ms) {
+                const int root = lookup(a);
+                if (root >= 0 && has_mse_[root]) {
+                    out.src[e] = -2 - root;  // settled: nothing of it runs
+                    continue;
+                }
+            }
+            for (int64_t q = a; q < a1; ++q) {
+                if (sz[q] < ms) continue;
+                const int s = lookup(q);
+                if (s >= 0)
+                    q += sz[q] - 1;
+                else if (s == -1)
+                    miss[t].push_back(q);
             }
         }
     };
@@ -273,19 +305,6 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out, const std::funct
 
     ++epoch_;
     auto ready = [&](int s) { return born_epoch_[s] < epoch_; };
-    // >= 0: slot with a matching body; -1: no slot; -2: the key belongs to
-    // another body (that subtree is never cached)
-    auto lookup = [&](int64_t q) -> int {
-        const int s = slot_of_.find(h[q]);
-        if (s < 0) return -1;
-        const int n = sz[q];
-        if ((int)body_tok_[s].size() != n) return -2;
-        if (std::memcmp(body_tok_[s].data(), tok + q, (size_t)n * 4) != 0) return -2;
-        const double *bv = body_val_[s].data();
-        for (int k = 0; k < n; ++k)
-            if (dbits(bv[k]) != dbits(vals[q + k])) return -2;
-        return s;
-    };
     auto full = [&]() { return free_.empty() && used_ >= cap_; };
     // Program for the subtree at p (n tokens): readable columns as OP_REF,
     // column-less subtrees inline, wrapped in OP_STORE when store(q) allows,
@@ -324,14 +343,6 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out, const std::funct
         return root_slot;
     };
 
-    // Expressions settled by a remembered MSE.
-    std::vector<int> root_of(E, -1);
-    for (int e = 0; e < E; ++e) {
-        const int64_t p = pop.offsets[e];
-        if (sz[p] < ms) continue;
-        root_of[e] = lookup(p);
-        if (root_of[e] >= 0 && has_mse_[root_of[e]]) out.src[e] = -2 - root_of[e];
-    }
     FlatMap &root_prog = root_prog_;  // root slot -> program index
     root_prog.reset(1024);
 
@@ -350,27 +361,16 @@ void MemoPlanner::plan(const sr_population &pop, MemoPlan &out, const std::funct
         FlatMap &idx = occ_idx_;
         idx.reset((size_t)T / 4 + 64);
         std::vector<std::pair<int, int64_t>> occ;  // (count, first position)
-        for (int e = 0; e < E; ++e) {
-            if (out.src[e] <= -2) continue;
-            const int64_t p = pop.offsets[e];
-            for (int64_t q = p; q < p + pop.lengths[e]; ++q) {
-                if (sz[q] < ms) continue;
-                const int s = lookup(q);
-                if (s >= 0) {
-                    q += sz[q] - 1;
-                    continue;
+        for (int t = 0; t < nt; ++t)  // expression order
+            for (int64_t q : miss[t]) {
+                int k = idx.find(h[q]);
+                if (k < 0) {
+                    k = (int)occ.size();
+                    idx.put(h[q], k);
+                    occ.emplace_back(0, q);
                 }
-                if (s == -1) {
-                    int k = idx.find(h[q]);
-                    if (k < 0) {
-                        k = (int)occ.size();
-                        idx.put(h[q], k);
-                        occ.emplace_back(0, q);
-                    }
-                    occ[k].first++;
-                }
+                occ[k].first++;
             }
-        }
         std::vector<std::pair<int, int64_t>> cand;  // (-size, first position)
         for (auto &o : occ)
             if (o.first >= 2 && sz[o.second] < MAX_EXPR_LEN) cand.emplace_back(-sz[o.second], o.second);
