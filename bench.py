@@ -50,6 +50,10 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "traffic_k_batched_evolution_1000x1M.json")
 TRAFFIC_FILE_REF = os.path.join(HERE, "profiles", "traffic_k_batched_mseref_cached_evolution.json")
+# VALU issue rate of both interpreter kernels from a counter run of this
+# script (SQ_INSTS_VALU over the kernels' time against one wave64 VALU
+# instruction per CU per cycle)
+VALU_FILE = os.path.join(HERE, "profiles", "valu_issue_bench_default.json")
 
 
 def log(*a):
@@ -360,6 +364,20 @@ def main():
             traffic = json.load(f).get("hbm_bytes_per_launch")
     b_ms = batched0["interp_kernel_ms"]
     b_ach = flops0 / (b_ms * 1e-3) / 1e12
+    valu_ref = valu_mse = None
+    if os.path.exists(VALU_FILE):
+        with open(VALU_FILE) as f:
+            vj = json.load(f)
+        for k, v in vj.items():
+            if not isinstance(v, dict):
+                continue
+            issue = {"achieved": v["valu_instr_per_s"], "peak": v["valu_issue_peak_per_s"],
+                     "unit": "VALU wave64 instructions/s", "frac": v["valu_issue_frac"],
+                     "source": "profiles/valu_issue_bench_default.json (counter run)"}
+            if "MODE_MSE_REF" in k:
+                valu_ref = issue
+            elif "MODE_MSE" in k:
+                valu_mse = issue
     b_traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
@@ -405,6 +423,7 @@ def main():
             "hbm_algorithmic_bytes_per_launch": hbm_alg,
             "hbm_achieved_GBs": hbm_alg / (k_ms * 1e-3) / 1e9,
             "hbm_frac": hbm_alg / (k_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "valu_issue": valu_ref,
         },
         "roofline_batched": {
             "bound": "valu", "achieved": b_ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
@@ -413,6 +432,7 @@ def main():
             "kernel_ms": b_ms, "flops_per_launch": flops0,
             "hbm_algorithmic_bytes_per_launch": xy_bytes,
             "hbm_achieved_GBs": xy_bytes / (b_ms * 1e-3) / 1e9,
+            "valu_issue": valu_mse,
         },
         "cpu_baseline": cpu,
         "cached_evolution": cached_gen,
