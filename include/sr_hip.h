@@ -208,6 +208,13 @@ int sr_param_grad(sr_ctx *ctx, const float *d_cvals, float *d_mse, float *d_grad
 /* Central differences (step h) on the GPU: 2 forward passes per used slot. */
 int sr_param_fd_grad(sr_ctx *ctx, const float *d_cvals, float h, float *d_grad);
 int sr_sync(sr_ctx *ctx);
+/* The gradient entry point of the drop-in boundary on host arrays: dMSE/d
+ * constant at the population's own values for every CONST token whose
+ * const_mask byte is nonzero (each its own parameter, at most 8 per
+ * expression), by the reverse-mode adjoint above; synchronous.  grad_out is
+ * packed like pop->tokens (0 where the mask is 0); mse_out optional [E]. */
+int sr_grad_consts(sr_ctx *ctx, const sr_population *pop, const uint8_t *const_mask, float *grad_out,
+                   float *mse_out);
 
 /* Compile one expression to the batched interpreter's bytecode (host only, no
  * GPU needed).  out_code/out_arg: [len]; returns the instruction count, or a
@@ -303,6 +310,15 @@ int sr_ga_next(sr_ctx *ctx, const sr_select_params *sp, const float *fitness, ui
  * fitness of each evaluated generation.  ms_out (optional): wall ms. */
 int sr_ga_run(sr_ctx *ctx, const sr_select_params *sp, int mode, int32_t gens, uint32_t gen0,
               uint64_t seed, int use_graph, float *mse_hist, double *ms_out, sr_stats *stats);
+/* One select + vary step on host arrays, the evolve step of the drop-in
+ * boundary (cl_sr_system.cpp:285-330 applied to a caller's population):
+ * packed pop_in (validated) and fitness [E] -> the next generation, stride
+ * cap_out = max_tokens + max_depth + 2: tok_out / val_out [E*cap_out],
+ * len_out [E].  Equals sr_ga_set_population + sr_ga_next + sr_ga_get_population
+ * and replaces the context's GA population. */
+int sr_evolve_step(sr_ctx *ctx, const sr_population *pop_in, const float *fitness, const sr_ga_params *p,
+                   const sr_select_params *sp, uint64_t seed, uint32_t gen, int32_t cap_out, int32_t *tok_out,
+                   float *val_out, int32_t *len_out);
 
 /* Batch subtree replacement on the device (the evogp mutate / crossover ABI,
  * src/utils/gpu_kernel.h:31-58, on prefix tokens): child k is left tree

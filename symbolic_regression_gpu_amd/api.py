@@ -154,6 +154,9 @@ _SIGNATURES = {
     "sr_param_info": [_vp, C.POINTER(C.c_int32), _vp, C.POINTER(C.c_int64)],
     "sr_param_eval": [_vp, _vp, _vp],
     "sr_param_grad": [_vp, _vp, _vp, _vp],
+    "sr_grad_consts": [_vp, C.POINTER(_Pop), _vp, _vp, _vp],
+    "sr_evolve_step": [_vp, C.POINTER(_Pop), _vp, C.POINTER(GaParams), C.POINTER(SelectParams), C.c_uint64,
+                       C.c_uint32, C.c_int32, _vp, _vp, _vp],
     "sr_param_fd_grad": [_vp, _vp, C.c_float, _vp],
     "sr_sync": [_vp],
     "sr_tree_splice": [_vp, C.c_int32, C.c_int32, _i32p, _f32p, _i32p, C.c_int32, _i32p, _i32p,
@@ -614,6 +617,22 @@ class Context:
         self._check(self._L.sr_ga_next(self._h, C.byref(sel), None if f is None else f.ctypes.data,
                                        seed, gen))
 
+    def evolve_step(self, pop: Population, fitness, params: GaParams, sel: SelectParams, seed: int,
+                    gen: int):
+        """One select + vary step on a host population (sr_evolve_step):
+        returns the next generation (tok, val, len, cap), stride cap."""
+        E = pop.num_exprs
+        cap = params.max_tokens + params.max_depth + 2
+        tok = np.zeros(E * cap, np.int32)
+        val = np.zeros(E * cap, np.float32)
+        ln = np.zeros(E, np.int32)
+        f = np.ascontiguousarray(fitness, np.float32)
+        assert len(f) == E
+        self._check(self._L.sr_evolve_step(self._h, C.byref(pop._c()), f.ctypes.data, C.byref(params),
+                                           C.byref(sel), seed, gen, cap, tok.ctypes.data, val.ctypes.data,
+                                           ln.ctypes.data))
+        return tok, val, ln, cap
+
     def ga_run(self, sel: SelectParams, mode: int, gens: int, gen0: int, seed: int,
                use_graph: bool = True):
         """gens generations; returns (fitness history [gens, pop], ms, stats)."""
@@ -687,6 +706,16 @@ class Context:
         out = np.zeros(max(n.value, 1), np.int32)
         self._check(self._L.sr_param_info(self._h, C.byref(n), out.ctypes.data, C.byref(t)))
         return out[:n.value], t.value
+
+    def grad_consts(self, pop: Population, const_mask):
+        """dMSE/dconstant for every masked CONST token at the population's own
+        values (sr_grad_consts): returns (grad packed like the tokens, mse)."""
+        m = np.ascontiguousarray(const_mask, np.uint8)
+        g = np.zeros(max(len(m), 1), np.float32)
+        mse = np.zeros(max(pop.num_exprs, 1), np.float32)
+        self._check(self._L.sr_grad_consts(self._h, C.byref(pop._c()), m.ctypes.data, g.ctypes.data,
+                                           mse.ctypes.data))
+        return g[:len(m)], mse[:pop.num_exprs]
 
     def param_eval(self, d_cvals: int, d_mse: int):
         self._check(self._L.sr_param_eval(self._h, d_cvals, d_mse))

@@ -171,12 +171,9 @@ int sr_ga_params_default(sr_ga_params *p)
     return SR_OK;
 }
 
-int sr_ga_init(sr_ctx *x, const sr_ga_params *p, int32_t pop, uint64_t seed)
+// Validate and adopt GA parameters (population storage follows in ga_alloc).
+static int ga_configure(Ctx &c, const sr_ga_params *p)
 {
-    if (!x || !p || pop <= 0) return SR_ERR_INVALID_ARG;
-    Ctx &c = x->c;
-    std::lock_guard<std::recursive_mutex> lk(c.mu);
-    hipSetDevice(c.device);
     if (p->max_tokens < 1 || p->max_tokens > MAX_EXPR_LEN - 64 || p->max_depth < 0 || p->max_depth > 60 ||
         p->n_unary < 1 || p->n_unary > GA_MAX_OPS || p->n_binary < 1 || p->n_binary > GA_MAX_OPS)
         return c.fail(SR_ERR_INVALID_ARG, "GA parameters out of range");
@@ -186,14 +183,25 @@ int sr_ga_init(sr_ctx *x, const sr_ga_params *p, int32_t pop, uint64_t seed)
     for (int k = 0; k < p->n_binary; ++k)
         if (!(p->binary_ops[k] >= 1 && p->binary_ops[k] <= 9))
             return c.fail(SR_ERR_INVALID_ARG, "binary op set holds a non-binary opcode");
+    std::memcpy(&c.ga.p, p, sizeof c.ga.p);
+    c.ga.release_graphs();
+    return SR_OK;
+}
+
+int sr_ga_init(sr_ctx *x, const sr_ga_params *p, int32_t pop, uint64_t seed)
+{
+    if (!x || !p || pop <= 0) return SR_ERR_INVALID_ARG;
+    Ctx &c = x->c;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    hipSetDevice(c.device);
+    int rc = ga_configure(c, p);
+    if (rc) return rc;
     GaState &g = c.ga;
-    std::memcpy(&g.p, p, sizeof g.p);
     int cap = p->max_tokens + p->max_depth + 2;
-    int rc = ga_alloc(c, pop, cap);
+    rc = ga_alloc(c, pop, cap);
     if (rc) return rc;
     g.cur = 0;
     g.ready = true;
-    g.release_graphs();
     hipMemsetAsync(g.status.p, 0, 4, c.stream);
     hipError_t he = launch_ga_init(g.p, pop, cap, seed, g.tok[0].as<int32_t>(), g.val[0].as<float>(),
                                    g.len[0].as<int32_t>(), g.status.as<int32_t>(), c.stream);
@@ -599,4 +607,45 @@ extern "C" int sr_ga_export_evogp(sr_ctx *x, int32_t max_len, float *d_value, in
     if (he == hipSuccess) he = hipStreamSynchronize(c.stream);
     if (he != hipSuccess) return c.hip_fail(he, "evogp export");
     return SR_OK;
+}
+
+// One select + vary step on host arrays (the evolve step of the drop-in
+// boundary): the packed population and its fitness in, the next generation
+// out in stride-cap layout.  Uses the context's GA storage.
+extern "C" int sr_evolve_step(sr_ctx *x, const sr_population *pop_in, const float *fitness, const sr_ga_params *p,
+                              const sr_select_params *sp, uint64_t seed, uint32_t gen, int32_t cap_out,
+                              int32_t *tok_out, float *val_out, int32_t *len_out)
+{
+    if (!x || !pop_in || !fitness || !p || !sp || !tok_out || !val_out || !len_out || pop_in->num_exprs <= 0 ||
+        !pop_in->tokens || !pop_in->offsets || !pop_in->lengths || (!pop_in->values && !pop_in->values_f32))
+        return SR_ERR_INVALID_ARG;
+    Ctx &c = x->c;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    hipSetDevice(c.device);
+    int rc = ga_configure(c, p);
+    if (rc) return rc;
+    const int cap = p->max_tokens + p->max_depth + 2;
+    if (cap_out != cap) return c.fail(SR_ERR_INVALID_ARG, "cap_out must equal max_tokens + max_depth + 2");
+    rc = validate_population(c, *pop_in);
+    if (rc) return rc;
+    const int E = pop_in->num_exprs;
+    std::vector<int32_t> tok((size_t)E * cap, 0), len(E);
+    std::vector<float> val((size_t)E * cap, 0.0f);
+    for (int e = 0; e < E; ++e) {
+        const int32_t L = pop_in->lengths[e], o = pop_in->offsets[e];
+        if (L > cap) return c.fail(SR_ERR_INVALID_ARG, "expression " + std::to_string(e) + " longer than cap");
+        len[e] = L;
+        for (int k = 0; k < L; ++k) {
+            tok[(size_t)e * cap + k] = pop_in->tokens[o + k];
+            val[(size_t)e * cap + k] = pop_in->values ? (float)pop_in->values[o + k] : pop_in->values_f32[o + k];
+        }
+    }
+    rc = ga_alloc(c, E, cap);
+    if (rc) return rc;
+    c.ga.cur = 0;
+    c.ga.ready = true;
+    if ((rc = sr_ga_set_population(x, tok.data(), val.data(), len.data(), E, cap))) return rc;
+    if ((rc = sr_ga_next(x, sp, fitness, seed, gen))) return rc;
+    int32_t np = 0, nc = 0;
+    return sr_ga_get_population(x, tok_out, val_out, len_out, &np, &nc);
 }

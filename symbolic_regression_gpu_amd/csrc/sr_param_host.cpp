@@ -307,6 +307,55 @@ int sr_param_info(sr_ctx *x, int32_t *num_exprs, int32_t *nslots_out, int64_t *t
     return SR_OK;
 }
 
+int sr_grad_consts(sr_ctx *x, const sr_population *pop, const uint8_t *const_mask, float *grad_out,
+                   float *mse_out)
+{
+    if (!x || !pop || !const_mask || !grad_out || pop->num_exprs <= 0 || !pop->tokens || !pop->offsets ||
+        !pop->lengths)
+        return SR_ERR_INVALID_ARG;
+    Ctx &c = x->c;
+    std::lock_guard<std::recursive_mutex> lk(c.mu);
+    hipSetDevice(c.device);
+    const int E = pop->num_exprs;
+    int64_t T = 0;
+    for (int e = 0; e < E; ++e) T = std::max<int64_t>(T, (int64_t)pop->offsets[e] + pop->lengths[e]);
+    // one parameter slot per masked constant, in token order
+    std::vector<int32_t> slot((size_t)T, -1);
+    for (int e = 0; e < E; ++e) {
+        int k = 0;
+        for (int32_t t = 0; t < pop->lengths[e]; ++t) {
+            const int64_t q = (int64_t)pop->offsets[e] + t;
+            if (!const_mask[q]) continue;
+            if (pop->tokens[q] != TOK_CONST)
+                return c.fail(SR_ERR_INVALID_ARG, "expression " + std::to_string(e) + ": mask on a non-constant token");
+            if (k >= MAX_SLOTS)
+                return c.fail(SR_ERR_UNSUPPORTED, "expression " + std::to_string(e) + ": more than 8 masked constants");
+            slot[q] = k++;
+        }
+    }
+    std::vector<float> cv((size_t)E * MAX_SLOTS, 0.0f), mse(E), g((size_t)E * MAX_SLOTS);
+    int rc = sr_param_prepare(x, pop, slot.data(), cv.data());
+    if (rc) return rc;
+    DevBuf d_c, d_m, d_g;
+    hipError_t he = d_c.ensure(cv.size() * 4);
+    if (he == hipSuccess) he = d_m.ensure((size_t)E * 4);
+    if (he == hipSuccess) he = d_g.ensure(g.size() * 4);
+    if (he == hipSuccess) he = hipMemcpyAsync(d_c.p, cv.data(), cv.size() * 4, hipMemcpyHostToDevice, c.stream);
+    if (he != hipSuccess) return c.hip_fail(he, "gradient buffers");
+    if ((rc = sr_param_grad(x, d_c.as<float>(), d_m.as<float>(), d_g.as<float>()))) return rc;
+    he = hipMemcpyAsync(g.data(), d_g.p, g.size() * 4, hipMemcpyDeviceToHost, c.stream);
+    if (he == hipSuccess) he = hipMemcpyAsync(mse.data(), d_m.p, (size_t)E * 4, hipMemcpyDeviceToHost, c.stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c.stream);
+    if (he != hipSuccess) return c.hip_fail(he, "gradient readback");
+    for (int e = 0; e < E; ++e)
+        for (int32_t t = 0; t < pop->lengths[e]; ++t) {
+            const int64_t q = (int64_t)pop->offsets[e] + t;
+            grad_out[q] = slot[q] >= 0 ? g[(size_t)e * MAX_SLOTS + slot[q]] : 0.0f;
+        }
+    if (mse_out) std::memcpy(mse_out, mse.data(), (size_t)E * 4);
+    return SR_OK;
+}
+
 int sr_sync(sr_ctx *x)
 {
     if (!x) return SR_ERR_INVALID_ARG;
