@@ -364,6 +364,11 @@ def main():
             traffic = json.load(f).get("hbm_bytes_per_launch")
     b_ms = batched0["interp_kernel_ms"]
     b_ach = flops0 / (b_ms * 1e-3) / 1e12
+    def xc_of(dpt):
+        """Variable-register class of the batched kernel variant (<= 4 variables)."""
+        return 3 if nv <= 3 and dpt == 3 else 4
+
+    cache_dpt = tuned.cache_dpt if tuned.cache_dpt > 0 else 3
     valu_ref = valu_mse = None
     if os.path.exists(VALU_FILE):
         with open(VALU_FILE) as f:
@@ -415,7 +420,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": traffic,
-            "kernel": f"k_batched<{tuned.dpt},9,4,MODE_MSE_REF> (cache skeletons), average over the "
+            "kernel": f"k_batched<{cache_dpt},9,{xc_of(cache_dpt)},MODE_MSE_REF> (cache skeletons), average over the "
                       "timed generations' launches",
             "kernel_ms": k_ms,
             "launches": launches,
@@ -428,7 +433,7 @@ def main():
         "roofline_batched": {
             "bound": "valu", "achieved": b_ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": b_ach / PEAK_FP32_TFLOPS, "traffic": b_traffic,
-            "kernel": f"k_batched<{tuned.dpt},9,4,MODE_MSE>, one launch = the initial 1000 expressions x 1M",
+            "kernel": f"k_batched<{tuned.dpt},9,{xc_of(tuned.dpt)},MODE_MSE>, one launch = the initial 1000 expressions x 1M",
             "kernel_ms": b_ms, "flops_per_launch": flops0,
             "hbm_algorithmic_bytes_per_launch": xy_bytes,
             "hbm_achieved_GBs": xy_bytes / (b_ms * 1e-3) / 1e9,

@@ -31,10 +31,11 @@ SR_DEV float qnan() { return __builtin_nanf(""); }
 // Variable registers per datapoint.  The backend reads an array of <= 8
 // floats at a uniform dynamic index through a compare/select chain over every
 // element, larger ones through GPR-index mode (one move).  With <= 4
-// variables the 4-element chain (3 selects) costs less than the 5 extra
-// registers per datapoint a 9-element array would take from occupancy.
+// variables the 4-element chain (3 selects; 2 for a 3-element array) costs
+// less than the 5 extra registers per datapoint a 9-element array would take
+// from occupancy.
 template <int XC>
-constexpr int kXRegs = XC > 8 ? XC : (XC == 4 ? 4 : 9);
+constexpr int kXRegs = XC > 8 ? XC : (XC <= 4 ? XC : 9);
 
 template <int S, int XC>
 struct Lane {

@@ -121,6 +121,7 @@ int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp)
     lp.dpt = mode == MODE_MSE_REF ? (c.tune.cache_dpt > 0 ? c.tune.cache_dpt : 3) : c.tune.dpt;
     if (lp.dpt < 1 || lp.dpt > 4) lp.dpt = 4;
     if (lp.dpt == 3 && lp.xc != 4) lp.dpt = 4;  // dpt 3 is built for <= 4 variables only
+    lp.xc = pick_xc_batched(c.nv, lp.dpt);
     if (lp.xc == 32 && lp.dpt > 2) lp.dpt = 2;
     lp.nq = c.tune.n_queues > 1 ? c.tune.n_queues : 1;
     // Chunks are multiples of 768 = lcm(64 * dpt) and queue slices of 1536,

@@ -36,5 +36,8 @@ hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t s);
 
 // Smallest register-cached variable count that holds nv variables.
 inline int pick_xc(int nv) { return nv <= 4 ? 4 : (nv <= 16 ? 16 : 32); }
+// The batched interpreter also has a 3-variable class for its 3-datapoint
+// variant (a variable operand is then 2 selects per datapoint instead of 3).
+inline int pick_xc_batched(int nv, int dpt) { return nv <= 3 && dpt == 3 ? 3 : pick_xc(nv); }
 
 }  // namespace sr
