@@ -332,3 +332,29 @@ def test_generational_cache_vs_fp64_subtree_oracle(ctx):
     finally:
         st64.close()
         ctx.cache_policy(CACHE_REFERENCE, 0, 4)
+
+
+@pytest.mark.parametrize("nv", [3, 4])
+def test_shared_policy_matches_batched_per_variable_class(ctx, nv):
+    """Cache skeletons run the 3-register kernel for nv <= 3 and the
+    4-register one for nv = 4; both equal the batched MSEs bit for bit."""
+    from symbolic_regression_gpu_amd.api import CACHE_GENERATIONAL_SHARED, CACHE_REFERENCE
+    base = random_population(1, 40 + nv, nv=nv, max_tokens=45, max_depth=7, unary=GRAMMAR_UNARY,
+                             binary=GRAMMAR_BINARY, min_len=30)
+    b = base.expr(0)
+    pop = O.Population([b] + mutants(b, 150, 40 + nv, nv))
+    N = 30_011
+    X = uniform_data(nv, N, 6)
+    ctx.set_data(X)
+    ctx.cache_policy(CACHE_GENERATIONAL_SHARED, 0, 3)
+    try:
+        api = to_api(pop)
+        m_b, v_b, _ = ctx.eval_mse(api, EVAL_BATCHED)
+        for _ in range(2):
+            m_c, v_c, st = ctx.eval_mse(api, EVAL_CACHED)
+            assert np.array_equal(m_c, m_b, equal_nan=True)
+            assert np.array_equal(v_c, v_b)
+        assert st["executed_node_evals"] < st["logical_node_evals"]
+        check_population(pop, X, nv, N, m_c, v_c)
+    finally:
+        ctx.cache_policy(CACHE_REFERENCE, 0, 4)

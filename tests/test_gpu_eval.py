@@ -56,7 +56,9 @@ def test_single_expression_predictions_c1_shape(ctx):
         assert st["logical_node_evals"] == int(pop.len.sum()) * N
 
 
-@pytest.mark.parametrize("nv", [1, 3, 10, 20])
+# variable classes of the batched kernel at its default shape: nv <= 3 -> 3
+# registers, 4 -> 4, 5..16 -> 16, 17..32 -> 32
+@pytest.mark.parametrize("nv", [1, 3, 4, 10, 20, 32])
 def test_population_mse_all_ops(ctx, nv):
     pop = random_population(120, 100 + nv, nv=nv, max_tokens=60, max_depth=7,
                             unary=ALL_UNARY, binary=ALL_BINARY, const_min=-3, const_max=3)
