@@ -72,6 +72,7 @@ SR_DEV void apply_unary(int op, L &...l)
     case OP_SQRT: ((l.top = f_sqrt(l.top)), ...); break;
     case OP_LOOSE_SQRT: ((l.top = f_loose_sqrt(l.top)), ...); break;
     case OP_EXP_CLAMPED: ((l.top = f_exp_clamped(l.top)), ...); break;
+    case OPC_STORE: break;  // the cache store pseudo-op leaves top unchanged
     default: ((l.top = 0.0f), ...); break;
     }
 }

@@ -105,10 +105,10 @@ SR_DEV void interp_step(int code, int arg, float cv, int &sp, RefLoad &&ref, L &
 constexpr int kProgLds = MAX_EXPR_LEN + 1;
 
 // Waves per SIMD the register allocation must leave room for.  Occupancy
-// decides this latency-bound interpreter: the 3-datapoint variant runs 4 waves
-// per SIMD (<= 128 VGPRs, a few spills) for MSE and 3 for cache skeletons;
-// with 4-element variable arrays the 4-datapoint MSE variant fits 3 waves.
-constexpr int batched_min_waves(int dpt, int mode) { return dpt == 3 ? (mode == MODE_MSE ? 4 : 3) : 1; }
+// decides this interpreter: the 3-datapoint variant runs 4 waves per SIMD
+// (<= 128 VGPRs; the cache-skeleton mode spills a few) in every mode; the
+// 4-datapoint variant fits 3.
+constexpr int batched_min_waves(int dpt, int) { return dpt == 3 ? 4 : 1; }
 
 template <int DPT, int S, int XC, int MODE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(batched_min_waves(DPT, MODE))))
@@ -217,6 +217,11 @@ void k_batched(BatchArgs a)
                 };
                 for (int i = 0; i < n; ++i) {
                     const Ins nx = lp[i + 1];
+                    // OP_STORE runs through the step as a unary no-op (no
+                    // operand, top kept), then writes top to its column: a
+                    // branch around the whole step would make the compiler
+                    // copy the stack registers between the two paths.
+                    interp_step<has_ref(MODE)>(code, arg, __int_as_float(arg), sp, ref, L...);
                     if (has_ref(MODE) && code == STORE_CODE) {
                         float *cs = a.out + (int64_t)arg * N + t0;
                         for_each_lane(
@@ -225,8 +230,6 @@ void k_batched(BatchArgs a)
                                 if (i <= last) cs[i] = l.top;
                             },
                             L...);
-                    } else {
-                        interp_step<has_ref(MODE)>(code, arg, __int_as_float(arg), sp, ref, L...);
                     }
                     code = __builtin_amdgcn_readfirstlane(nx.code);
                     arg = __builtin_amdgcn_readfirstlane(nx.arg);
