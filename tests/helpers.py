@@ -7,7 +7,7 @@ import numpy as np
 
 from oracle import oracle as O
 
-# BASELINE grammar: binary {+,-,*,protected /}, unary {sin, cos, exp(clamped),
+# The bench grammar: binary {+,-,*,protected /}, unary {sin, cos, exp(clamped),
 # protected log, sqrt(abs)}.
 GRAMMAR_UNARY = [10, 11, 29, 22, 27]
 GRAMMAR_BINARY = [1, 2, 3, 8]
