@@ -1,7 +1,7 @@
 /*
  * sr_oracle.c -- CPU restatement of the reference hot path (test infrastructure).
  *
- * This is the checker for the HIP product path and its CPU timing baseline.
+ * It is the oracle of the HIP product path and its CPU timing baseline.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
  * See sr_oracle.h for what pins each function.
  *

@@ -1,7 +1,7 @@
 /*
  * sr_oracle.h -- CPU restatement of the reference symbolic-regression hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle of the HIP product
  * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
  * may load it.  Nothing in symbolic_regression_gpu_amd/ links or calls it.
  *
