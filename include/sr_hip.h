@@ -89,6 +89,7 @@ typedef struct {
     int64_t executed_op_evals;   /* operator evaluations run (operator tokens * N; 0 = not counted) */
     int64_t column_reads;        /* cached-column operands read (datapoints) */
     int64_t column_writes;       /* cached-column values written (datapoints) */
+    int32_t graph_captures;      /* hipGraphs captured by this call (sr_ga_run) */
 } sr_stats;
 
 /* Launch shape of the batched interpreter. */

@@ -445,7 +445,66 @@ def ga_op_with_hooks(op, p: GaParams, a, b, uniform_py, randint_py):
     return tok[:n].copy(), val[:n].copy()
 
 
+class _PhiloxStreamC(C.Structure):
+    _fields_ = [("key", C.c_uint32 * 2), ("c1", C.c_uint32), ("c2", C.c_uint32), ("c3", C.c_uint32),
+                ("word", C.c_uint32)]
+
+
+class PhiloxStream:
+    """The oracle's Philox4x32-10 stream (seed, stream, gen, domain) behind its
+    own sro_rng binding: uniform(lo, hi) / randint(lo, hi) as the GA draws."""
+
+    def __init__(self, seed, stream, gen, domain):
+        L = lib()
+        self._st = _PhiloxStreamC()
+        self._r = RngHooks()
+        L.sro_philox_stream_init(C.byref(self._st), C.c_uint64(seed), C.c_uint32(stream), C.c_uint32(gen),
+                                 C.c_uint32(domain))
+        L.sro_rng_bind_philox(C.byref(self._r), C.byref(self._st))
+
+    def uniform(self, lo, hi):
+        return self._r.uniform(C.addressof(self._r), lo, hi)
+
+    def randint(self, lo, hi):
+        return self._r.randint(C.addressof(self._r), lo, hi)
+
+
+CHILD_RNG_FN = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_int)
+
+
+def ga_next_generation_hooked(p: GaParams, tournament, p_mutation, elite_frac, pop, cap, tok, val, ln, fit,
+                              rng_for_child):
+    """sro_ga_next_generation_hooked: rng_for_child(c) returns (uniform_py,
+    randint_py) for child c.  Returns (rc, tok, val, len, parent1, parent2)."""
+    L = lib()
+    sp = SelectParams(tournament, p_mutation, elite_frac)
+    keep = []
+
+    def child(_ud, c):
+        u, r = rng_for_child(c)
+        h = RngHooks(UNIFORM_FN(lambda _s, lo, hi: u(lo, hi)), RANDINT_FN(lambda _s, lo, hi: r(lo, hi)), None)
+        keep.append(h)
+        return C.addressof(h)
+
+    cb = CHILD_RNG_FN(child)
+    otok = np.zeros(pop * cap, np.int32)
+    oval = np.zeros(pop * cap, np.float32)
+    oln = np.zeros(pop, np.int32)
+    p1 = np.zeros(pop, np.int32)
+    p2 = np.zeros(pop, np.int32)
+    rc = L.sro_ga_next_generation_hooked(
+        C.byref(p), C.byref(sp), pop, cap, np.ascontiguousarray(tok, np.int32),
+        np.ascontiguousarray(val, np.float32), np.ascontiguousarray(ln, np.int32),
+        np.ascontiguousarray(fit, np.float32), cb, None, otok, oval, oln, p1, p2)
+    return rc, otok, oval, oln, p1, p2
+
+
 def _bind_ga_hooks(L):
+    L.sro_philox_stream_init.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
+    L.sro_rng_bind_philox.argtypes = [C.c_void_p, C.c_void_p]
+    L.sro_ga_next_generation_hooked.argtypes = [
+        C.POINTER(GaParams), C.POINTER(SelectParams), C.c_int, C.c_int, _i32p, _f32p, _i32p, _f32p,
+        CHILD_RNG_FN, C.c_void_p, _i32p, _f32p, _i32p, _i32p, _i32p]
     L.sro_ga_generate.argtypes = [C.POINTER(GaParams), C.c_int, C.POINTER(RngHooks), _i32p, _f32p]
     L.sro_ga_mutate.argtypes = [C.POINTER(GaParams), _i32p, _f32p, C.c_int, C.POINTER(RngHooks),
                                 _i32p, _f32p]

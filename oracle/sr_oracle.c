@@ -1798,10 +1798,14 @@ static int tournament(sro_rng *r, const float *sfit, int pop, int t)
     return best;
 }
 
-int sro_ga_next_generation(const sro_ga_params *p, const sro_select_params *sp, int pop,
-                           int cap, const int *tok, const float *val, const int *len,
-                           const float *fit, uint64_t seed, uint32_t gen, int *out_tok,
-                           float *out_val, int *out_len)
+/* Selection + variation loop of cl_sr_system.cpp:285-330 with the random
+ * stream of child c supplied by rng_for(ud, c); parent1 / parent2 (nullable)
+ * receive the parents of every child (an elite: its index and -2; a mutant:
+ * -1 as parent2). */
+static int next_generation(const sro_ga_params *p, const sro_select_params *sp, int pop, int cap,
+                           const int *tok, const float *val, const int *len, const float *fit,
+                           sro_rng *(*rng_for)(void *, int), void *ud, int *out_tok, float *out_val,
+                           int *out_len, int *parent1, int *parent2)
 {
     int *order = (int *)malloc(sizeof(int) * (size_t)pop);
     float *sfit = (float *)malloc(sizeof(float) * (size_t)pop);
@@ -1818,22 +1822,20 @@ int sro_ga_next_generation(const sro_ga_params *p, const sro_select_params *sp, 
         if (c < elites) {
             int s = order[c];
             n = copy_expr(tok + (size_t)s * cap, val + (size_t)s * cap, len[s], ct, cv);
+            if (parent1) parent1[c] = s;
+            if (parent2) parent2[c] = -2;
         } else {
-            sro_philox_stream st;
-            sro_rng r;
-            sro_philox_stream_init(&st, seed, (uint32_t)c, gen, 1u);
-            sro_rng_bind_philox(&r, &st);
-            int b1 = order[tournament(&r, sfit, pop, sp->tournament)];
+            sro_rng *r = rng_for(ud, c);
+            int b1 = order[tournament(r, sfit, pop, sp->tournament)];
+            int b2 = -1;
             int tmp[SRO_GA_TMP];
             float tv[SRO_GA_TMP];
-            if (r.uniform(&r, 0.0f, 1.0f) < sp->p_mutation) {
-                n = sro_ga_mutate(p, tok + (size_t)b1 * cap, val + (size_t)b1 * cap, len[b1],
-                                  &r, tmp, tv);
+            if (r->uniform(r, 0.0f, 1.0f) < sp->p_mutation) {
+                n = sro_ga_mutate(p, tok + (size_t)b1 * cap, val + (size_t)b1 * cap, len[b1], r, tmp, tv);
             } else {
-                int b2 = order[tournament(&r, sfit, pop, sp->tournament)];
+                b2 = order[tournament(r, sfit, pop, sp->tournament)];
                 n = sro_ga_crossover(p, tok + (size_t)b1 * cap, val + (size_t)b1 * cap, len[b1],
-                                     tok + (size_t)b2 * cap, val + (size_t)b2 * cap, len[b2],
-                                     &r, tmp, tv);
+                                     tok + (size_t)b2 * cap, val + (size_t)b2 * cap, len[b2], r, tmp, tv);
             }
             if (n > cap) {
                 rc = -1;
@@ -1841,12 +1843,51 @@ int sro_ga_next_generation(const sro_ga_params *p, const sro_select_params *sp, 
             }
             memcpy(ct, tmp, sizeof(int) * (size_t)n);
             memcpy(cv, tv, sizeof(float) * (size_t)n);
+            if (parent1) parent1[c] = b1;
+            if (parent2) parent2[c] = b2;
         }
         out_len[c] = n;
     }
     free(order);
     free(sfit);
     return rc;
+}
+
+/* Per-child Philox streams (seed, child, gen, domain 1). */
+typedef struct {
+    uint64_t seed;
+    uint32_t gen;
+    sro_philox_stream st;
+    sro_rng r;
+} philox_children;
+
+static sro_rng *philox_child(void *ud, int c)
+{
+    philox_children *pc = (philox_children *)ud;
+    sro_philox_stream_init(&pc->st, pc->seed, (uint32_t)c, pc->gen, 1u);
+    sro_rng_bind_philox(&pc->r, &pc->st);
+    return &pc->r;
+}
+
+int sro_ga_next_generation(const sro_ga_params *p, const sro_select_params *sp, int pop,
+                           int cap, const int *tok, const float *val, const int *len,
+                           const float *fit, uint64_t seed, uint32_t gen, int *out_tok,
+                           float *out_val, int *out_len)
+{
+    philox_children pc;
+    pc.seed = seed;
+    pc.gen = gen;
+    return next_generation(p, sp, pop, cap, tok, val, len, fit, philox_child, &pc, out_tok, out_val, out_len,
+                           NULL, NULL);
+}
+
+int sro_ga_next_generation_hooked(const sro_ga_params *p, const sro_select_params *sp, int pop, int cap,
+                                  const int *tok, const float *val, const int *len, const float *fit,
+                                  sro_rng *(*rng_for)(void *ud, int child), void *ud, int *out_tok,
+                                  float *out_val, int *out_len, int *parent1, int *parent2)
+{
+    return next_generation(p, sp, pop, cap, tok, val, len, fit, rng_for, ud, out_tok, out_val, out_len, parent1,
+                           parent2);
 }
 
 int sro_ga_init_population(const sro_ga_params *p, int pop, int cap, uint64_t seed,

@@ -18,11 +18,18 @@
  *   sro_detect
  *       pinned: detect_common_subtrees (detect.cpp:120-387) compiled in place,
  *       compared as hash sets / occurrence sets / hint maps modulo ids.
- *   sro_mark_cached / sro_skeleton / sro_cache_update
- *       pinned by brute force (O(n^2) subtree comparison) and invariants; the
- *       reference implementation (cl_gpu_eval.cu:294-329,384-397,433-459) is
- *       CUDA source that cannot be compiled here.  Slots compare bodies as
- *       well as flat hashes (the reference's flat-hash-only keys collide).
+ *   sro_mark_cached / sro_cache_update / sro_skeleton / sro_predict32_cached
+ *       the reference implementation (cl_gpu_eval.cu:165-186,294-329,384-397,
+ *       433-459) is CUDA source that cannot be compiled here.  Pinned by
+ *       tests/test_oracle_pins.py against brute-force models written there:
+ *       hint maps equal the unique outermost-first marking of cached bodies
+ *       (every subtree compared with every cached body); new slots follow the
+ *       detected-subtree order up to capacity, skipping cached bodies (forced
+ *       flat-hash collisions included: slots compare bodies as well as flat
+ *       hashes, the reference's flat-hash-only keys collide); a skeleton
+ *       expanded by its slots' bodies is the original stream, and evaluated
+ *       over predict32 columns it equals predict32 of the expression bit for
+ *       bit.
  *   sro_subtree_predict64
  *       pinned: eval_cpu_common_subtree_batch (cpu_subtree.cpp:135-340)
  *       compiled in place, FP64 predictions bit for bit; equal to
@@ -39,10 +46,16 @@
  *       distributions, outputs equal the reference's
  *       generate_random_expression / mutate_expression / crossover_expressions
  *       (utils.cpp:986-1272) compiled in place, bit for bit.
- *   sro_ga_next_generation (sort + elitism + tournament, cl_sr_system.cpp:285-330)
- *       pinned by invariants (elites = best-k, tournament winner = min fitness
- *       of its draws); the reference code sits inside main() and cannot be
- *       called.
+ *   sro_fitness_order / sro_ga_next_generation(_hooked) (sort + elitism +
+ *       tournament + variation choice, cl_sr_system.cpp:285-330; the reference
+ *       code sits inside main() and cannot be called): pinned by
+ *       tests/test_oracle_pins.py with scripted draws: order = a stable sort
+ *       with NaN last; elites = the best max(1, floor(pop * frac)) copied;
+ *       per child t draws in [0, pop), strict < keeps the earlier draw (a NaN
+ *       drawn first is never displaced, as in the reference), mutation iff the
+ *       next uniform draw < p_mutation, a second tournament only for
+ *       crossover, and the child equals the pinned mutate / crossover replayed
+ *       on the remaining draws with those parents.
  *   sro_op32 for SR_OP_EXP_CLAMPED (a new opcode 29, not in the reference
  *       evaluator) follows the clamped exp of evolution_tests.cpp:40; parity
  *       for it is closed-form only.
@@ -269,6 +282,14 @@ int sro_ga_next_generation(const sro_ga_params *p, const sro_select_params *sp, 
                            int cap, const int *tok, const float *val, const int *len,
                            const float *fit, uint64_t seed, uint32_t gen, int *out_tok,
                            float *out_val, int *out_len);
+/* The same loop with the random stream of child c supplied by rng_for(ud, c)
+ * (so a test can script every draw); parent1 / parent2 (nullable, [pop])
+ * receive each child's parents: an elite gets its index and -2, a mutant -1
+ * as parent2. */
+int sro_ga_next_generation_hooked(const sro_ga_params *p, const sro_select_params *sp, int pop, int cap,
+                                  const int *tok, const float *val, const int *len, const float *fit,
+                                  sro_rng *(*rng_for)(void *ud, int child), void *ud, int *out_tok,
+                                  float *out_val, int *out_len, int *parent1, int *parent2);
 /* Initial population: individual i uses stream (seed, i, 0xFFFFFFFF, domain 2). */
 int sro_ga_init_population(const sro_ga_params *p, int pop, int cap, uint64_t seed,
                            int *out_tok, float *out_val, int *out_len);

@@ -1,6 +1,6 @@
 """MI355X-native symbolic-regression evaluator (HIP kernels behind a C ABI).
 
-The compute path is libsrhip.so (csrc/), loaded through ctypes by _lib.py.
+The compute path is libsrhip.so (csrc/), loaded through ctypes by api.py.
 Importing the package never falls back to a CPU evaluator: when the shared
 library is missing, the first call raises.
 """

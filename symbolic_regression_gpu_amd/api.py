@@ -53,7 +53,7 @@ class _Stats(C.Structure):
         ("num_kernel_launches", C.c_int32), ("logical_node_evals", C.c_int64),
         ("executed_node_evals", C.c_int64), ("interp_kernel_ms", C.c_double),
         ("interp_launches", C.c_int32), ("executed_op_evals", C.c_int64),
-        ("column_reads", C.c_int64), ("column_writes", C.c_int64),
+        ("column_reads", C.c_int64), ("column_writes", C.c_int64), ("graph_captures", C.c_int32),
     ]
 
     def as_dict(self):

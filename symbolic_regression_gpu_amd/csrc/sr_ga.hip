@@ -68,7 +68,10 @@ __global__ void k_ga_next(GaParams p, SelectParams s, int pop, int cap, const in
 
 // One thread per expression: validate and compile into the batched
 // interpreter's bytecode (program e at progs + e*cap).  A malformed
-// expression sets status bit 4 and gets the program "LOAD NaN".
+// expression sets status bit 4, one needing more than 8 stack slots bit 8
+// (that takes >= 2047 tokens: a need of k slots needs two subtrees of need
+// k - 1 below one node, and need 0 is one operator over two leaves); either
+// gets the program "LOAD NaN".
 __global__ void k_compile_pop(const int32_t *tok, const float *val, const int32_t *len, int pop,
                               int cap, int nv, Ins *progs, int32_t *prog_len, int32_t *size_scratch,
                               int32_t *need_scratch, CompileFrame *frames, int32_t *max_need,
@@ -88,7 +91,7 @@ __global__ void k_compile_pop(const int32_t *tok, const float *val, const int32_
         n = compile_expr(t, v, L, nv, size_scratch + (size_t)e * cap, need_scratch + (size_t)e * cap,
                          frames + (size_t)e * (cap + 1), out, &sn);
     if (n < 0 || sn > 8) {
-        atomicOr((unsigned *)status, 4u);
+        atomicOr((unsigned *)status, n < 0 ? 4u : 8u);
         out[0].code = 0 | (SRC_CONST << 8);
         out[0].arg = f32_bits(__builtin_nanf(""));
         n = 1;

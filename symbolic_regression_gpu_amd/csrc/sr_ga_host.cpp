@@ -62,6 +62,7 @@ static int ga_status(Ctx &c)
     if (st) {
         hipMemset(c.ga.status.p, 0, 4);
         if (st & 4) return c.fail(SR_ERR_BAD_EXPR, "GA population holds a malformed expression");
+        if (st & 8) return c.fail(SR_ERR_UNSUPPORTED, "GA expression needs more than 8 stack slots");
         return c.fail(SR_ERR_CAPACITY, "GA individual exceeded its token capacity");
     }
     return SR_OK;
@@ -406,11 +407,16 @@ int sr_ga_run(sr_ctx *x, const sr_select_params *spp, int mode, int32_t gens, ui
 
     if (use_graph) {
         uint64_t key = 0;
-        for (uint64_t v : {(uint64_t)g.pop, (uint64_t)g.cap, (uint64_t)c.N, (uint64_t)c.nv, (uint64_t)(uintptr_t)c.X,
-                           (uint64_t)lp.dpt, (uint64_t)lp.chunk, (uint64_t)lp.nq, (uint64_t)lp.grid, seed,
-                           (uint64_t)sp.tournament, (uint64_t)sp.elites, (uint64_t)gen0, (uint64_t)gens,
-                           (uint64_t)(uintptr_t)g.hist.p, (uint64_t)(uintptr_t)c.d_mse.p,
-                           (uint64_t)(uintptr_t)c.d_psum.p, (uint64_t)(uintptr_t)s})
+        // every pointer a captured node holds: a buffer reallocated since the
+        // capture (any of them, whichever grew) forces a new capture
+        const void *ptrs[] = {g.tok[0].p, g.tok[1].p, g.val[0].p, g.val[1].p, g.len[0].p, g.len[1].p,
+                              g.progs.p, g.prog_len.p, g.size_s.p, g.need_s.p, g.frames.p, g.order_len.p,
+                              g.order_fit.p, g.status.p, g.max_need.p, g.gen.p, g.hist.p, g.hist_len.p,
+                              c.X, c.d_mse.p, c.d_cnt.p, c.d_psum.p, c.d_pcnt.p, c.d_counters.p, (const void *)s};
+        for (const void *q : ptrs) key = mix(key, (uint64_t)(uintptr_t)q);
+        for (uint64_t v : {(uint64_t)g.pop, (uint64_t)g.cap, (uint64_t)c.N, (uint64_t)c.nv, (uint64_t)lp.dpt,
+                           (uint64_t)lp.chunk, (uint64_t)lp.nq, (uint64_t)lp.grid, (uint64_t)lp.P, seed,
+                           (uint64_t)sp.tournament, (uint64_t)sp.elites, (uint64_t)gen0, (uint64_t)gens})
             key = mix(key, v);
         uint32_t pm;
         std::memcpy(&pm, &sp.p_mutation, 4);
@@ -433,6 +439,7 @@ int sr_ga_run(sr_ctx *x, const sr_select_params *spp, int mode, int32_t gens, ui
                 if (he != hipSuccess) return c.hip_fail(he, "graph instantiate");
             }
             g.graph_key = key;
+            acc.graph_captures += 2;
         }
     }
     if ((he = launch_u32_set(g.gen.as<uint32_t>(), gen0, s)) != hipSuccess) return c.hip_fail(he, "gen set");

@@ -29,7 +29,10 @@ def test_null_arguments_are_status_codes():
     L = api.lib()
     assert L.sr_ctx_create(0, None) == -1
     assert L.sr_ctx_destroy(None) == 0
-    assert L.sr_set_data(None, None, 0, 0) == -1 if False else True  # typed binding rejects None
+    raw = C.CDLL(api.LIB_PATH)  # untyped symbols: null pointers pass through ctypes
+    assert raw.sr_set_data(None, None, 0, C.c_int64(0)) == -1
+    assert raw.sr_eval_mse(None, None, 0, None, None, None) == -1
+    assert raw.sr_ga_run(None, None, 0, 0, 0, C.c_uint64(0), 0, None, None, None) == -1
 
 
 def test_no_device_is_a_status_code():

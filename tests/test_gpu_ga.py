@@ -108,6 +108,37 @@ def test_graph_and_stream_runs_agree(ctx):
         assert np.array_equal(p[2], out[0][1][2])
 
 
+def test_graph_recaptured_after_a_buffer_grows():
+    """A captured generation bakes in the context's evaluation buffers; when
+    an evaluation in between grows any of them, the next graph run must
+    capture again (and still match the stream run).  A context of its own, so
+    the buffer sizes are this test's."""
+    from symbolic_regression_gpu_amd import Context
+    from tests.helpers import random_population, to_api
+    ctx = Context(0)
+    gp, _ = _params(**C4)
+    X = uniform_data(3, 8192, 3)
+    ctx.set_data(X)
+    sel = SelectParams(7, 0.1, 0.05)
+    ctx.ga_init(gp, 300, 9)
+    start = ctx.ga_population()
+    _, _, st1 = ctx.ga_run(sel, EVAL_BATCHED, 2, 0, 9, use_graph=True)
+    ctx.ga_set_population(*start)
+    h2, _, st2 = ctx.ga_run(sel, EVAL_BATCHED, 2, 0, 9, use_graph=True)
+    assert st2["graph_captures"] == 0  # same buffers: replayed
+    # 1.25x + 40 expressions: the count / partial buffers grow
+    big = random_population(300 * 5 // 4 + 40, 3, nv=3, max_tokens=30, max_depth=5)
+    ctx.eval_mse(to_api(big), EVAL_BATCHED)
+    ctx.ga_set_population(*start)
+    h3, _, st3 = ctx.ga_run(sel, EVAL_BATCHED, 2, 0, 9, use_graph=True)
+    assert st3["graph_captures"] == 2
+    ctx.ga_set_population(*start)
+    h4, _, _ = ctx.ga_run(sel, EVAL_BATCHED, 2, 0, 9, use_graph=False)
+    assert np.array_equal(h2, h3, equal_nan=True) and np.array_equal(h3, h4, equal_nan=True)
+    assert st1["graph_captures"] == 2
+    ctx.close()
+
+
 def test_host_mode_evaluation_matches_device(ctx):
     gp, _ = _params(**C4)
     X = uniform_data(3, 5000, 8)
