@@ -120,6 +120,25 @@ int sr_set_data_device(sr_ctx *ctx, const float *d_X_colmajor, int32_t nv, int64
  * prediction counts; stats optional. */
 int sr_eval_mse(sr_ctx *ctx, const sr_population *pop, int mode, float *mse_out,
                 int64_t *valid_out, sr_stats *stats);
+/* Allocation-free fitness.  No counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  SR_EVAL_BATCHED on a
+ * packed population held in DEVICE memory: d_tokens / d_values (fp32) of
+ * num_tokens entries, d_offsets / d_lengths [num_exprs]; outputs d_mse and
+ * d_valid (optional) [num_exprs] in device memory.  Only enqueues work on the
+ * context stream (no synchronisation, no hipMalloc / hipFree, no memset
+ * nodes): programs are compiled on the device, and every intermediate buffer
+ * lives in the library's static device scratch of 64 MiB per device, which
+ * bounds a call: programs take 8 B and compile scratch 28 B per token, MSE
+ * partials 16 B per (expression, datapoint chunk).  A call whose layout does
+ * not fit returns SR_ERR_CAPACITY.  The sum of the lengths must not exceed
+ * num_tokens.  MSEs and counts equal sr_eval_mse (SR_EVAL_BATCHED) bit for
+ * bit; an expression sr_eval_mse would reject (malformed, outside the token
+ * arrays) gets MSE NaN and count 0 instead of an error.  Every call on a
+ * device shares that device's scratch: calls that overlap on one device race. */
+int sr_eval_mse_noalloc(sr_ctx *ctx, int32_t num_exprs, int64_t num_tokens, const int32_t *d_tokens,
+                        const float *d_values, const int32_t *d_offsets, const int32_t *d_lengths, float *d_mse,
+                        int64_t *d_valid);
 /* Predictions, host [num_exprs][N]. */
 int sr_eval_predictions(sr_ctx *ctx, const sr_population *pop, float *pred_out);
 

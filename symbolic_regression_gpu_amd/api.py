@@ -104,6 +104,7 @@ _SIGNATURES = {
     "sr_set_data": [_vp, _f32p, C.c_int32, C.c_int64],
     "sr_set_data_device": [_vp, _vp, C.c_int32, C.c_int64],
     "sr_eval_mse": [_vp, C.POINTER(_Pop), C.c_int, _f32p, _i64p, C.POINTER(_Stats)],
+    "sr_eval_mse_noalloc": [_vp, C.c_int32, C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp],
     "sr_eval_predictions": [_vp, C.POINTER(_Pop), _f32p],
     "sr_get_tuning": [_vp, C.POINTER(_Tuning)],
     "sr_autotune": [_vp, C.POINTER(_Pop), C.c_int32, C.POINTER(_Tuning), C.POINTER(C.c_double)],
@@ -503,6 +504,14 @@ class Context:
         st = _Stats()
         self._check(self._L.sr_eval_mse(self._h, C.byref(pop._c()), mode, mse, valid, C.byref(st)))
         return mse[:E], valid[:E], st.as_dict()
+
+    def eval_mse_noalloc(self, tok, val, off, ln, mse, valid=None):
+        """sr_eval_mse_noalloc on device tensors (int32 tok / off / ln, float32
+        val, float32 mse [E], int64 valid [E] or None); only enqueues on the
+        context stream -- call sync() before reading the outputs."""
+        self._check(self._L.sr_eval_mse_noalloc(
+            self._h, int(ln.numel()), int(tok.numel()), tok.data_ptr(), val.data_ptr(), off.data_ptr(),
+            ln.data_ptr(), mse.data_ptr(), valid.data_ptr() if valid is not None else None))
 
     def eval_predictions(self, pop: Population):
         out = np.empty(max(pop.num_exprs, 1) * self.N, np.float32)

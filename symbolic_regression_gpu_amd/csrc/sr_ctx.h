@@ -180,8 +180,13 @@ struct JitState {
     ~JitState();
 };
 
+// Device address of the static scratch array of sr_eval_mse_noalloc on
+// `device` (sr_noalloc.hip); call with that device current.
+hipError_t noalloc_scratch_init(int device, void **out);
+
 struct Ctx {
     int device = 0;
+    void *noalloc_scratch = nullptr;  // this device's static scratch (sr_eval_mse_noalloc)
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
     int num_cu = 0;
@@ -249,14 +254,28 @@ struct LaunchPlan {
     int64_t slice = 0, chunk = 0;
 };
 struct BatchArgs;
-// Pick the variant + work decomposition for E expressions and make sure the
-// partial / result buffers exist.
+// Pick the variant + work decomposition (no allocation).
+int plan_launch(Ctx &c, int mode, LaunchPlan &lp);
+// plan_launch for E expressions, and make sure the context's partial /
+// result buffers exist.
 int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp);
+// Device buffers one batched evaluation writes: MSE partials [E][P], the
+// per-XCD work counters, and (MSE modes) the finalized mse / count [E].
+struct EvalBufs {
+    double *psum;
+    long long *pcnt;
+    uint32_t *counters;
+    float *mse;
+    long long *cnt;
+};
 // Enqueue counter reset + batched kernel (+ finalize for MODE_MSE).  a must
 // carry the program / order / output pointers; the rest is filled in here.
 // k0/k1 (optional) are recorded right before and after the interpreter launch.
 int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, hipStream_t s,
                     hipEvent_t k0 = nullptr, hipEvent_t k1 = nullptr);
+// Same into explicit buffers (the context's own are not touched).
+int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, hipStream_t s,
+                    const EvalBufs &b, hipEvent_t k0 = nullptr, hipEvent_t k1 = nullptr);
 
 // Batched evaluation over a compiled program already described by host
 // vectors; uploads, launches, finalizes.  mode: MODE_MSE/MODE_PRED/MODE_COLUMN.

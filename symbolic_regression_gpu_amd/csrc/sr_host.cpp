@@ -114,9 +114,8 @@ int build_program(Ctx &c, const sr_population &pop, bool raw, Program &out, int 
 
 static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
-int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp)
+int plan_launch(Ctx &c, int mode, LaunchPlan &lp)
 {
-    hipError_t he;
     lp.xc = pick_xc(c.nv);
     lp.dpt = mode == MODE_MSE_REF ? (c.tune.cache_dpt > 0 ? c.tune.cache_dpt : 3) : c.tune.dpt;
     if (lp.dpt < 1 || lp.dpt > 4) lp.dpt = 4;
@@ -139,6 +138,14 @@ int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp)
     int bpc = c.tune.blocks_per_cu;
     if (bpc <= 0 || bpc > occ) bpc = occ;
     lp.grid = (int)round_up((int64_t)c.num_cu * bpc, lp.nq);
+    return SR_OK;
+}
+
+int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp)
+{
+    hipError_t he;
+    int rc = plan_launch(c, mode, lp);
+    if (rc) return rc;
     if ((he = c.d_counters.ensure(64 * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
     if (is_mse_mode(mode)) {
         if ((he = c.d_psum.ensure((size_t)E * lp.P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
@@ -152,11 +159,19 @@ int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp)
 int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, hipStream_t s,
                     hipEvent_t k0, hipEvent_t k1)
 {
+    const EvalBufs b{c.d_psum.as<double>(), c.d_pcnt.as<long long>(), c.d_counters.as<uint32_t>(),
+                     c.d_mse.as<float>(), c.d_cnt.as<long long>()};
+    return enqueue_batched(c, mode, lp, a, E, s, b, k0, k1);
+}
+
+int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, hipStream_t s,
+                    const EvalBufs &b, hipEvent_t k0, hipEvent_t k1)
+{
     hipError_t he;
     a.X = c.X;
-    a.psum = c.d_psum.as<double>();
-    a.pcnt = c.d_pcnt.as<long long>();
-    a.counters = c.d_counters.as<uint32_t>();
+    a.psum = b.psum;
+    a.pcnt = b.pcnt;
+    a.counters = b.counters;
     a.N = c.N;
     a.slice = lp.slice;
     a.chunk = lp.chunk;
@@ -171,8 +186,7 @@ int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, 
         return c.hip_fail(he, "batched launch");
     if (k1) hipEventRecord(k1, s);
     if (is_mse_mode(mode)) {
-        if ((he = launch_finalize(a.psum, a.pcnt, lp.P, E, c.d_mse.as<float>(), c.d_cnt.as<long long>(),
-                                  s)) != hipSuccess)
+        if ((he = launch_finalize(a.psum, a.pcnt, lp.P, E, b.mse, b.cnt, s)) != hipSuccess)
             return c.hip_fail(he, "finalize launch");
     }
     return SR_OK;
@@ -372,6 +386,10 @@ int sr_ctx_create(int device, sr_ctx **out)
     c.stream = c.own_stream;
     for (auto &e : c.ev) hipEventCreate(&e);
     for (auto &e : c.ev2) hipEventCreate(&e);
+    if (noalloc_scratch_init(device, &c.noalloc_scratch) != hipSuccess) {
+        sr_ctx_destroy(x);
+        return SR_ERR_HIP;
+    }
     *out = x;
     return SR_OK;
 }
