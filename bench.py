@@ -7,14 +7,16 @@ tree (the reference's evaluate -> fitness -> select -> vary step,
 cl_sr_system.cpp:168-333).
 
 One step = one generation: the population is evaluated by the batched
-persistent interpreter over the generational subtree cache (host plan, one or
-two device launches, fused MSE), then ranked, selected and varied on the
-device.  `value` = logical node-evals/s of the timed generations (sum_e len_e
-* N per generation / wall time, host planning and transfers included);
-`ms_per_step` = ms per generation.  The same generations are also run with
-the uncached batched evaluation replayed as one hipGraph per generation (their
-fitness vectors must be identical), and the plain one-launch-per-expression
-interpreter is timed on the initial and the final population.  Beside them:
+persistent interpreter over the generational subtree cache planned on the
+device (hash, look up, evict, allocate, build and compile skeletons, two
+interpreter launches, fused MSE), then ranked, selected and varied on the
+device -- one captured hipGraph per generation, no host work.  `value` =
+logical node-evals/s of the timed generations (sum_e len_e * N per generation
+/ wall time); `ms_per_step` = ms per generation.  The same generations are
+also run with the uncached batched evaluation replayed as one hipGraph per
+generation and with the host-planned cache (their fitness vectors must be
+identical), and the plain one-launch-per-expression interpreter is timed on
+the initial and the final population.  Beside them:
 the reference-policy subtree cache, and the constant-fitting workload (4096
 expressions of 50-200 tokens, 2-8 free constants, 5 variables, 256k
 datapoints): adjoint vs central-difference gradient evaluations/s.
@@ -260,7 +262,8 @@ def main():
 
     import torch
     from symbolic_regression_gpu_amd import EVAL_BATCHED, EVAL_CACHED, EVAL_PLAIN, Context
-    from symbolic_regression_gpu_amd.api import CACHE_GENERATIONAL_SHARED, CACHE_REFERENCE, SelectParams
+    from symbolic_regression_gpu_amd.api import (CACHE_DEVICE, CACHE_GENERATIONAL_SHARED, CACHE_REFERENCE,
+                                                 SelectParams)
     torch.cuda.init()
     ctx = Context(0)
     N = args.N
@@ -307,35 +310,60 @@ def main():
                    "device_ms_per_generation": ms_dev / K}
 
     # (B) the timed steps: the same generations (same seed, same initial
-    # population) with the generational subtree cache; its MSEs equal the
-    # batched ones bit for bit, so selection and offspring are identical.
-    ctx.cache_policy(CACHE_GENERATIONAL_SHARED, 0, 3)
+    # population) with the generational subtree cache planned on the device,
+    # one captured graph per generation; its MSEs equal the batched ones bit
+    # for bit, so selection and offspring are identical.
+    ctx.cache_policy(CACHE_DEVICE, 0, 3)
     ctx.ga_set_population(tok, val, ln, cap)
-    log(f"[bench] cached generations: {W} warm-up + {K} timed")
-    ctx.ga_run(sel, EVAL_CACHED, W, 0, args.seed, False)
+    log(f"[bench] cached generations (device-planned cache, hipGraph): {W} warm-up + {K} timed")
+    ctx.ga_run(sel, EVAL_CACHED, W, 0, args.seed, True)
     ctx.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    hist_b, _, st_b = ctx.ga_run(sel, EVAL_CACHED, K, W, args.seed, False)
+    hist_b, ms_dev_b, st_b = ctx.ga_run(sel, EVAL_CACHED, K, W, args.seed, True)
     ctx.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     value = st_b["logical_node_evals"] / wall
     identical = bool(np.array_equal(hist_a, hist_b, equal_nan=True))
     best = float(np.nanmin(np.where(np.isfinite(hist_b[-1]), hist_b[-1], np.nan)))
+    popK = to_population(*ctx.ga_population())
+    # the same generations again without graphs: HIP events around each of
+    # the two interpreter launches (the roofline's kernel time)
+    log("[bench] the same cached generations, launches timed with events")
+    ctx.cache_policy(CACHE_DEVICE, 0, 3)
+    ctx.ga_set_population(tok, val, ln, cap)
+    ctx.ga_run(sel, EVAL_CACHED, W, 0, args.seed, False)
+    hist_t, _, st_t = ctx.ga_run(sel, EVAL_CACHED, K, W, args.seed, False)
+    assert np.array_equal(hist_t, hist_b, equal_nan=True)
     cached_gen = {"node_evals_per_s": value, "ms_per_generation": 1e3 * wall / K,
+                  "device_ms_per_generation": ms_dev_b / K,
                   "executed_over_logical": st_b["executed_node_evals"] / max(st_b["logical_node_evals"], 1),
-                  "host_plan_ms_per_generation": st_b["detect_time_ms"] / K,
-                  "interp_kernel_ms_per_generation": st_b["interp_kernel_ms"] / K,
-                  "launches_per_generation": st_b["interp_launches"] / K,
+                  "interp_kernel_ms_per_generation": st_t["interp_kernel_ms"] / K,
+                  "launches_per_generation": st_t["interp_launches"] / K,
                   "columns_written_per_generation": st_b["column_writes"] / N / K,
                   "columns_read_per_generation": st_b["column_reads"] / N / K,
                   "columns_live": st_b["num_cached"],
+                  "host_work_per_generation": "none (planned on the device, one hipGraph per generation)",
                   "fitness_identical_to_batched": identical}
+
+    # (B2) for comparison: the host-planned generational cache (SHARED policy)
+    ctx.cache_policy(CACHE_GENERATIONAL_SHARED, 0, 3)
+    ctx.ga_set_population(tok, val, ln, cap)
+    log(f"[bench] cached generations (host-planned cache): {W} warm-up + {K} timed")
+    ctx.ga_run(sel, EVAL_CACHED, W, 0, args.seed, False)
+    ctx.sync()
+    t0 = time.perf_counter()
+    hist_h, _, st_h = ctx.ga_run(sel, EVAL_CACHED, K, W, args.seed, False)
+    ctx.sync()
+    wall_h = time.perf_counter() - t0
+    host_cached = {"node_evals_per_s": st_h["logical_node_evals"] / wall_h, "ms_per_generation": 1e3 * wall_h / K,
+                   "host_plan_ms_per_generation": st_h["detect_time_ms"] / K,
+                   "executed_over_logical": st_h["executed_node_evals"] / max(st_h["logical_node_evals"], 1),
+                   "fitness_identical_to_batched": bool(np.array_equal(hist_a, hist_h, equal_nan=True))}
     ctx.cache_policy(CACHE_REFERENCE, 0, 4)
 
-    # (C) plain and batched on the last population of the run
-    popK = to_population(*ctx.ga_population())
+    # (C) plain and batched on the last population of the timed run
     log("[bench] plain and batched evaluation of the final population")
     plainK = time_evals(ctx, popK, EVAL_PLAIN, 2)
     batchedK = time_evals(ctx, popK, EVAL_BATCHED, 3)
@@ -353,11 +381,11 @@ def main():
     # roofline of the dominant kernel of the timed steps: the skeleton
     # interpreter (k_batched, MODE_MSE_REF), over every launch of the timed
     # generations (HIP events around each launch on the context stream)
-    launches = max(st_b["interp_launches"], 1)
-    k_ms = st_b["interp_kernel_ms"] / launches
-    flops = st_b["executed_op_evals"] / launches
+    launches = max(st_t["interp_launches"], 1)
+    k_ms = st_t["interp_kernel_ms"] / launches
+    flops = st_t["executed_op_evals"] / launches
     achieved = flops / (k_ms * 1e-3) / 1e12
-    hbm_alg = (st_b["column_reads"] + st_b["column_writes"]) * 4 / launches + xy_bytes
+    hbm_alg = (st_t["column_reads"] + st_t["column_writes"]) * 4 / launches + xy_bytes
     traffic = None
     if os.path.exists(TRAFFIC_FILE_REF):
         with open(TRAFFIC_FILE_REF) as f:
@@ -404,8 +432,8 @@ def main():
             "workload": "evolution loop: population 1000, 1M datapoints, 3 vars, max depth 8, "
                         "<=255 tokens, tournament 7, crossover 0.9 / mutation 0.1, elites 5%, "
                         "target depth-5 tree; one step = one generation: batched persistent "
-                        "evaluation over the generational subtree cache (host plan, device "
-                        "launches), fused MSE, device select/vary",
+                        "evaluation over the generational subtree cache planned on the device, "
+                        "fused MSE, device select/vary, one hipGraph per generation",
             "N": N, "population": 1000, "tokens_initial": int(pop0.tokens.size),
             "tokens_final": int(popK.tokens.size),
             "node_evals_per_generation": st_b["logical_node_evals"] // max(K, 1),
@@ -441,6 +469,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "cached_evolution": cached_gen,
+        "host_planned_cached_evolution": host_cached,
         "batched_evolution": batched_gen,
         "plain_final_population": plainK,
         "batched_final_population": batchedK,

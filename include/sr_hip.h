@@ -174,7 +174,15 @@ enum {
     SR_CACHE_REFERENCE = 0,
     SR_CACHE_GENERATIONAL = 1,
     SR_CACHE_GENERATIONAL_REUSED = 2,
-    SR_CACHE_GENERATIONAL_SHARED = 3
+    SR_CACHE_GENERATIONAL_SHARED = 3,
+    /* The generational cache planned on the device, for the context's GA
+     * population (sr_ga_run, sr_ga_evaluate): every subtree of >= min_size
+     * tokens with a column is read, every subtree executed at least twice in
+     * the population is evaluated once and its column kept, columns of
+     * subtrees the population no longer holds are evicted; no host work, so
+     * sr_ga_run with use_graph replays one captured graph per generation.
+     * sr_eval_mse on a host population does not take this policy. */
+    SR_CACHE_DEVICE = 4
 };
 int sr_cache_policy(sr_ctx *ctx, int32_t policy, int32_t capacity, int32_t min_size);
 

@@ -18,6 +18,8 @@ import numpy as np
 
 EVAL_PLAIN, EVAL_BATCHED, EVAL_CACHED, EVAL_JIT = 0, 1, 2, 3
 CACHE_REFERENCE, CACHE_GENERATIONAL, CACHE_GENERATIONAL_REUSED, CACHE_GENERATIONAL_SHARED = 0, 1, 2, 3
+# generational cache planned on the device (GA population: ga_run / ga_evaluate)
+CACHE_DEVICE = 4
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libsrhip.so")
@@ -555,7 +557,9 @@ class Context:
         one, CACHE_GENERATIONAL_REUSED one met twice (in one population or in
         two consecutive ones), CACHE_GENERATIONAL_SHARED one met twice in the
         population plus one nested in those that the previous population
-        held.  capacity 0 = sized from free device memory."""
+        held; or CACHE_DEVICE, planned on the device for the GA population
+        (ga_run / ga_evaluate; a subtree executed twice in the population is
+        evaluated once and kept).  capacity 0 = sized from free device memory."""
         self._check(self._L.sr_cache_policy(self._h, int(policy), int(capacity), int(min_size)))
 
     def cache_clear(self):

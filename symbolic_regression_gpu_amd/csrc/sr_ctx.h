@@ -115,6 +115,7 @@ struct SubtreeCache {
 // value columns [capacity][N] its slots index.
 struct MemoState {
     bool enabled = false;
+    bool device = false;  // SR_CACHE_DEVICE: planned on the device (sr_dmemo.hip)
     int capacity = 0;  // requested columns; 0 = sized from free HBM
     int min_size = 4;
     int policy = SR_CACHE_GENERATIONAL;
@@ -125,6 +126,30 @@ struct MemoState {
     int64_t N = 0;
     uint64_t data_version = 0;
     DevBuf cols;
+};
+
+// Generational subtree cache planned on the device (sr_dmemo.hip): slot
+// table, per-position planning arrays of the GA population, skeleton
+// programs of both launches and running statistics, all fixed-size so that
+// a generation is one captured graph.
+struct DMemoState {
+    bool ready = false;
+    int C = 0;        // slots (value columns)
+    int H = 0;        // lookup table entries (power of two >= 2C)
+    int H2 = 0;       // per-generation count table entries (power of two >= 2 * E * cap)
+    int E = 0, cap = 0, PC = 0, L1 = 0;  // population shape, program stride, launch-1 capacity
+    int64_t N = 0;
+    uint64_t data_version = 0;
+    int min_size = 3;
+    DevBuf cols;                                                      // [C][N] fp32
+    DevBuf s_key, s_len, s_tok, s_val, s_seen, s_live, s_mse, s_cnt, s_has, s_free;  // per slot
+    DevBuf ht_key, ht_slot;                                           // key -> slot
+    DevBuf ct_key, ct_cnt, ct_rep, ct_root, ct_slot, ct_wr, ct_off, ct_prog;  // per generation
+    DevBuf sz, hh, hit, cov, sh, anc, pslot, ctx_idx, flag, rank;     // per position [E*cap]
+    DevBuf p1_tok, p1_val, p1_len, p1_slot, p1_pos, p1_ins, p1_plen, p1_order, p1_pf;
+    DevBuf p2_tok, p2_val, p2_len, p2_ins, p2_plen, p2_order, p2_pf, e_src;
+    DevBuf cs_size, cs_need, cs_frames;                               // compile scratch
+    DevBuf mse1, cnt1, mse2, cnt2, ctr, stats, blk;
 };
 
 // Device-resident GA population (double-buffered, fixed stride cap) and the
@@ -212,6 +237,7 @@ struct Ctx {
     SubtreeCache cache;
     MemoState memo;
     GaState ga;
+    DMemoState dm;
     ParamState par;
     JitState jit;
     // ev[0..3]: transfer / kernel / transfer brackets of one call;
@@ -267,6 +293,7 @@ struct EvalBufs {
     uint32_t *counters;
     float *mse;
     long long *cnt;
+    const int *fin_count = nullptr;  // finalize only programs below this device count
 };
 // Enqueue counter reset + batched kernel (+ finalize for MODE_MSE).  a must
 // carry the program / order / output pointers; the rest is filled in here.
@@ -289,6 +316,13 @@ void add_batched_timing(Ctx &c, int set, sr_stats *st);
 // Same checks as build_program without compiling (lengths, stream shape,
 // variable indices; OP_REF / OP_STORE rejected).
 int validate_population(Ctx &c, const sr_population &pop);
+
+// Device-planned generational cache (sr_dmemo.hip): size it for the GA
+// population and data, enqueue the evaluation of GA buffer b (fitness into
+// c.d_mse), and read its statistics (see dmemo_stats).
+int dmemo_setup(Ctx &c, int capacity, int min_size);
+int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *ev = nullptr);
+int dmemo_stats(Ctx &c, long long out[9]);
 
 }  // namespace sr
 

@@ -1,0 +1,977 @@
+// sr_dmemo.hip -- the generational subtree cache planned on the device.
+//
+// No counterpart in the reference: its cache (cl_gpu_eval.cu:294-329,
+// 384-397) is planned on the host, keeps the first subtrees it detects and
+// never evicts.  Here a generation of the device-resident GA population is
+// planned by kernels, so evaluate -> fitness -> select -> vary runs with no
+// host work and is captured as one hipGraph per generation.
+//
+// Per generation (E expressions of stride cap, positions p = e * cap + q):
+//   1. sizes and 64-bit structural hashes of every subtree;
+//   2. every subtree of >= min_size tokens is looked up among the slots; a
+//      hit (key and token body equal) will be read from its column;
+//   3. positions outside hits are "executed"; executed subtrees without a
+//      column are counted by key;
+//   4. slots whose subtree the population no longer holds are evicted;
+//   5. a subtree executed at least twice ("shared") gets a slot: a shared
+//      subtree with an occurrence inside no other shared occurrence is a
+//      root: launch 1 evaluates it once (from its first such occurrence) and
+//      writes its column; a shared subtree only ever met inside others is
+//      written by an OP_STORE inside the launch-1 program of the root that
+//      holds its first occurrence;
+//   6. launch 2 runs the expressions with every hit and every shared subtree
+//      read as an OP_REF; an expression that is itself a root takes its MSE
+//      from launch 1, one whose whole tree has a column and a remembered MSE
+//      runs nothing.
+// Columns hold exactly what the interpreter computes for a subtree (the same
+// bytecode, the same operands), so every MSE equals the uncached batched one
+// bit for bit.  Slot numbers follow the first positions of the new subtrees,
+// so plans are deterministic.
+#include <algorithm>
+#include <climits>
+
+#include "sr_ctx.h"
+#include "sr_ga_launch.h"
+#include "sr_kernels.h"
+#include "sr_launch.h"
+
+namespace sr {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int32_t kNone = INT_MAX;
+constexpr int32_t kFromLaunch1 = 0x40000000;  // e_src: launch-1 program r is kFromLaunch1 | r
+
+// counters (int32): generation, free slots, launch-1 programs, nested slots
+// given, launch-2 programs, roots found, nested found, status
+enum { C_GEN = 0, C_FREE, C_L1, C_NEST, C_L2, C_ROOTS, C_NESTED, C_STATUS, C_COUNT };
+// statistics (int64), accumulated over generations
+enum { S_LOGICAL = 0, S_EXEC, S_REFS, S_STORES, S_OPS, S_L1, S_L2, S_LIVE, S_GENS, S_COUNT };
+
+SR_DEV uint64_t mix64(uint64_t x)
+{
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    return x;
+}
+
+SR_DEV int tok_arity(int t) { return t <= 0 ? 0 : (t < 10 ? 2 : 1); }
+
+struct DmArgs {
+    const int32_t *tok;
+    const float *val;
+    const int32_t *len;
+    int E, cap, PC, L1, C, H, H2, ms;
+    int32_t *sz;
+    uint64_t *hh;
+    int32_t *hit, *cov, *sh, *anc, *pslot, *cidx, *flag, *rank;
+    uint64_t *s_key;
+    int32_t *s_len, *s_tok;
+    float *s_val;
+    uint32_t *s_seen;
+    int32_t *s_live;
+    float *s_mse;
+    long long *s_cnt;
+    int32_t *s_has, *s_free;
+    uint64_t *ht_key;
+    int32_t *ht_slot;
+    uint64_t *ct_key;
+    int32_t *ct_cnt, *ct_rep, *ct_root, *ct_slot, *ct_wr, *ct_off, *ct_prog;
+    int32_t *p1_tok;
+    float *p1_val;
+    int32_t *p1_len, *p1_slot, *p1_pos;
+    int32_t *p2_tok;
+    float *p2_val;
+    int32_t *p2_len, *e_src;
+    Ins *p1_ins, *p2_ins;
+    int32_t *p1_plen, *p2_plen, *p1_pf, *p2_pf;
+    int32_t *cs_size, *cs_need;
+    CompileFrame *cs_frames;
+    float *mse1, *mse2, *fit;
+    long long *cnt1, *cnt2;
+    int32_t *ctr;
+    long long *stats;
+};
+
+SR_DEV bool same_body(const DmArgs &a, int64_t p, int64_t r, int n)
+{
+    for (int k = 0; k < n; ++k)
+        if (a.tok[p + k] != a.tok[r + k] || __float_as_uint(a.val[p + k]) != __float_as_uint(a.val[r + k]))
+            return false;
+    return true;
+}
+
+SR_DEV bool executed(const DmArgs &a, int64_t p)
+{
+    const int e = (int)(p / a.cap), q = (int)(p - (int64_t)e * a.cap);
+    if (q >= a.len[e] || a.cov[p]) return false;
+    const int r = a.hit[(int64_t)e * a.cap];
+    return !(r >= 0 && a.s_has[r]);  // a settled expression runs nothing
+}
+
+// ---- per generation: reset -------------------------------------------------
+__global__ void k_dm_begin(DmArgs a)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        a.ctr[C_GEN] += 1;
+        for (int k = 1; k < C_COUNT; ++k) a.ctr[k] = 0;
+        a.stats[S_GENS] += 1;
+    }
+}
+
+__global__ void k_dm_clear_pos(DmArgs a)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < (int64_t)a.E * a.cap) {
+        a.hit[p] = -3;
+        a.cov[p] = 0;
+        a.sh[p] = 0;
+        a.anc[p] = kNone;
+        a.pslot[p] = -1;
+        a.cidx[p] = -1;
+        a.flag[p] = 0;
+    }
+    if (p < a.L1) {
+        a.p1_len[p] = 0;
+        a.p1_plen[p] = 0;
+    }
+    if (p < a.E) {
+        a.p2_len[p] = 0;
+        a.p2_plen[p] = 0;
+    }
+}
+
+__global__ void k_dm_clear_ct(DmArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.H2) return;
+    a.ct_key[i] = 0;
+    a.ct_cnt[i] = 0;
+    a.ct_rep[i] = kNone;
+    a.ct_root[i] = kNone;
+    a.ct_slot[i] = -1;
+    a.ct_wr[i] = -1;
+    a.ct_off[i] = 0;
+    a.ct_prog[i] = -1;
+}
+
+// ---- 1. sizes and structural hashes (one wave per expression) ---------------
+// A non-commutative chain over (token, value bits, child hashes) through a
+// full 64-bit finaliser; 0 is reserved for empty table entries.  The stream
+// is staged in LDS and hashed right to left by lane 0 (the recursion is
+// serial), then written back by the whole wave.
+__global__ __launch_bounds__(64) void k_dm_hash(DmArgs a)
+{
+    extern __shared__ unsigned char lds[];
+    uint64_t *lh = (uint64_t *)lds;
+    int32_t *lt = (int32_t *)(lh + a.cap);
+    uint32_t *lv = (uint32_t *)(lt + a.cap);
+    int32_t *ls = (int32_t *)(lv + a.cap);
+    const int e = blockIdx.x, lane = threadIdx.x;
+    const int64_t b = (int64_t)e * a.cap;
+    const int L = min(a.len[e], a.cap);
+    for (int k = lane; k < L; k += 64) {
+        lt[k] = a.tok[b + k];
+        lv[k] = __float_as_uint(a.val[b + k]);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        for (int i = L - 1; i >= 0; --i) {
+            const int t = lt[i];
+            uint64_t x = mix64((uint64_t)(uint32_t)t * 0x9e3779b97f4a7c15ULL ^ mix64((uint64_t)lv[i] + 0x632be59bd9b4e019ULL));
+            int sz = 1, pos = i + 1;
+            for (int k = 0; k < tok_arity(t); ++k) {
+                if (pos >= L) break;
+                x = mix64(x * 0xbf58476d1ce4e5b9ULL + lh[pos] + (uint64_t)(k + 1));
+                sz += ls[pos];
+                pos += ls[pos];
+            }
+            lh[i] = x ? x : 1;
+            ls[i] = sz;
+        }
+    }
+    __syncthreads();
+    for (int k = lane; k < L; k += 64) {
+        a.hh[b + k] = lh[k];
+        a.sz[b + k] = ls[k];
+    }
+}
+
+// ---- 2. lookup of every subtree of >= min_size tokens ------------------------
+__global__ void k_dm_lookup(DmArgs a)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)a.E * a.cap) return;
+    const int e = (int)(p / a.cap), q = (int)(p - (int64_t)e * a.cap);
+    if (q >= a.len[e] || a.sz[p] < a.ms) return;
+    const uint64_t h = a.hh[p];
+    const int n = a.sz[p];
+    for (uint32_t i = (uint32_t)mix64(h) & (a.H - 1);; i = (i + 1) & (a.H - 1)) {
+        const uint64_t k = a.ht_key[i];
+        if (k == 0) {
+            a.hit[p] = -1;
+            return;
+        }
+        if (k != h) continue;
+        const int s = a.ht_slot[i];
+        bool same = a.s_len[s] == n;
+        const int64_t sb = (int64_t)s * a.cap;
+        for (int j = 0; same && j < n; ++j)
+            same = a.s_tok[sb + j] == a.tok[p + j] && __float_as_uint(a.s_val[sb + j]) == __float_as_uint(a.val[p + j]);
+        if (same) {
+            a.hit[p] = s;
+            a.s_seen[s] = (uint32_t)a.ctr[C_GEN];
+        } else {
+            a.hit[p] = -2;  // the key belongs to another body: never cached
+        }
+        return;
+    }
+}
+
+// Interior of every hit is read from the column, not executed.
+__global__ void k_dm_cover(DmArgs a)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)a.E * a.cap || a.hit[p] < 0) return;
+    for (int k = 1; k < a.sz[p]; ++k) a.cov[p + k] = 1;
+}
+
+// ---- 3. executed subtrees without a column, counted by key -------------------
+__global__ void k_dm_count(DmArgs a)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)a.E * a.cap || a.hit[p] != -1 || !executed(a, p)) return;
+    const uint64_t h = a.hh[p];
+    for (uint32_t i = (uint32_t)mix64(h) & (a.H2 - 1);; i = (i + 1) & (a.H2 - 1)) {
+        uint64_t k = a.ct_key[i];
+        if (k == 0) {
+            const uint64_t prev = atomicCAS((unsigned long long *)&a.ct_key[i], 0ull, (unsigned long long)h);
+            k = prev == 0 ? h : prev;
+        }
+        if (k != h) continue;
+        atomicAdd(&a.ct_cnt[i], 1);
+        atomicMin(&a.ct_rep[i], (int32_t)p);
+        a.cidx[p] = (int32_t)i;
+        return;
+    }
+}
+
+// An occurrence is "shared" when its key was executed at least twice and its
+// body equals the key's first occurrence (a hash collision stays inline).
+__global__ void k_dm_verify(DmArgs a)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)a.E * a.cap) return;
+    const int i = a.cidx[p];
+    if (i < 0 || a.ct_cnt[i] < 2) return;
+    const int64_t r = a.ct_rep[i];
+    a.sh[p] = (p == r || (a.sz[p] == a.sz[r] && same_body(a, p, r, a.sz[p]))) ? 1 : 0;
+}
+
+// Outermost shared occurrence holding each shared position: the smallest
+// position among its shared ancestors (ancestors precede descendants in
+// prefix order), or itself; an outermost occurrence makes its key a root.
+__global__ void k_dm_outer(DmArgs a)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)a.E * a.cap || !a.sh[p]) return;
+    for (int k = 1; k < a.sz[p]; ++k)
+        if (a.sh[p + k]) atomicMin(&a.anc[p + k], (int32_t)p);
+}
+
+__global__ void k_dm_anc(DmArgs a)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)a.E * a.cap || !a.sh[p]) return;
+    if (a.anc[p] == kNone) {
+        a.anc[p] = (int32_t)p;
+        atomicMin(&a.ct_root[a.cidx[p]], (int32_t)p);
+    }
+}
+
+// A shared key with no outermost occurrence is written inside the program of
+// the root holding its first occurrence, at that occurrence's offset.
+__global__ void k_dm_writer(DmArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.H2 || a.ct_cnt[i] < 2 || a.ct_root[i] != kNone) return;
+    const int32_t r = a.ct_rep[i];
+    const int32_t A = a.anc[r];
+    if (A == kNone) return;
+    a.ct_wr[i] = a.cidx[A];
+    a.ct_off[i] = r - A;
+}
+
+// ---- 4. eviction and the free list ------------------------------------------------
+__global__ void k_dm_evict(DmArgs a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.C) return;
+    if (a.s_live[s] && a.s_seen[s] != (uint32_t)a.ctr[C_GEN]) {
+        a.s_live[s] = 0;
+        a.s_has[s] = 0;
+    }
+}
+
+// Ordered compaction in three passes (256 items per block): count the
+// selected items of each block, scan the block counts, then give each
+// selected item its rank.  Selection: kind 0 = flag[i] == v over positions
+// (rank[i] written), kind 1 = a free slot (s_free[rank] = i).
+SR_DEV bool scan_pick(const DmArgs &a, int kind, int v, int64_t i)
+{
+    return kind == 0 ? a.flag[i] == v : a.s_live[i] == 0;
+}
+
+SR_DEV int64_t scan_n(const DmArgs &a, int kind) { return kind == 0 ? (int64_t)a.E * a.cap : a.C; }
+
+__global__ __launch_bounds__(256) void k_dm_scan_count(DmArgs a, int kind, int v, int32_t *blk)
+{
+    __shared__ int wc[4];
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool pick = i < scan_n(a, kind) && scan_pick(a, kind, v, i);
+    const uint64_t m = __ballot(pick);
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) blk[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+// Exclusive scan of nb block counts in place (one workgroup); the total goes
+// to ctr[ci].
+__global__ __launch_bounds__(1024) void k_dm_scan_blocks(DmArgs a, int32_t *blk, int nb, int ci)
+{
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (nb + 1023) / 1024;
+    const int lo = min(nb, t * per), hi = min(nb, lo + per);
+    int n = 0;
+    for (int k = lo; k < hi; ++k) n += blk[k];
+    part[t] = n;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int w = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += w;
+        __syncthreads();
+    }
+    int at = part[t] - n;
+    for (int k = lo; k < hi; ++k) {
+        const int c = blk[k];
+        blk[k] = at;
+        at += c;
+    }
+    if (t == 1023) a.ctr[ci] = part[1023];
+}
+
+__global__ __launch_bounds__(256) void k_dm_scan_write(DmArgs a, int kind, int v, const int32_t *blk)
+{
+    __shared__ int wc[4];
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool pick = i < scan_n(a, kind) && scan_pick(a, kind, v, i);
+    const uint64_t m = __ballot(pick);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) wc[w] = __popcll(m);
+    __syncthreads();
+    if (!pick) return;
+    int r = blk[blockIdx.x];
+    for (int k = 0; k < w; ++k) r += wc[k];
+    r += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    if (kind == 0) a.rank[i] = r;
+    else a.s_free[r] = (int32_t)i;
+}
+
+// ---- 5. slots for roots, then for nested shared keys ------------------------------
+__global__ void k_dm_flag(DmArgs a, int roots)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.H2 || a.ct_cnt[i] < 2) return;
+    if (roots) {
+        if (a.ct_root[i] != kNone) a.flag[a.ct_root[i]] = 1;
+    } else if (a.ct_root[i] == kNone && a.ct_wr[i] >= 0 && a.ct_slot[a.ct_wr[i]] >= 0) {
+        a.flag[a.ct_rep[i]] = 2;
+    }
+}
+
+SR_DEV void claim_slot(const DmArgs &a, int i, int slot, int64_t body)
+{
+    const int n = a.sz[body];
+    a.ct_slot[i] = slot;
+    a.s_key[slot] = a.ct_key[i];
+    a.s_len[slot] = n;
+    for (int k = 0; k < n; ++k) {
+        a.s_tok[(int64_t)slot * a.cap + k] = a.tok[body + k];
+        a.s_val[(int64_t)slot * a.cap + k] = a.val[body + k];
+    }
+    a.s_live[slot] = 1;
+    a.s_has[slot] = 0;
+    a.s_seen[slot] = (uint32_t)a.ctr[C_GEN];
+}
+
+__global__ void k_dm_alloc(DmArgs a, int roots)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nfree = a.ctr[C_FREE];
+    const int nroot = min(min(a.ctr[C_ROOTS], nfree), a.L1);
+    if (i == 0) {
+        if (roots) a.ctr[C_L1] = nroot;
+        else a.ctr[C_NEST] = min(a.ctr[C_NESTED], nfree - nroot);
+    }
+    if (i >= a.H2 || a.ct_cnt[i] < 2) return;
+    if (roots) {
+        if (a.ct_root[i] == kNone) return;
+        const int r = a.rank[a.ct_root[i]];
+        if (r >= nroot) return;
+        const int slot = a.s_free[r];
+        claim_slot(a, i, slot, a.ct_root[i]);
+        a.ct_prog[i] = r;
+        a.p1_slot[r] = slot;
+        a.p1_pos[r] = a.ct_root[i];
+    } else {
+        if (a.flag[a.ct_rep[i]] != 2) return;
+        const int r = nroot + a.rank[a.ct_rep[i]];
+        if (r >= nfree) return;
+        claim_slot(a, i, a.s_free[r], a.ct_rep[i]);
+    }
+}
+
+__global__ void k_dm_ht_clear(DmArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.H) a.ht_key[i] = 0;
+}
+
+__global__ void k_dm_ht_insert(DmArgs a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.C || !a.s_live[s]) return;
+    const uint64_t h = a.s_key[s];
+    for (uint32_t i = (uint32_t)mix64(h) & (a.H - 1);; i = (i + 1) & (a.H - 1)) {
+        if (atomicCAS((unsigned long long *)&a.ht_key[i], 0ull, (unsigned long long)h) == 0ull) {
+            a.ht_slot[i] = s;
+            return;
+        }
+    }
+}
+
+__global__ void k_dm_pslot(DmArgs a)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)a.E * a.cap || !a.sh[p]) return;
+    a.pslot[p] = a.ct_slot[a.cidx[p]];
+}
+
+// ---- 6. skeleton programs -------------------------------------------------------
+SR_DEV void put(int32_t *t, float *v, int &k, int tok, float val)
+{
+    t[k] = tok;
+    v[k] = val;
+    ++k;
+}
+
+// Launch 1: one program per root with a slot, from its first outermost
+// occurrence: the root's column stored, nested keys stored where this root
+// is their writer, old columns read.  One wave per program: the occurrence
+// is staged in LDS, lane 0 emits, the wave writes the program out.
+__global__ __launch_bounds__(64) void k_dm_emit1(DmArgs a)
+{
+    extern __shared__ unsigned char lds[];
+    int32_t *lhit = (int32_t *)lds, *lst = lhit + a.cap, *lsz = lst + a.cap, *ltk = lsz + a.cap;
+    float *lvl = (float *)(ltk + a.cap);
+    int32_t *otk = (int32_t *)(lvl + a.cap);
+    float *ovl = (float *)(otk + a.PC);
+    __shared__ int on;
+    const int r = blockIdx.x, lane = threadIdx.x;
+    if (r >= a.ctr[C_L1]) return;
+    const int64_t P = a.p1_pos[r];
+    const int n = a.sz[P];
+    const int ri = a.cidx[P];
+    for (int k = lane; k < n; k += 64) {
+        const int64_t x = P + k;
+        lhit[k] = a.hit[x];
+        int st = -1;
+        if (k > 0 && a.sh[x] && a.pslot[x] >= 0) {
+            const int i = a.cidx[x];
+            if (a.ct_root[i] == kNone && a.ct_wr[i] == ri && a.ct_off[i] == k) st = a.pslot[x];
+        }
+        lst[k] = st;
+        lsz[k] = a.sz[x];
+        ltk[k] = a.tok[x];
+        lvl[k] = a.val[x];
+    }
+    __syncthreads();
+    if (lane == 0) {
+        int k = 0;
+        put(otk, ovl, k, OP_STORE, (float)a.p1_slot[r]);
+        for (int x = 0; x < n; ++x) {
+            if (x > 0) {
+                if (lhit[x] >= 0) {
+                    put(otk, ovl, k, OP_REF, (float)lhit[x]);
+                    x += lsz[x] - 1;
+                    continue;
+                }
+                if (lst[x] >= 0) put(otk, ovl, k, OP_STORE, (float)lst[x]);
+            }
+            put(otk, ovl, k, ltk[x], lvl[x]);
+        }
+        on = k;
+        a.p1_len[r] = k;
+    }
+    __syncthreads();
+    for (int k = lane; k < on; k += 64) {
+        a.p1_tok[(int64_t)r * a.PC + k] = otk[k];
+        a.p1_val[(int64_t)r * a.PC + k] = ovl[k];
+    }
+}
+
+// Launch 2: one program per expression that needs one (one wave each).
+__global__ __launch_bounds__(64) void k_dm_emit2(DmArgs a)
+{
+    extern __shared__ unsigned char lds[];
+    int32_t *lref = (int32_t *)lds, *lsz = lref + a.cap, *ltk = lsz + a.cap;
+    float *lvl = (float *)(ltk + a.cap);
+    int32_t *otk = (int32_t *)(lvl + a.cap);
+    float *ovl = (float *)(otk + a.cap);
+    __shared__ int on;
+    const int e = blockIdx.x, lane = threadIdx.x;
+    const int64_t b = (int64_t)e * a.cap;
+    const int L = min(a.len[e], a.cap);
+    const int root = a.hit[b];
+    if (root >= 0 && a.s_has[root]) {
+        if (lane == 0) a.e_src[e] = -2 - root;
+        return;
+    }
+    if (a.sh[b] && a.pslot[b] >= 0 && a.ct_prog[a.cidx[b]] >= 0) {
+        if (lane == 0) a.e_src[e] = kFromLaunch1 | a.ct_prog[a.cidx[b]];
+        return;
+    }
+    for (int k = lane; k < L; k += 64) {
+        const int h = a.hit[b + k], ps = a.pslot[b + k];
+        lref[k] = h >= 0 ? h : ps;
+        lsz[k] = a.sz[b + k];
+        ltk[k] = a.tok[b + k];
+        lvl[k] = a.val[b + k];
+    }
+    __syncthreads();
+    if (lane == 0) {
+        int k = 0;
+        for (int x = 0; x < L; ++x) {
+            if (lref[x] >= 0) {
+                put(otk, ovl, k, OP_REF, (float)lref[x]);
+                x += lsz[x] - 1;
+                continue;
+            }
+            put(otk, ovl, k, ltk[x], lvl[x]);
+        }
+        on = k;
+        a.p2_len[e] = k;
+        a.e_src[e] = e;
+    }
+    __syncthreads();
+    for (int k = lane; k < on; k += 64) {
+        a.p2_tok[(int64_t)e * a.PC + k] = otk[k];
+        a.p2_val[(int64_t)e * a.PC + k] = ovl[k];
+    }
+}
+
+// Bytecode of both launches' programs (one wave per program, compile scratch
+// in LDS), the first OP_REF operands marked for loading a tile ahead, and
+// token statistics.
+__global__ __launch_bounds__(64) void k_dm_compile(DmArgs a, int nv)
+{
+    extern __shared__ unsigned char lds[];
+    CompileFrame *lfr = (CompileFrame *)lds;
+    Ins *lout = (Ins *)(lfr + a.PC + 1);
+    int32_t *lt = (int32_t *)(lout + a.PC);
+    float *lv = (float *)(lt + a.PC);
+    int32_t *lsize = (int32_t *)(lv + a.PC);
+    int32_t *lneed = lsize + a.PC;
+    __shared__ int on;
+    const int g = blockIdx.x, lane = threadIdx.x;
+    const bool first = g < a.L1;
+    const int idx = first ? g : g - a.L1;
+    const int L = first ? a.p1_len[idx] : a.p2_len[idx];
+    if (L <= 0) return;
+    const int64_t o = (int64_t)idx * a.PC;
+    const int32_t *t = (first ? a.p1_tok : a.p2_tok) + o;
+    const float *v = (first ? a.p1_val : a.p2_val) + o;
+    for (int k = lane; k < L; k += 64) {
+        lt[k] = t[k];
+        lv[k] = v[k];
+    }
+    __syncthreads();
+    if (lane == 0) {
+        int32_t *pf = (first ? a.p1_pf : a.p2_pf) + (int64_t)idx * kRefPrefetch;
+        int sn = 0;
+        int n = raw_stack_depth(lt, L, true) >= 0 ? compile_expr(lt, lv, L, nv, lsize, lneed, lfr, lout, &sn) : -1;
+        if (n < 0 || sn > kStackSlots) {
+            atomicOr(&a.ctr[C_STATUS], 1);
+            lout[0].code = 0 | (SRC_CONST << 8);
+            lout[0].arg = f32_bits(__builtin_nanf(""));
+            n = 1;
+        }
+        int j = 0;
+        for (int i = 0; i < n && j < kRefPrefetch; ++i) {
+            if (bc_src(lout[i].code) != SRC_REF) continue;
+            lout[i].code |= (j + 1) << BC_PF_SHIFT;
+            pf[j++] = lout[i].arg;
+        }
+        for (; j < kRefPrefetch; ++j) pf[j] = -1;
+        (first ? a.p1_plen : a.p2_plen)[idx] = n;
+        on = n;
+        long long refs = 0, stores = 0, ops = 0;
+        for (int i = 0; i < L; ++i) {
+            refs += lt[i] == OP_REF;
+            stores += lt[i] == OP_STORE;
+            ops += lt[i] > 0;
+        }
+        atomicAdd((unsigned long long *)&a.stats[S_EXEC], (unsigned long long)L);
+        atomicAdd((unsigned long long *)&a.stats[S_REFS], (unsigned long long)refs);
+        atomicAdd((unsigned long long *)&a.stats[S_STORES], (unsigned long long)stores);
+        atomicAdd((unsigned long long *)&a.stats[S_OPS], (unsigned long long)ops);
+        if (!first) atomicAdd(&a.ctr[C_L2], 1);
+    }
+    __syncthreads();
+    Ins *out = (first ? a.p1_ins : a.p2_ins) + o;
+    for (int k = lane; k < on; k += 64) out[k] = lout[k];
+}
+
+// ---- fitness of every expression; MSEs of launch-1 roots remembered ---------------
+__global__ void k_dm_gather(DmArgs a, long long *cnt_out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < a.L1 && e < a.ctr[C_L1]) {
+        const int s = a.p1_slot[e];
+        a.s_mse[s] = a.mse1[e];
+        a.s_cnt[s] = a.cnt1[e];
+        a.s_has[s] = 1;
+    }
+    if (e >= a.E) return;
+    const int src = a.e_src[e];
+    float m;
+    long long c;
+    if (src <= -2) {
+        m = a.s_mse[-2 - src];
+        c = a.s_cnt[-2 - src];
+    } else if (src & kFromLaunch1) {
+        m = a.mse1[src & ~kFromLaunch1];
+        c = a.cnt1[src & ~kFromLaunch1];
+    } else {
+        m = a.mse2[e];
+        c = a.cnt2[e];
+    }
+    a.fit[e] = m;
+    if (cnt_out) cnt_out[e] = c;
+    atomicAdd((unsigned long long *)&a.stats[S_LOGICAL], (unsigned long long)a.len[e]);
+    if (e == 0) {
+        a.stats[S_L1] += a.ctr[C_L1];
+        a.stats[S_L2] += a.ctr[C_L2];
+        a.stats[S_LIVE] = (long long)a.C - a.ctr[C_FREE] + a.ctr[C_L1] + a.ctr[C_NEST];
+    }
+}
+
+__global__ void k_dm_init_slots(DmArgs a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) {
+        for (int k = 0; k < C_COUNT; ++k) a.ctr[k] = 0;
+        for (int k = 0; k < S_COUNT; ++k) a.stats[k] = 0;
+    }
+    if (s >= a.C) return;
+    a.s_live[s] = 0;
+    a.s_has[s] = 0;
+    a.s_seen[s] = 0;
+    a.s_len[s] = 0;
+}
+
+unsigned grid_of(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+int pow2_at_least(int64_t v)
+{
+    int64_t p = 64;
+    while (p < v) p <<= 1;
+    return (int)p;
+}
+
+DmArgs dm_args(Ctx &c, int b)
+{
+    DMemoState &d = c.dm;
+    GaState &g = c.ga;
+    DmArgs a{};
+    a.tok = g.tok[b].as<int32_t>();
+    a.val = g.val[b].as<float>();
+    a.len = g.len[b].as<int32_t>();
+    a.E = d.E;
+    a.cap = d.cap;
+    a.PC = d.PC;
+    a.L1 = d.L1;
+    a.C = d.C;
+    a.H = d.H;
+    a.H2 = d.H2;
+    a.ms = d.min_size;
+    a.sz = d.sz.as<int32_t>();
+    a.hh = d.hh.as<uint64_t>();
+    a.hit = d.hit.as<int32_t>();
+    a.cov = d.cov.as<int32_t>();
+    a.sh = d.sh.as<int32_t>();
+    a.anc = d.anc.as<int32_t>();
+    a.pslot = d.pslot.as<int32_t>();
+    a.cidx = d.ctx_idx.as<int32_t>();
+    a.flag = d.flag.as<int32_t>();
+    a.rank = d.rank.as<int32_t>();
+    a.s_key = d.s_key.as<uint64_t>();
+    a.s_len = d.s_len.as<int32_t>();
+    a.s_tok = d.s_tok.as<int32_t>();
+    a.s_val = d.s_val.as<float>();
+    a.s_seen = d.s_seen.as<uint32_t>();
+    a.s_live = d.s_live.as<int32_t>();
+    a.s_mse = d.s_mse.as<float>();
+    a.s_cnt = d.s_cnt.as<long long>();
+    a.s_has = d.s_has.as<int32_t>();
+    a.s_free = d.s_free.as<int32_t>();
+    a.ht_key = d.ht_key.as<uint64_t>();
+    a.ht_slot = d.ht_slot.as<int32_t>();
+    a.ct_key = d.ct_key.as<uint64_t>();
+    a.ct_cnt = d.ct_cnt.as<int32_t>();
+    a.ct_rep = d.ct_rep.as<int32_t>();
+    a.ct_root = d.ct_root.as<int32_t>();
+    a.ct_slot = d.ct_slot.as<int32_t>();
+    a.ct_wr = d.ct_wr.as<int32_t>();
+    a.ct_off = d.ct_off.as<int32_t>();
+    a.ct_prog = d.ct_prog.as<int32_t>();
+    a.p1_tok = d.p1_tok.as<int32_t>();
+    a.p1_val = d.p1_val.as<float>();
+    a.p1_len = d.p1_len.as<int32_t>();
+    a.p1_slot = d.p1_slot.as<int32_t>();
+    a.p1_pos = d.p1_pos.as<int32_t>();
+    a.p2_tok = d.p2_tok.as<int32_t>();
+    a.p2_val = d.p2_val.as<float>();
+    a.p2_len = d.p2_len.as<int32_t>();
+    a.e_src = d.e_src.as<int32_t>();
+    a.p1_ins = d.p1_ins.as<Ins>();
+    a.p2_ins = d.p2_ins.as<Ins>();
+    a.p1_plen = d.p1_plen.as<int32_t>();
+    a.p2_plen = d.p2_plen.as<int32_t>();
+    a.p1_pf = d.p1_pf.as<int32_t>();
+    a.p2_pf = d.p2_pf.as<int32_t>();
+    a.cs_size = d.cs_size.as<int32_t>();
+    a.cs_need = d.cs_need.as<int32_t>();
+    a.cs_frames = d.cs_frames.as<CompileFrame>();
+    a.mse1 = d.mse1.as<float>();
+    a.mse2 = d.mse2.as<float>();
+    a.cnt1 = d.cnt1.as<long long>();
+    a.cnt2 = d.cnt2.as<long long>();
+    a.fit = c.d_mse.as<float>();
+    a.ctr = d.ctr.as<int32_t>();
+    a.stats = d.stats.as<long long>();
+    return a;
+}
+
+}  // namespace
+
+// Size (or re-size) the device cache for the context's GA population and
+// data; columns from free HBM unless `capacity` > 0.  Not captured.
+int dmemo_setup(Ctx &c, int capacity, int min_size)
+{
+    DMemoState &d = c.dm;
+    GaState &g = c.ga;
+    const int E = g.pop, cap = g.cap;
+    const int PC = 2 * cap + 2;
+    if (PC > MAX_EXPR_LEN)
+        return c.fail(SR_ERR_UNSUPPORTED, "device-planned cache: GA stride too large for skeleton programs");
+    if (d.ready && d.E == E && d.cap == cap && d.N == c.N && d.data_version == c.data_version &&
+        d.min_size == min_size)
+        return SR_OK;
+    c.ga.release_graphs();
+    int C = capacity;
+    if (C <= 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)1 << 34;
+        free_b += d.cols.bytes;
+        const double col = (double)c.N * sizeof(float);
+        C = (int)std::max<int64_t>(64, std::min<int64_t>((int64_t)(0.5 * (double)free_b / col), 1 << 17));
+    }
+    const int64_t M = (int64_t)E * cap;
+    d.E = E;
+    d.cap = cap;
+    d.PC = PC;
+    d.L1 = 2 * E;
+    d.C = C;
+    d.H = pow2_at_least(2 * (int64_t)C);
+    d.H2 = pow2_at_least(2 * M);
+    d.N = c.N;
+    d.data_version = c.data_version;
+    d.min_size = min_size;
+    d.ready = false;
+    hipError_t he = hipSuccess;
+    auto need = [&](DevBuf &buf, size_t bytes) {
+        if (he == hipSuccess) he = buf.ensure(bytes);
+    };
+    d.cols.release();
+    need(d.cols, (size_t)C * (size_t)c.N * 4);
+    need(d.s_key, (size_t)C * 8);
+    need(d.s_len, (size_t)C * 4);
+    need(d.s_tok, (size_t)C * cap * 4);
+    need(d.s_val, (size_t)C * cap * 4);
+    need(d.s_seen, (size_t)C * 4);
+    need(d.s_live, (size_t)C * 4);
+    need(d.s_mse, (size_t)C * 4);
+    need(d.s_cnt, (size_t)C * 8);
+    need(d.s_has, (size_t)C * 4);
+    need(d.s_free, (size_t)C * 4);
+    need(d.ht_key, (size_t)d.H * 8);
+    need(d.ht_slot, (size_t)d.H * 4);
+    for (DevBuf *bb : {&d.ct_cnt, &d.ct_rep, &d.ct_root, &d.ct_slot, &d.ct_wr, &d.ct_off, &d.ct_prog})
+        need(*bb, (size_t)d.H2 * 4);
+    need(d.ct_key, (size_t)d.H2 * 8);
+    need(d.hh, (size_t)M * 8);
+    for (DevBuf *bb : {&d.sz, &d.hit, &d.cov, &d.sh, &d.anc, &d.pslot, &d.ctx_idx, &d.flag, &d.rank})
+        need(*bb, (size_t)M * 4);
+    const int64_t P1 = (int64_t)d.L1 * PC, P2 = (int64_t)E * PC;
+    need(d.p1_tok, P1 * 4);
+    need(d.p1_val, P1 * 4);
+    need(d.p1_ins, P1 * sizeof(Ins));
+    need(d.p2_tok, P2 * 4);
+    need(d.p2_val, P2 * 4);
+    need(d.p2_ins, P2 * sizeof(Ins));
+    for (DevBuf *bb : {&d.p1_len, &d.p1_slot, &d.p1_pos, &d.p1_plen, &d.p1_order, &d.mse1})
+        need(*bb, (size_t)d.L1 * 4);
+    need(d.cnt1, (size_t)d.L1 * 8);
+    need(d.p1_pf, (size_t)d.L1 * kRefPrefetch * 4);
+    for (DevBuf *bb : {&d.p2_len, &d.p2_plen, &d.p2_order, &d.e_src, &d.mse2}) need(*bb, (size_t)E * 4);
+    need(d.cnt2, (size_t)E * 8);
+    need(d.p2_pf, (size_t)E * kRefPrefetch * 4);
+    need(d.cs_size, (size_t)(d.L1 + E) * PC * 4);
+    need(d.cs_need, (size_t)(d.L1 + E) * PC * 4);
+    need(d.cs_frames, (size_t)(d.L1 + E) * (PC + 1) * sizeof(CompileFrame));
+    need(d.ctr, C_COUNT * 4);
+    need(d.blk, (size_t)(std::max<int64_t>(M, C) / 256 + 2) * 4);
+    need(d.stats, S_COUNT * 8);
+    if (he != hipSuccess) return c.hip_fail(he, "hipMalloc device cache");
+    LaunchPlan lp;  // MSE partials and fitness buffers for max(launch-1 capacity, E) programs
+    int rc = plan_batched(c, MODE_MSE_REF, std::max(d.L1, E), lp);
+    if (rc) return rc;
+    DmArgs a = dm_args(c, 0);
+    hipLaunchKernelGGL(k_dm_init_slots, dim3(grid_of(C)), dim3(kThreads), 0, c.stream, a);
+    hipLaunchKernelGGL(k_dm_ht_clear, dim3(grid_of(d.H)), dim3(kThreads), 0, c.stream, a);
+    if ((he = hipGetLastError()) == hipSuccess) he = hipStreamSynchronize(c.stream);
+    if (he != hipSuccess) return c.hip_fail(he, "device cache init");
+    d.ready = true;
+    return SR_OK;
+}
+
+// Enqueue the evaluation of GA buffer b with the device cache: fitness into
+// c.d_mse (count into cnt_out when set); ev (optional, 4 events) brackets the
+// two interpreter launches.  Graph-capturable (without events).
+int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *ev)
+{
+    DMemoState &d = c.dm;
+    if (!d.ready) return c.fail(SR_ERR_INVALID_ARG, "device cache not set up");
+    const DmArgs a = dm_args(c, b);
+    const int64_t M = (int64_t)d.E * d.cap;
+    const int64_t big = std::max<int64_t>(M, std::max(d.L1, d.E));
+    hipError_t he;
+#define DM_LAUNCH(K, N, ...)                                                                     \
+    do {                                                                                         \
+        hipLaunchKernelGGL(K, dim3(grid_of(N)), dim3(kThreads), 0, s, a, ##__VA_ARGS__);         \
+        if ((he = hipGetLastError()) != hipSuccess) return c.hip_fail(he, #K);                   \
+    } while (0)
+    // one wave per item with dynamic LDS
+#define DM_WAVES(K, N, BYTES, ...)                                                               \
+    do {                                                                                         \
+        hipLaunchKernelGGL(K, dim3((unsigned)(N)), dim3(64), (BYTES), s, a, ##__VA_ARGS__);       \
+        if ((he = hipGetLastError()) != hipSuccess) return c.hip_fail(he, #K);                   \
+    } while (0)
+    int32_t *blk = d.blk.as<int32_t>();
+    auto scan = [&](int kind, int v, int ci) -> int {
+        const int64_t n = kind == 0 ? M : d.C;
+        const int nb = (int)((n + 255) / 256);
+        hipLaunchKernelGGL(k_dm_scan_count, dim3(nb), dim3(256), 0, s, a, kind, v, blk);
+        hipLaunchKernelGGL(k_dm_scan_blocks, dim3(1), dim3(1024), 0, s, a, blk, nb, ci);
+        hipLaunchKernelGGL(k_dm_scan_write, dim3(nb), dim3(256), 0, s, a, kind, v, (const int32_t *)blk);
+        if ((he = hipGetLastError()) != hipSuccess) return c.hip_fail(he, "device cache scan");
+        return SR_OK;
+    };
+    const size_t cap4 = (size_t)d.cap * 4, pc4 = (size_t)d.PC * 4;
+    hipLaunchKernelGGL(k_dm_begin, dim3(1), dim3(64), 0, s, a);
+    DM_LAUNCH(k_dm_clear_pos, big);
+    DM_LAUNCH(k_dm_clear_ct, d.H2);
+    DM_WAVES(k_dm_hash, d.E, 5 * cap4);
+    DM_LAUNCH(k_dm_lookup, M);
+    DM_LAUNCH(k_dm_cover, M);
+    DM_LAUNCH(k_dm_count, M);
+    DM_LAUNCH(k_dm_verify, M);
+    DM_LAUNCH(k_dm_outer, M);
+    DM_LAUNCH(k_dm_anc, M);
+    DM_LAUNCH(k_dm_writer, d.H2);
+    DM_LAUNCH(k_dm_evict, d.C);
+    if (int r = scan(1, 0, C_FREE)) return r;
+    DM_LAUNCH(k_dm_flag, d.H2, 1);
+    if (int r = scan(0, 1, C_ROOTS)) return r;
+    DM_LAUNCH(k_dm_alloc, d.H2, 1);
+    DM_LAUNCH(k_dm_flag, d.H2, 0);
+    if (int r = scan(0, 2, C_NESTED)) return r;
+    DM_LAUNCH(k_dm_alloc, d.H2, 0);
+    DM_LAUNCH(k_dm_ht_clear, d.H);
+    DM_LAUNCH(k_dm_ht_insert, d.C);
+    DM_LAUNCH(k_dm_pslot, M);
+    DM_WAVES(k_dm_emit1, d.L1, 5 * cap4 + 2 * pc4);
+    DM_WAVES(k_dm_emit2, d.E, 6 * cap4);
+    DM_WAVES(k_dm_compile, (int64_t)d.L1 + d.E,
+             (size_t)(d.PC + 1) * sizeof(CompileFrame) + (size_t)d.PC * sizeof(Ins) + 4 * pc4, c.nv);
+#undef DM_WAVES
+#undef DM_LAUNCH
+    if ((he = launch_order_by_len(d.p1_plen.as<int32_t>(), d.L1, d.p1_order.as<int32_t>(), s)) != hipSuccess ||
+        (he = launch_order_by_len(d.p2_plen.as<int32_t>(), d.E, d.p2_order.as<int32_t>(), s)) != hipSuccess)
+        return c.hip_fail(he, "device cache order");
+    LaunchPlan lp;
+    int rc = plan_launch(c, MODE_MSE_REF, lp);
+    if (rc) return rc;
+    float *cols = d.cols.as<float>();
+    // launch 1: the roots (count on the device); launch 2: the expressions,
+    // numbered by expression, so every expression's MSE slot is finalized
+    for (int launch = 0; launch < 2; ++launch) {
+        const bool one = launch == 0;
+        BatchArgs ba{};
+        ba.progs = one ? d.p1_ins.as<Ins>() : d.p2_ins.as<Ins>();
+        ba.prog_off = nullptr;
+        ba.prog_stride = d.PC;
+        ba.prog_len = one ? d.p1_plen.as<int32_t>() : d.p2_plen.as<int32_t>();
+        ba.order = one ? d.p1_order.as<int32_t>() : d.p2_order.as<int32_t>();
+        ba.ref_pf = one ? d.p1_pf.as<int32_t>() : d.p2_pf.as<int32_t>();
+        ba.cache = cols;
+        ba.out = cols;
+        ba.n_exprs_dev = d.ctr.as<int32_t>() + (one ? C_L1 : C_L2);
+        EvalBufs bufs{c.d_psum.as<double>(), c.d_pcnt.as<long long>(), c.d_counters.as<uint32_t>(),
+                      one ? d.mse1.as<float>() : d.mse2.as<float>(),
+                      one ? d.cnt1.as<long long>() : d.cnt2.as<long long>()};
+        bufs.fin_count = one ? ba.n_exprs_dev : nullptr;
+        rc = enqueue_batched(c, MODE_MSE_REF, lp, ba, one ? d.L1 : d.E, s, bufs, ev ? ev[2 * launch] : nullptr,
+                             ev ? ev[2 * launch + 1] : nullptr);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_dm_gather, dim3(grid_of(std::max(d.L1, d.E))), dim3(kThreads), 0, s, a, cnt_out);
+    if ((he = hipGetLastError()) != hipSuccess) return c.hip_fail(he, "k_dm_gather");
+    return SR_OK;
+}
+
+// Statistics since setup: logical / executed tokens, OP_REF / OP_STORE tokens,
+// operator tokens, launch-1 / launch-2 programs, live slots, generations.
+int dmemo_stats(Ctx &c, long long out[9])
+{
+    if (!c.dm.ready) {
+        for (int k = 0; k < S_COUNT; ++k) out[k] = 0;
+        return SR_OK;
+    }
+    hipError_t he = hipMemcpy(out, c.dm.stats.p, S_COUNT * 8, hipMemcpyDeviceToHost);
+    if (he != hipSuccess) return c.hip_fail(he, "device cache statistics");
+    int32_t st = 0;
+    he = hipMemcpy(&st, c.dm.ctr.as<int32_t>() + C_STATUS, 4, hipMemcpyDeviceToHost);
+    if (he != hipSuccess) return c.hip_fail(he, "device cache status");
+    if (st) return c.fail(SR_ERR_BAD_EXPR, "device cache: a skeleton program failed to compile");
+    return SR_OK;
+}
+
+}  // namespace sr

@@ -54,10 +54,12 @@ hipError_t launch_plain(int s_slots, int xc, const Ins *prog, int len, const flo
 // One wave per expression; partials summed lane-strided then by a fixed
 // shuffle tree, so the result is deterministic.
 __global__ __launch_bounds__(kBlock) void k_finalize(const double *psum, const long long *pcnt,
-                                                     int P, int E, float *mse, long long *cnt)
+                                                     int P, int E, const int *E_dev, float *mse,
+                                                     long long *cnt)
 {
     const int e = (int)((blockIdx.x * (unsigned)kBlock + threadIdx.x) / kWave);
     const int lane = threadIdx.x & (kWave - 1);
+    if (E_dev && e >= *E_dev) return;
     if (e >= E) return;
     double s = 0.0;
     long long c = 0;
@@ -74,11 +76,11 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double *psum, const l
 }
 
 hipError_t launch_finalize(const double *psum, const long long *pcnt, int P, int E, float *mse,
-                           long long *cnt, hipStream_t s)
+                           long long *cnt, hipStream_t s, const int *E_dev)
 {
     const int per_block = kBlock / kWave;
     hipLaunchKernelGGL(k_finalize, dim3((E + per_block - 1) / per_block), dim3(kBlock), 0, s, psum,
-                       pcnt, P, E, mse, cnt);
+                       pcnt, P, E, E_dev, mse, cnt);
     return hipGetLastError();
 }
 

@@ -267,12 +267,18 @@ int sr_ga_evaluate(sr_ctx *x, int mode, float *mse_out, sr_stats *st)
     if (!c.X) return c.fail(SR_ERR_NO_DATA, "no data: call sr_set_data first");
     GaState &g = c.ga;
     g.last_mse.resize(g.pop);
-    if (mode == SR_EVAL_BATCHED) {
+    const bool device_cache = mode == SR_EVAL_CACHED && c.memo.device;
+    if (mode == SR_EVAL_BATCHED || device_cache) {
         LaunchPlan lp;
         rc = plan_batched(c, MODE_MSE, g.pop, lp);
         if (rc) return rc;
+        long long before[9] = {0};
+        if (device_cache) {
+            if ((rc = dmemo_setup(c, c.memo.capacity, c.memo.min_size)) != SR_OK) return rc;
+            if ((rc = dmemo_stats(c, before)) != SR_OK) return rc;
+        }
         auto t0 = std::chrono::steady_clock::now();
-        rc = ga_enqueue_eval(c, g.cur, lp, c.stream);
+        rc = device_cache ? dmemo_enqueue(c, g.cur, c.stream, nullptr) : ga_enqueue_eval(c, g.cur, lp, c.stream);
         if (rc) return rc;
         hipError_t he = hipMemcpyAsync(g.last_mse.data(), c.d_mse.p, (size_t)g.pop * 4,
                                        hipMemcpyDeviceToHost, c.stream);
@@ -289,6 +295,16 @@ int sr_ga_evaluate(sr_ctx *x, int mode, float *mse_out, sr_stats *st)
             st->num_kernel_launches = 5;
             st->total_eval_time_ms =
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (device_cache) {
+                long long after[9];
+                if ((rc = dmemo_stats(c, after)) != SR_OK) return rc;
+                st->executed_node_evals = (after[1] - before[1]) * c.N;
+                st->column_reads = (after[2] - before[2]) * c.N;
+                st->column_writes = (after[3] - before[3]) * c.N;
+                st->executed_op_evals = (after[4] - before[4]) * c.N;
+                st->num_cached = (int32_t)after[7];
+                st->num_subtrees = (int32_t)after[7];
+            }
         }
     } else {
         std::vector<int32_t> tok, off, len;
@@ -345,7 +361,8 @@ int sr_ga_run(sr_ctx *x, const sr_select_params *spp, int mode, int32_t gens, ui
     sr_stats acc{};
     const SelectParams sp = to_select(*spp, g.pop);
     hipError_t he;
-    if (mode != SR_EVAL_BATCHED) {
+    const bool device_cache = mode == SR_EVAL_CACHED && c.memo.device;
+    if (mode != SR_EVAL_BATCHED && !device_cache) {
         // Host-assisted modes: evaluate through the host path, vary on the device.
         auto t0 = std::chrono::steady_clock::now();
         std::vector<float> m(g.pop);
@@ -381,6 +398,11 @@ int sr_ga_run(sr_ctx *x, const sr_select_params *spp, int mode, int32_t gens, ui
     LaunchPlan lp;
     rc = plan_batched(c, MODE_MSE, g.pop, lp);
     if (rc) return rc;
+    long long dm_before[9] = {0};
+    if (device_cache) {
+        if ((rc = dmemo_setup(c, c.memo.capacity, c.memo.min_size)) != SR_OK) return rc;
+        if ((rc = dmemo_stats(c, dm_before)) != SR_OK) return rc;
+    }
     if ((he = g.hist.ensure((size_t)std::max(gens, 1) * g.pop * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
     if ((he = g.hist_len.ensure((size_t)std::max(gens, 1) * g.pop * 4)) != hipSuccess)
         return c.hip_fail(he, "hipMalloc");
@@ -391,8 +413,13 @@ int sr_ga_run(sr_ctx *x, const sr_select_params *spp, int mode, int32_t gens, ui
         if ((he = hipStreamSynchronize(nullptr)) != hipSuccess) return c.hip_fail(he, "sync");
         s = c.own_stream;
     }
+    // without a graph, the device cache's two interpreter launches are timed
+    // with events (one synchronisation per generation)
+    const bool time_launches = device_cache && !use_graph;
+    hipEvent_t kev[4] = {c.ev[4], c.ev[5], c.ev2[4], c.ev2[5]};
     auto enqueue_gen = [&](int b) -> int {
-        int r = ga_enqueue_eval(c, b, lp, s);
+        int r = device_cache ? dmemo_enqueue(c, b, s, nullptr, time_launches ? kev : nullptr)
+                             : ga_enqueue_eval(c, b, lp, s);
         if (r) return r;
         hipError_t e = launch_record(c.d_mse.p, g.pop, g.hist.p, g.gen.as<uint32_t>(), gen0, gens, s);
         if (e == hipSuccess)
@@ -412,11 +439,13 @@ int sr_ga_run(sr_ctx *x, const sr_select_params *spp, int mode, int32_t gens, ui
         const void *ptrs[] = {g.tok[0].p, g.tok[1].p, g.val[0].p, g.val[1].p, g.len[0].p, g.len[1].p,
                               g.progs.p, g.prog_len.p, g.size_s.p, g.need_s.p, g.frames.p, g.order_len.p,
                               g.order_fit.p, g.status.p, g.max_need.p, g.gen.p, g.hist.p, g.hist_len.p,
-                              c.X, c.d_mse.p, c.d_cnt.p, c.d_psum.p, c.d_pcnt.p, c.d_counters.p, (const void *)s};
+                              c.X, c.d_mse.p, c.d_cnt.p, c.d_psum.p, c.d_pcnt.p, c.d_counters.p, (const void *)s,
+                              device_cache ? c.dm.cols.p : nullptr, device_cache ? c.dm.ctr.p : nullptr};
         for (const void *q : ptrs) key = mix(key, (uint64_t)(uintptr_t)q);
         for (uint64_t v : {(uint64_t)g.pop, (uint64_t)g.cap, (uint64_t)c.N, (uint64_t)c.nv, (uint64_t)lp.dpt,
                            (uint64_t)lp.chunk, (uint64_t)lp.nq, (uint64_t)lp.grid, (uint64_t)lp.P, seed,
-                           (uint64_t)sp.tournament, (uint64_t)sp.elites, (uint64_t)gen0, (uint64_t)gens})
+                           (uint64_t)sp.tournament, (uint64_t)sp.elites, (uint64_t)gen0, (uint64_t)gens,
+                           (uint64_t)device_cache, (uint64_t)lp.dpt, (uint64_t)c.tune.cache_dpt})
             key = mix(key, v);
         uint32_t pm;
         std::memcpy(&pm, &sp.p_mutation, 4);
@@ -451,6 +480,13 @@ int sr_ga_run(sr_ctx *x, const sr_select_params *spp, int mode, int32_t gens, ui
         } else {
             rc = enqueue_gen(g.cur);
             if (rc) return rc;
+            if (time_launches) {
+                if ((he = hipEventSynchronize(kev[3])) != hipSuccess) return c.hip_fail(he, "device cache");
+                float t1 = 0.f, t2 = 0.f;
+                hipEventElapsedTime(&t1, kev[0], kev[1]);
+                hipEventElapsedTime(&t2, kev[2], kev[3]);
+                acc.interp_kernel_ms += t1 + t2;
+            }
         }
         g.cur ^= 1;
     }
@@ -470,6 +506,18 @@ int sr_ga_run(sr_ctx *x, const sr_select_params *spp, int mode, int32_t gens, ui
     }
     for (int v : lens) acc.logical_node_evals += (int64_t)v * c.N;
     acc.executed_node_evals = acc.logical_node_evals;
+    if (device_cache) {
+        long long after[9];
+        if ((rc = dmemo_stats(c, after)) != SR_OK) return rc;
+        acc.executed_node_evals = (after[1] - dm_before[1]) * c.N;
+        acc.column_reads = (after[2] - dm_before[2]) * c.N;
+        acc.column_writes = (after[3] - dm_before[3]) * c.N;
+        acc.executed_op_evals = (after[4] - dm_before[4]) * c.N;
+        acc.num_cached = (int32_t)after[7];
+        acc.num_subtrees = (int32_t)after[7];
+        acc.num_new_subtrees = (int32_t)(after[3] - dm_before[3]);
+        acc.interp_launches = 2 * gens;
+    }
     acc.kernel_time_ms = ms;
     acc.total_eval_time_ms = ms;
     acc.num_kernel_launches = gens * 11;

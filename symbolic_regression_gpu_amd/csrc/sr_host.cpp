@@ -186,7 +186,7 @@ int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, 
         return c.hip_fail(he, "batched launch");
     if (k1) hipEventRecord(k1, s);
     if (is_mse_mode(mode)) {
-        if ((he = launch_finalize(a.psum, a.pcnt, lp.P, E, b.mse, b.cnt, s)) != hipSuccess)
+        if ((he = launch_finalize(a.psum, a.pcnt, lp.P, E, b.mse, b.cnt, s, b.fin_count)) != hipSuccess)
             return c.hip_fail(he, "finalize launch");
     }
     return SR_OK;
@@ -500,6 +500,9 @@ int sr_eval_mse(sr_ctx *x, const sr_population *pop, int mode, float *mse_out, i
 
     if (mode == SR_EVAL_JIT) {
         rc = eval_jit(c, *pop, S);
+    } else if (mode == SR_EVAL_CACHED && c.memo.device) {
+        return c.fail(SR_ERR_UNSUPPORTED, "SR_CACHE_DEVICE evaluates the context's GA population "
+                                          "(sr_ga_run / sr_ga_evaluate), not a host population");
     } else if (mode == SR_EVAL_CACHED && c.memo.enabled) {
         // results come back through the cache (some are remembered MSEs)
         rc = eval_memo(c, *pop, S, mse_out, valid_out);

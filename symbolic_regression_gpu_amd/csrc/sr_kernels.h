@@ -43,6 +43,7 @@ struct BatchArgs {
     int64_t slice;            // datapoints per XCD queue (multiple of 64)
     int64_t chunk;            // datapoints per work item
     int n_exprs;
+    const int *n_exprs_dev;   // when set: the program count, read on the device (n_exprs = capacity)
     int nv;
     int n_queues;             // 1 or 8 (one per XCD)
     int chunks_per_queue;
@@ -122,7 +123,8 @@ void k_batched(BatchArgs a)
     const int lane = threadIdx.x & (kWave - 1);
     Ins *lp = s_prog[threadIdx.x / kWave];
     const int q = a.n_queues > 1 ? (int)(blockIdx.x % (unsigned)a.n_queues) : 0;
-    const int items = a.n_exprs * a.chunks_per_queue;
+    const int n_exprs = a.n_exprs_dev ? min(*a.n_exprs_dev, a.n_exprs) : a.n_exprs;
+    const int items = n_exprs * a.chunks_per_queue;
     const int64_t N = a.N;
     constexpr int64_t kTile = (int64_t)kWave * DPT;
     // OP_REF operands held in registers (current and next tile): 2 * KR * DPT
@@ -139,8 +141,8 @@ void k_batched(BatchArgs a)
             // chunk-major: the waves of a queue work on one datapoint chunk
             // at a time, so its X columns stay in the XCD's L2 across
             // expressions; within a chunk, longest expression first.
-            const int chunk = item / a.n_exprs;
-            const int rank = item - chunk * a.n_exprs;
+            const int chunk = item / n_exprs;
+            const int rank = item - chunk * n_exprs;
             const int e = a.order[rank];
             const Ins *prog =
                 a.progs + (a.prog_off ? (int64_t)a.prog_off[e] : (int64_t)e * a.prog_stride);

@@ -28,9 +28,10 @@ int batched_occupancy(int mode, int dpt, int xc);
 hipError_t launch_plain(int s_slots, int xc, const Ins *prog, int len, const float *X,
                         int64_t N, int nv, double *psum, long long *pcnt, hipStream_t s);
 
-// mse[e] = sum / count over P partials (NaN when count == 0).
+// mse[e] = sum / count over P partials (NaN when count == 0), for e < E
+// (and e < *E_dev when E_dev is set).
 hipError_t launch_finalize(const double *psum, const long long *pcnt, int P, int E,
-                           float *mse, long long *cnt, hipStream_t s);
+                           float *mse, long long *cnt, hipStream_t s, const int *E_dev = nullptr);
 // Zero n words (graph-capturable: a kernel, not a memset node).
 hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t s);
 

@@ -582,16 +582,20 @@ int sr_cache_policy(sr_ctx *x, int32_t policy, int32_t capacity, int32_t min_siz
     Ctx &c = x->c;
     std::lock_guard<std::recursive_mutex> lk(c.mu);
     if (policy != SR_CACHE_REFERENCE && policy != SR_CACHE_GENERATIONAL &&
-        policy != SR_CACHE_GENERATIONAL_REUSED && policy != SR_CACHE_GENERATIONAL_SHARED)
+        policy != SR_CACHE_GENERATIONAL_REUSED && policy != SR_CACHE_GENERATIONAL_SHARED &&
+        policy != SR_CACHE_DEVICE)
         return c.fail(SR_ERR_INVALID_ARG, "unknown cache policy");
     if (capacity < 0 || min_size < 2) return c.fail(SR_ERR_INVALID_ARG, "cache capacity < 0 or min_size < 2");
     MemoState &m = c.memo;
-    m.enabled = policy != SR_CACHE_REFERENCE;
+    m.device = policy == SR_CACHE_DEVICE;
+    m.enabled = policy != SR_CACHE_REFERENCE && !m.device;
     m.capacity = capacity;
     m.min_size = min_size;
     m.policy = m.enabled ? policy : SR_CACHE_GENERATIONAL;
     m.planner.forget();
     m.cols.release();  // re-sized and re-configured at the next evaluation
+    c.dm.ready = false;
+    c.ga.release_graphs();
     return SR_OK;
 }
 
