@@ -379,24 +379,37 @@ def main():
         cpu = cpu_baseline(X, nv, pop0.tokens, pop0.values, pop0.offsets, pop0.lengths)
 
     # roofline of the dominant kernel of the timed steps: the skeleton
-    # interpreter (k_batched, MODE_MSE_REF), over every launch of the timed
+    # interpreter (MODE_MSE_REF), over every launch of the timed
     # generations (HIP events around each launch on the context stream)
     launches = max(st_t["interp_launches"], 1)
     k_ms = st_t["interp_kernel_ms"] / launches
     flops = st_t["executed_op_evals"] / launches
     achieved = flops / (k_ms * 1e-3) / 1e12
     hbm_alg = (st_t["column_reads"] + st_t["column_writes"]) * 4 / launches + xy_bytes
+    def profiled_traffic(path, kernel):
+        """HBM bytes per launch of a FETCH_SIZE + WRITE_SIZE counter run, if it covers this kernel."""
+        if not os.path.exists(path):
+            return None
+        with open(path) as f:
+            tj = json.load(f)
+        return tj.get("hbm_bytes_per_launch") if tj.get("kernel", "").startswith(kernel.split("<")[0]) else None
+
     traffic = None
-    if os.path.exists(TRAFFIC_FILE_REF):
-        with open(TRAFFIC_FILE_REF) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
     b_ms = batched0["interp_kernel_ms"]
     b_ach = flops0 / (b_ms * 1e-3) / 1e12
     def xc_of(dpt):
         """Variable-register class of the batched kernel variant (<= 4 variables)."""
         return 3 if nv <= 3 and dpt == 3 else 4
 
+    def kernel_name(kind, dpt, mode):
+        """Name of the interpreter kernel a launch plan runs (as the kernel trace shows it)."""
+        if kind == 1:
+            return f"k_vec<{mode}>"
+        return f"k_batched<{dpt},9,{xc_of(dpt)},{mode}>"
+
     cache_dpt = tuned.cache_dpt if tuned.cache_dpt > 0 else 3
+    ref_kernel = kernel_name(tuned.cache_kernel, cache_dpt, "MODE_MSE_REF")
+    mse_kernel = kernel_name(tuned.kernel, tuned.dpt, "MODE_MSE")
     valu_ref = valu_mse = None
     if os.path.exists(VALU_FILE):
         with open(VALU_FILE) as f:
@@ -407,14 +420,14 @@ def main():
             issue = {"achieved": v["valu_instr_per_s"], "peak": v["valu_issue_peak_per_s"],
                      "unit": "VALU wave64 instructions/s", "frac": v["valu_issue_frac"],
                      "source": "profiles/valu_issue_bench_default.json (counter run)"}
-            if "MODE_MSE_REF" in k:
+            # a counter run describes the kernel it names, nothing else
+            kk = k.replace(" ", "")
+            if kk.startswith(ref_kernel.split("<")[0]) and "MODE_MSE_REF" in kk:
                 valu_ref = issue
-            elif "MODE_MSE" in k:
+            elif kk.startswith(mse_kernel.split("<")[0]) and "MODE_MSE" in kk and "MODE_MSE_REF" not in kk:
                 valu_mse = issue
-    b_traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            b_traffic = json.load(f).get("hbm_bytes_per_launch")
+    traffic = profiled_traffic(TRAFFIC_FILE_REF, ref_kernel)
+    b_traffic = profiled_traffic(TRAFFIC_FILE, mse_kernel)
     out = {
         "metric": METRIC,
         "value": value,
@@ -438,8 +451,8 @@ def main():
             "tokens_final": int(popK.tokens.size),
             "node_evals_per_generation": st_b["logical_node_evals"] // max(K, 1),
             "best_mse_last_generation": best,
-            "tuning": {"dpt": tuned.dpt, "chunk": tuned.chunk, "n_queues": tuned.n_queues,
-                       "blocks_per_cu": tuned.blocks_per_cu},
+            "tuning": {"kernel": mse_kernel, "cache_kernel": ref_kernel, "chunk": tuned.chunk,
+                       "n_queues": tuned.n_queues, "blocks_per_cu": tuned.blocks_per_cu},
         },
         "roofline": {
             "bound": "valu",
@@ -448,8 +461,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": traffic,
-            "kernel": f"k_batched<{cache_dpt},9,{xc_of(cache_dpt)},MODE_MSE_REF> (cache skeletons), average over the "
-                      "timed generations' launches",
+            "kernel": f"{ref_kernel} (cache columns and skeletons), average over the timed generations' launches",
             "kernel_ms": k_ms,
             "launches": launches,
             "flops_per_launch": flops,
@@ -461,7 +473,7 @@ def main():
         "roofline_batched": {
             "bound": "valu", "achieved": b_ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": b_ach / PEAK_FP32_TFLOPS, "traffic": b_traffic,
-            "kernel": f"k_batched<{tuned.dpt},9,{xc_of(tuned.dpt)},MODE_MSE>, one launch = the initial 1000 expressions x 1M",
+            "kernel": f"{mse_kernel}, one launch = the initial 1000 expressions x 1M",
             "kernel_ms": b_ms, "flops_per_launch": flops0,
             "hbm_algorithmic_bytes_per_launch": xy_bytes,
             "hbm_achieved_GBs": xy_bytes / (b_ms * 1e-3) / 1e9,

@@ -99,6 +99,9 @@ typedef struct {
     int32_t n_queues;      /* 1, or 8 = one queue per XCD */
     int32_t blocks_per_cu; /* persistent workgroups per CU (0 = occupancy limit) */
     int32_t cache_dpt;     /* datapoints per lane of cache-skeleton launches (0 = 3) */
+    int32_t kernel;        /* whole expressions: 0 = lane interpreter (dpt datapoints per lane),
+                              1 = vector interpreter (384 datapoints per instruction; default) */
+    int32_t cache_kernel;  /* the same choice for cache skeletons (default 1) */
 } sr_tuning;
 
 int sr_ctx_create(int device, sr_ctx **out);

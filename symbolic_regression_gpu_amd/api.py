@@ -64,7 +64,8 @@ class _Stats(C.Structure):
 
 class _Tuning(C.Structure):
     _fields_ = [("dpt", C.c_int32), ("chunk", C.c_int32), ("n_queues", C.c_int32),
-                ("blocks_per_cu", C.c_int32), ("cache_dpt", C.c_int32)]
+                ("blocks_per_cu", C.c_int32), ("cache_dpt", C.c_int32), ("kernel", C.c_int32),
+                ("cache_kernel", C.c_int32)]
 
 
 class GaParams(C.Structure):
@@ -444,6 +445,8 @@ class Tuning:
     n_queues: int = 8
     blocks_per_cu: int = 0
     cache_dpt: int = 3
+    kernel: int = 1        # whole expressions: 0 lane interpreter, 1 vector interpreter
+    cache_kernel: int = 1  # cache skeletons: 0 lane interpreter, 1 vector interpreter
 
 
 class Context:
@@ -524,11 +527,11 @@ class Context:
     def tuning(self) -> Tuning:
         t = _Tuning()
         self._check(self._L.sr_get_tuning(self._h, C.byref(t)))
-        return Tuning(t.dpt, t.chunk, t.n_queues, t.blocks_per_cu, t.cache_dpt)
+        return Tuning(t.dpt, t.chunk, t.n_queues, t.blocks_per_cu, t.cache_dpt, t.kernel, t.cache_kernel)
 
     @tuning.setter
     def tuning(self, v: Tuning):
-        t = _Tuning(v.dpt, v.chunk, v.n_queues, v.blocks_per_cu, v.cache_dpt)
+        t = _Tuning(v.dpt, v.chunk, v.n_queues, v.blocks_per_cu, v.cache_dpt, v.kernel, v.cache_kernel)
         self._check(self._L.sr_set_tuning(self._h, C.byref(t)))
 
     # ---- subtree cache ----------------------------------------------------
@@ -538,7 +541,7 @@ class Context:
         t = _Tuning()
         ms = C.c_double()
         self._check(self._L.sr_autotune(self._h, C.byref(pop._c()), max_sample, C.byref(t), C.byref(ms)))
-        return Tuning(t.dpt, t.chunk, t.n_queues, t.blocks_per_cu, t.cache_dpt), ms.value
+        return Tuning(t.dpt, t.chunk, t.n_queues, t.blocks_per_cu, t.cache_dpt, t.kernel, t.cache_kernel), ms.value
 
     def param_autotune(self, d_cvals: int):
         """Datapoints per lane for the forward and adjoint constant-fitting kernels."""

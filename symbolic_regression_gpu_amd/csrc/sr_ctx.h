@@ -225,12 +225,12 @@ struct Ctx {
     int64_t N = 0;
     uint64_t data_version = 0;
 
-    sr_tuning tune{3, 3072, 8, 0, 3};
+    sr_tuning tune{3, 3072, 8, 0, 3, 1, 1};
 
     // staging (host pinned) and device buffers for one evaluation; the
     // second program set lets a launch be prepared while one is in flight
     HostBuf h_prog, h_meta, h_out;
-    DevBuf d_prog, d_meta, d_psum, d_pcnt, d_mse, d_cnt, d_counters, d_pred;
+    DevBuf d_prog, d_meta, d_psum, d_pcnt, d_mse, d_cnt, d_counters, d_pred, d_spill;
     HostBuf h_prog2, h_meta2, h_res[2];
     DevBuf d_prog2, d_meta2;
 
@@ -277,11 +277,13 @@ void expr_values_f32(const sr_population &pop, int e, std::vector<float> &out);
 // Launch shape of one batched evaluation (see plan_batched).
 struct LaunchPlan {
     int dpt = 4, xc = 4, nq = 8, cpq = 1, P = 8, grid = 0;
+    int kind = 0;  // KERNEL_LANE or KERNEL_VEC
     int64_t slice = 0, chunk = 0;
 };
 struct BatchArgs;
-// Pick the variant + work decomposition (no allocation).
-int plan_launch(Ctx &c, int mode, LaunchPlan &lp);
+// Pick the variant + work decomposition (no allocation); allow_vec = false
+// keeps the lane interpreter (no spill area).
+int plan_launch(Ctx &c, int mode, LaunchPlan &lp, bool allow_vec = true);
 // plan_launch for E expressions, and make sure the context's partial /
 // result buffers exist.
 int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp);
@@ -294,6 +296,7 @@ struct EvalBufs {
     float *mse;
     long long *cnt;
     const int *fin_count = nullptr;  // finalize only programs below this device count
+    float *spill = nullptr;          // vector interpreter stack spill (plan kind KERNEL_VEC)
 };
 // Enqueue counter reset + batched kernel (+ finalize for MODE_MSE).  a must
 // carry the program / order / output pointers; the rest is filled in here.

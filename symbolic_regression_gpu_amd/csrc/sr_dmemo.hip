@@ -948,6 +948,7 @@ int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *
                       one ? d.mse1.as<float>() : d.mse2.as<float>(),
                       one ? d.cnt1.as<long long>() : d.cnt2.as<long long>()};
         bufs.fin_count = one ? ba.n_exprs_dev : nullptr;
+        bufs.spill = c.d_spill.as<float>();
         rc = enqueue_batched(c, MODE_MSE_REF, lp, ba, one ? d.L1 : d.E, s, bufs, ev ? ev[2 * launch] : nullptr,
                              ev ? ev[2 * launch + 1] : nullptr);
         if (rc) return rc;

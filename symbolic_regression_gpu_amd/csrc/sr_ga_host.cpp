@@ -278,7 +278,12 @@ int sr_ga_evaluate(sr_ctx *x, int mode, float *mse_out, sr_stats *st)
             if ((rc = dmemo_stats(c, before)) != SR_OK) return rc;
         }
         auto t0 = std::chrono::steady_clock::now();
-        rc = device_cache ? dmemo_enqueue(c, g.cur, c.stream, nullptr) : ga_enqueue_eval(c, g.cur, lp, c.stream);
+        hipEvent_t kev[4] = {c.ev[4], c.ev[5], c.ev2[4], c.ev2[5]};
+        if (device_cache) {
+            LaunchPlan lr;
+            if ((rc = plan_batched(c, MODE_MSE_REF, std::max(c.dm.L1, g.pop), lr)) != SR_OK) return rc;
+        }
+        rc = device_cache ? dmemo_enqueue(c, g.cur, c.stream, nullptr, kev) : ga_enqueue_eval(c, g.cur, lp, c.stream);
         if (rc) return rc;
         hipError_t he = hipMemcpyAsync(g.last_mse.data(), c.d_mse.p, (size_t)g.pop * 4,
                                        hipMemcpyDeviceToHost, c.stream);
@@ -298,6 +303,11 @@ int sr_ga_evaluate(sr_ctx *x, int mode, float *mse_out, sr_stats *st)
             if (device_cache) {
                 long long after[9];
                 if ((rc = dmemo_stats(c, after)) != SR_OK) return rc;
+                float t1 = 0.f, t2 = 0.f;
+                hipEventElapsedTime(&t1, kev[0], kev[1]);
+                hipEventElapsedTime(&t2, kev[2], kev[3]);
+                st->interp_kernel_ms = t1 + t2;
+                st->interp_launches = 2;
                 st->executed_node_evals = (after[1] - before[1]) * c.N;
                 st->column_reads = (after[2] - before[2]) * c.N;
                 st->column_writes = (after[3] - before[3]) * c.N;
@@ -402,6 +412,8 @@ int sr_ga_run(sr_ctx *x, const sr_select_params *spp, int mode, int32_t gens, ui
     if (device_cache) {
         if ((rc = dmemo_setup(c, c.memo.capacity, c.memo.min_size)) != SR_OK) return rc;
         if ((rc = dmemo_stats(c, dm_before)) != SR_OK) return rc;
+        LaunchPlan lr;  // skeleton launches: buffers (spill area) sized before any capture
+        if ((rc = plan_batched(c, MODE_MSE_REF, std::max(c.dm.L1, g.pop), lr)) != SR_OK) return rc;
     }
     if ((he = g.hist.ensure((size_t)std::max(gens, 1) * g.pop * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
     if ((he = g.hist_len.ensure((size_t)std::max(gens, 1) * g.pop * 4)) != hipSuccess)
@@ -440,12 +452,13 @@ int sr_ga_run(sr_ctx *x, const sr_select_params *spp, int mode, int32_t gens, ui
                               g.progs.p, g.prog_len.p, g.size_s.p, g.need_s.p, g.frames.p, g.order_len.p,
                               g.order_fit.p, g.status.p, g.max_need.p, g.gen.p, g.hist.p, g.hist_len.p,
                               c.X, c.d_mse.p, c.d_cnt.p, c.d_psum.p, c.d_pcnt.p, c.d_counters.p, (const void *)s,
-                              device_cache ? c.dm.cols.p : nullptr, device_cache ? c.dm.ctr.p : nullptr};
+                              device_cache ? c.dm.cols.p : nullptr, device_cache ? c.dm.ctr.p : nullptr, c.d_spill.p};
         for (const void *q : ptrs) key = mix(key, (uint64_t)(uintptr_t)q);
         for (uint64_t v : {(uint64_t)g.pop, (uint64_t)g.cap, (uint64_t)c.N, (uint64_t)c.nv, (uint64_t)lp.dpt,
                            (uint64_t)lp.chunk, (uint64_t)lp.nq, (uint64_t)lp.grid, (uint64_t)lp.P, seed,
                            (uint64_t)sp.tournament, (uint64_t)sp.elites, (uint64_t)gen0, (uint64_t)gens,
-                           (uint64_t)device_cache, (uint64_t)lp.dpt, (uint64_t)c.tune.cache_dpt})
+                           (uint64_t)device_cache, (uint64_t)lp.dpt, (uint64_t)c.tune.cache_dpt,
+                           (uint64_t)c.tune.kernel, (uint64_t)c.tune.cache_kernel})
             key = mix(key, v);
         uint32_t pm;
         std::memcpy(&pm, &sp.p_mutation, 4);

@@ -114,8 +114,12 @@ int build_program(Ctx &c, const sr_population &pop, bool raw, Program &out, int 
 
 static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
-int plan_launch(Ctx &c, int mode, LaunchPlan &lp)
+int plan_launch(Ctx &c, int mode, LaunchPlan &lp, bool allow_vec)
 {
+    lp.kind = KERNEL_LANE;
+    if (allow_vec && ((mode == MODE_MSE && c.tune.kernel == KERNEL_VEC) ||
+                      (mode == MODE_MSE_REF && c.tune.cache_kernel == KERNEL_VEC)))
+        lp.kind = KERNEL_VEC;
     lp.xc = pick_xc(c.nv);
     lp.dpt = mode == MODE_MSE_REF ? (c.tune.cache_dpt > 0 ? c.tune.cache_dpt : 3) : c.tune.dpt;
     if (lp.dpt < 1 || lp.dpt > 4) lp.dpt = 4;
@@ -133,7 +137,7 @@ int plan_launch(Ctx &c, int mode, LaunchPlan &lp)
     if (lp.chunk > lp.slice) lp.chunk = lp.slice;
     lp.cpq = (int)((lp.slice + lp.chunk - 1) / lp.chunk);
     lp.P = lp.nq * lp.cpq;
-    int occ = batched_occupancy(mode, lp.dpt, lp.xc);
+    int occ = lp.kind == KERNEL_VEC ? vec_occupancy(mode) : batched_occupancy(mode, lp.dpt, lp.xc);
     if (occ <= 0) return c.fail(SR_ERR_UNSUPPORTED, "no kernel variant for this dpt / variable count");
     int bpc = c.tune.blocks_per_cu;
     if (bpc <= 0 || bpc > occ) bpc = occ;
@@ -147,6 +151,9 @@ int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp)
     int rc = plan_launch(c, mode, lp);
     if (rc) return rc;
     if ((he = c.d_counters.ensure(64 * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    if (lp.kind == KERNEL_VEC &&
+        (he = c.d_spill.ensure((size_t)lp.grid * (kBlock / kWave) * kVecSpillBytesPerWave)) != hipSuccess)
+        return c.hip_fail(he, "hipMalloc");
     if (is_mse_mode(mode)) {
         if ((he = c.d_psum.ensure((size_t)E * lp.P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
         if ((he = c.d_pcnt.ensure((size_t)E * lp.P * 8)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
@@ -159,8 +166,9 @@ int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp)
 int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, hipStream_t s,
                     hipEvent_t k0, hipEvent_t k1)
 {
-    const EvalBufs b{c.d_psum.as<double>(), c.d_pcnt.as<long long>(), c.d_counters.as<uint32_t>(),
-                     c.d_mse.as<float>(), c.d_cnt.as<long long>()};
+    EvalBufs b{c.d_psum.as<double>(), c.d_pcnt.as<long long>(), c.d_counters.as<uint32_t>(),
+               c.d_mse.as<float>(), c.d_cnt.as<long long>()};
+    b.spill = c.d_spill.as<float>();
     return enqueue_batched(c, mode, lp, a, E, s, b, k0, k1);
 }
 
@@ -182,8 +190,9 @@ int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, 
     a.P = lp.P;
     if ((he = launch_zero_u32(a.counters, lp.nq, s)) != hipSuccess) return c.hip_fail(he, "zero counters");
     if (k0) hipEventRecord(k0, s);
-    if ((he = launch_batched(mode, lp.dpt, lp.xc, a, lp.grid, s)) != hipSuccess)
-        return c.hip_fail(he, "batched launch");
+    a.spill = b.spill;
+    he = lp.kind == KERNEL_VEC ? launch_vec(mode, a, lp.grid, s) : launch_batched(mode, lp.dpt, lp.xc, a, lp.grid, s);
+    if (he != hipSuccess) return c.hip_fail(he, "batched launch");
     if (k1) hipEventRecord(k1, s);
     if (is_mse_mode(mode)) {
         if ((he = launch_finalize(a.psum, a.pcnt, lp.P, E, b.mse, b.cnt, s, b.fin_count)) != hipSuccess)
@@ -629,6 +638,8 @@ int sr_set_tuning(sr_ctx *x, const sr_tuning *t)
     Ctx &c = x->c;
     if (t->dpt < 1 || t->dpt > 4) return c.fail(SR_ERR_INVALID_ARG, "dpt must be 1, 2, 3 or 4");
     if (t->cache_dpt < 0 || t->cache_dpt > 4) return c.fail(SR_ERR_INVALID_ARG, "cache_dpt must be 0..4");
+    if (t->kernel < 0 || t->kernel > 1 || t->cache_kernel < 0 || t->cache_kernel > 1)
+        return c.fail(SR_ERR_INVALID_ARG, "kernel / cache_kernel must be 0 (lane) or 1 (vector)");
     if (t->chunk < 64 || (t->n_queues != 1 && t->n_queues != 8) || t->blocks_per_cu < 0)
         return c.fail(SR_ERR_INVALID_ARG, "bad tuning");
     c.tune = *t;

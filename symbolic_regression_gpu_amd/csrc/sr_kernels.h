@@ -49,6 +49,7 @@ struct BatchArgs {
     int chunks_per_queue;
     int P;                    // partial slots per expression = n_queues*chunks_per_queue
     int prog_stride;          // programs at e*prog_stride when prog_off is null
+    float *spill;             // vector interpreter: per-wave stack spill area (k_vec)
 };
 
 // ---------------------------------------------------------------------------

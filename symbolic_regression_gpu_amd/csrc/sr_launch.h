@@ -23,6 +23,14 @@ hipError_t launch_batched(int mode, int dpt, int xc, const BatchArgs &a, int gri
                           hipStream_t s);
 // Max resident workgroups per CU for a variant (0 if unsupported).
 int batched_occupancy(int mode, int dpt, int xc);
+// Vector interpreter (sr_vec.h), MODE_MSE and MODE_MSE_REF: a.spill must
+// hold kVecSpillBytesPerWave per launched wave.
+hipError_t launch_vec(int mode, const BatchArgs &a, int grid, hipStream_t s);
+int vec_occupancy(int mode);
+// 4 spilled stack slots x 384 datapoints x 4 bytes
+constexpr size_t kVecSpillBytesPerWave = (size_t)4 * 384 * 4;
+// Interpreter kinds of a launch plan.
+enum { KERNEL_LANE = 0, KERNEL_VEC = 1 };
 
 // Plain interpreter, one expression.  s_slots >= raw stack depth.
 hipError_t launch_plain(int s_slots, int xc, const Ins *prog, int len, const float *X,

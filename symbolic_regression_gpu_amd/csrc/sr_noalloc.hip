@@ -140,7 +140,7 @@ extern "C" int sr_eval_mse_noalloc(sr_ctx *x, int32_t num_exprs, int64_t num_tok
     if (num_exprs == 0) return SR_OK;
     hipSetDevice(c.device);
     LaunchPlan lp;
-    int rc = plan_launch(c, MODE_MSE, lp);
+    int rc = plan_launch(c, MODE_MSE, lp, false);  // the lane interpreter: no spill area
     if (rc) return rc;
     // Scratch layout for this call (every region 256-byte aligned):
     //   programs (num_tokens + 1 instructions, the last one LOAD NaN),

@@ -1,16 +1,19 @@
 // sr_tune.cpp -- autotuner for the launch shape of the batched interpreter
 // and of the constant-fitting kernels.
 //
-// The batched launch has these knobs (sr_tuning): datapoints per lane (DPT
-// 1..4: dispatch amortised over more datapoints vs. more registers per wave,
-// fewer waves per SIMD), datapoints per work item (chunk, multiples of 768:
-// LPT granularity vs. per-item program staging), XCD queues (1 or 8; tuned
-// runs keep 8) and persistent workgroups per CU (0 = occupancy limit).  The
-// tuner times every valid combination on a length-stratified sample of the
-// population (the interpreter kernel alone, HIP events), keeps the fastest,
-// and remembers it per (variable-register class, datapoint count), so later
-// calls with a similar shape reuse the result.  cache_dpt (skeleton launches
-// of the subtree caches) is left as set.
+// The batched launch has these knobs (sr_tuning): the interpreter (lane:
+// variables and stack in VGPRs, DPT 1..4 datapoints per lane -- dispatch
+// amortised over more datapoints vs. more registers per wave, fewer waves
+// per SIMD; or vector: 384-datapoint tiles, stack in LDS, any variable
+// count), datapoints per work item (chunk, multiples of 768: the number of
+// waves that share one expression, LPT granularity vs. per-item program
+// staging), XCD queues (1 or 8; tuned runs keep 8) and persistent workgroups
+// per CU (0 = occupancy limit).  The tuner times every valid combination on a
+// length-stratified sample of the population (the interpreter kernel alone,
+// HIP events), keeps the fastest, and remembers it per (variable-register
+// class, datapoint count), so later calls with a similar shape reuse the
+// result.  The skeleton launches of the subtree caches (cache_kernel,
+// cache_dpt) are left as set.
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -78,18 +81,19 @@ int sr_autotune(sr_ctx *x, const sr_population *pop, int32_t max_sample, sr_tuni
     const sr_tuning saved = c.tune;
     sr_tuning best = saved;
     double best_t = 1e300;
-    const int dpts[] = {1, 2, 3, 4};
     const int chunks[] = {3072, 6144, 12288, 24576};
     // one queue per XCD keeps each chunk's X in one L2; a small sample
     // cannot tell the queue layouts apart, so it is not a tuned knob
     const int queues[] = {8};
     const int bpcs[] = {0, 1};
-    for (int dpt : dpts) {
-        if ((xc == 32 && dpt > 2) || (dpt == 3 && xc != 4)) continue;
+    // dpt 0 stands for the vector interpreter
+    for (int dpt : {0, 1, 2, 3, 4}) {
+        if (dpt > 0 && ((xc == 32 && dpt > 2) || (dpt == 3 && xc != 4))) continue;
         for (int ch : chunks) {
             for (int nq : queues)
                 for (int bpc : bpcs) {
-                    c.tune = sr_tuning{dpt, ch, nq, bpc, saved.cache_dpt};
+                    c.tune = sr_tuning{dpt > 0 ? dpt : saved.dpt, ch, nq, bpc, saved.cache_dpt,
+                                       dpt > 0 ? KERNEL_LANE : KERNEL_VEC, saved.cache_kernel};
                     double t = 0.0;
                     for (int rep = 0; rep < 2; ++rep) {  // first run warms caches
                         sr_stats st{};

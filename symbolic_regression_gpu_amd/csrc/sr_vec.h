@@ -1,0 +1,225 @@
+// sr_vec.h -- the vector interpreter: one bytecode instruction at a time over
+// a 384-datapoint tile (6 datapoints per lane) per wave.
+//
+// The lane interpreter (k_batched, sr_kernels.h) holds the variables of its
+// tile and the stack in VGPRs, which caps it at 3 datapoints per lane; its
+// scalar decode (fetch, source / class / op branches) is then paid once per
+// 192 datapoints, and on short cache skeletons the per-tile loads of X|y and
+// columns, prefetched one tile ahead, cannot hide their latency.  Here a
+// wave decodes an instruction once per 384 datapoints, loads the
+// instruction's variable or column operand straight from memory (six
+// independent coalesced loads per lane, with the next instruction's operand
+// loaded while the current one computes), keeps the stack in LDS (slots
+// beyond kVecLdsSlots in a per-wave spill area of global memory) and fetches
+// instructions with scalar loads.  Nothing depends on the number of
+// variables, so one instantiation serves every nv.
+//
+// Work items, chunking, fused MSE partials and the per-lane summation order
+// (datapoint lane + 64 k, k ascending) are those of k_batched, so MSEs are
+// bit-identical to it for equal chunk and queue count.
+#pragma once
+
+#include "sr_device.h"
+#include "sr_kernels.h"
+
+namespace sr {
+
+constexpr int kVecD = 6;                 // datapoints per lane
+constexpr int kVecTile = kWave * kVecD;  // 384 datapoints per wave-instruction
+constexpr int kVecLdsSlots = 4;          // stack slots in LDS; deeper ones spill
+constexpr int kVecSpillSlots = 8 - kVecLdsSlots;
+
+using ConstIns = const __attribute__((address_space(4))) Ins;
+
+// Instruction i of a wave-uniform program through the constant address
+// space, so that it is fetched with a scalar load.
+SR_DEV Ins fetch_ins(ConstIns *p, int i)
+{
+    return Ins{p[i].code, p[i].arg};
+}
+
+template <typename F>
+SR_DEV void vec_each(F &&f)
+{
+#pragma unroll
+    for (int k = 0; k < kVecD; ++k) f(k);
+}
+
+SR_DEV void vec_unary(int op, float (&t)[kVecD])
+{
+    using namespace sr_ops;
+    switch (op) {
+    case OP_SIN: vec_each([&](int k) { t[k] = f_sin(t[k]); }); break;
+    case OP_COS: vec_each([&](int k) { t[k] = f_cos(t[k]); }); break;
+    case OP_TAN: vec_each([&](int k) { t[k] = f_tan(t[k]); }); break;
+    case OP_SINH: vec_each([&](int k) { t[k] = f_sinh(t[k]); }); break;
+    case OP_COSH: vec_each([&](int k) { t[k] = f_cosh(t[k]); }); break;
+    case OP_TANH: vec_each([&](int k) { t[k] = f_tanh(t[k]); }); break;
+    case OP_EXP: vec_each([&](int k) { t[k] = f_exp(t[k]); }); break;
+    case OP_LOG: vec_each([&](int k) { t[k] = f_log(t[k]); }); break;
+    case OP_INV: vec_each([&](int k) { t[k] = f_inv(t[k]); }); break;
+    case OP_ASIN: vec_each([&](int k) { t[k] = f_asin(t[k]); }); break;
+    case OP_ACOS: vec_each([&](int k) { t[k] = f_acos(t[k]); }); break;
+    case OP_ATAN: vec_each([&](int k) { t[k] = f_atan(t[k]); }); break;
+    case OP_LOOSE_LOG: vec_each([&](int k) { t[k] = f_loose_log(t[k]); }); break;
+    case OP_LOOSE_INV: vec_each([&](int k) { t[k] = f_loose_inv(t[k]); }); break;
+    case OP_ABS: vec_each([&](int k) { t[k] = f_abs(t[k]); }); break;
+    case OP_NEG: vec_each([&](int k) { t[k] = f_neg(t[k]); }); break;
+    case OP_SQRT: vec_each([&](int k) { t[k] = f_sqrt(t[k]); }); break;
+    case OP_LOOSE_SQRT: vec_each([&](int k) { t[k] = f_loose_sqrt(t[k]); }); break;
+    case OP_EXP_CLAMPED: vec_each([&](int k) { t[k] = f_exp_clamped(t[k]); }); break;
+    case OPC_STORE: break;
+    default: vec_each([&](int k) { t[k] = 0.0f; }); break;
+    }
+}
+
+SR_DEV void vec_binary(int op, float (&t)[kVecD], const float (&o)[kVecD])
+{
+    using namespace sr_ops;
+    switch (op) {
+    case OP_ADD: vec_each([&](int k) { t[k] = f_add(t[k], o[k]); }); break;
+    case OP_SUB: vec_each([&](int k) { t[k] = f_sub(t[k], o[k]); }); break;
+    case OP_MUL: vec_each([&](int k) { t[k] = f_mul(t[k], o[k]); }); break;
+    case OP_DIV: vec_each([&](int k) { t[k] = f_div(t[k], o[k]); }); break;
+    case OP_POW: vec_each([&](int k) { t[k] = f_pow(t[k], o[k]); }); break;
+    case OP_MIN: vec_each([&](int k) { t[k] = f_min(t[k], o[k]); }); break;
+    case OP_MAX: vec_each([&](int k) { t[k] = f_max(t[k], o[k]); }); break;
+    case OP_LOOSE_DIV: vec_each([&](int k) { t[k] = f_loose_div(t[k], o[k]); }); break;
+    case OP_LOOSE_POW: vec_each([&](int k) { t[k] = f_loose_pow(t[k], o[k]); }); break;
+    default: vec_each([&](int k) { t[k] = 0.0f; }); break;
+    }
+}
+
+// Same as vec_binary with the operands exchanged (top = f(o, top)).
+SR_DEV void vec_binary_swapped(int op, float (&t)[kVecD], const float (&o)[kVecD])
+{
+    float a[kVecD];
+    vec_each([&](int k) {
+        a[k] = o[k];
+    });
+    vec_binary(op, a, t);
+    vec_each([&](int k) { t[k] = a[k]; });
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
+{
+    __shared__ float s_stk[kBlock / kWave][kVecLdsSlots][kVecD][kWave];
+    const int w = threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    float(*stk)[kVecD][kWave] = s_stk[w];
+    float *spill = a.spill + ((int64_t)blockIdx.x * (kBlock / kWave) + w) * kVecSpillSlots * kVecTile;
+    const int q = a.n_queues > 1 ? (int)(blockIdx.x % (unsigned)a.n_queues) : 0;
+    const int n_exprs = a.n_exprs_dev ? min(*a.n_exprs_dev, a.n_exprs) : a.n_exprs;
+    const int items = n_exprs * a.chunks_per_queue;
+    const int64_t N = a.N;
+
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = (int)atomicAdd(&a.counters[q], 1u);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= items) break;
+        const int chunk = item / n_exprs;
+        const int rank = item - chunk * n_exprs;
+        const int e = a.order[rank];
+        ConstIns *prog =
+            (ConstIns *)(a.progs + (a.prog_off ? (int64_t)a.prog_off[e] : (int64_t)e * a.prog_stride));
+        const int n = a.prog_len[e];
+        const int64_t q0 = (int64_t)q * a.slice;
+        const int64_t q1 = q0 + a.slice < N ? q0 + a.slice : N;
+        const int64_t c0 = q0 + (int64_t)chunk * a.chunk;
+        const int64_t c1 = c0 + a.chunk < q1 ? c0 + a.chunk : q1;
+        double sum = 0.0;
+        long long cnt = 0;
+
+        for (int64_t t0 = c0; t0 < c1; t0 += kVecTile) {
+            const int last = (int)((c1 - t0 < kVecTile ? c1 - t0 : kVecTile) - 1);
+            const bool full = last == kVecTile - 1;
+            // six coalesced loads per lane from a uniform base (datapoint
+            // t0 + 64 k + lane), clamped to the tile's last datapoint
+            auto load = [&](const float *base, float(&dst)[kVecD]) {
+                if (full) {
+                    vec_each([&](int k) { dst[k] = base[k * kWave + lane]; });
+                } else {
+                    vec_each([&](int k) { dst[k] = base[min(k * kWave + lane, last)]; });
+                }
+            };
+            // operand of an instruction: its variable / column, a constant,
+            // or nothing (a popped operand is read when the instruction runs)
+            auto operand_ptr = [&](int code, int arg) -> const float * {
+                const int src = bc_src(code);
+                if (src == SRC_VAR) return a.X + (int64_t)arg * N + t0;
+                if (has_ref(MODE) && src == SRC_REF) return a.cache + (int64_t)arg * N + t0;
+                return nullptr;
+            };
+            float top[kVecD], o[kVecD], on[kVecD];
+            vec_each([&](int k) { top[k] = 0.0f; });
+            int sp = 0;
+            Ins cur = n > 0 ? fetch_ins(prog, 0) : Ins{0, 0};
+            int code = __builtin_amdgcn_readfirstlane(cur.code);
+            int arg = __builtin_amdgcn_readfirstlane(cur.arg);
+            if (const float *p = operand_ptr(code, arg)) load(p, on);
+            for (int i = 0; i < n; ++i) {
+                const Ins nx = i + 1 < n ? fetch_ins(prog, i + 1) : cur;
+                const int ncode = __builtin_amdgcn_readfirstlane(nx.code);
+                const int narg = __builtin_amdgcn_readfirstlane(nx.arg);
+                const int src = bc_src(code);
+                if (src == SRC_VAR || (has_ref(MODE) && src == SRC_REF)) vec_each([&](int k) { o[k] = on[k]; });
+                // the next instruction's operand is in flight while this one computes
+                if (i + 1 < n)
+                    if (const float *p = operand_ptr(ncode, narg)) load(p, on);
+                if (code & BC_PUSH) {
+                    if (sp < kVecLdsSlots) vec_each([&](int k) { stk[sp][k][lane] = top[k]; });
+                    else vec_each([&](int k) { spill[(sp - kVecLdsSlots) * kVecTile + k * kWave + lane] = top[k]; });
+                    ++sp;
+                }
+                if (src == SRC_POP) {
+                    --sp;
+                    if (sp < kVecLdsSlots) vec_each([&](int k) { o[k] = stk[sp][k][lane]; });
+                    else vec_each([&](int k) { o[k] = spill[(sp - kVecLdsSlots) * kVecTile + k * kWave + lane]; });
+                } else if (src == SRC_CONST) {
+                    const float cv = __int_as_float(arg);
+                    vec_each([&](int k) { o[k] = cv; });
+                }
+                const int cls = bc_cls(code);
+                if (cls != CL_BIN) {
+                    if (src != SRC_NONE) vec_each([&](int k) { top[k] = o[k]; });
+                    if (cls == CL_UN) vec_unary(bc_op(code), top);
+                } else if (code & BC_SWAP) {
+                    vec_binary_swapped(bc_op(code), top, o);
+                } else {
+                    vec_binary(bc_op(code), top, o);
+                }
+                if (has_ref(MODE) && code == STORE_CODE) {
+                    float *cs = a.out + (int64_t)arg * N + t0;
+                    if (full) vec_each([&](int k) { cs[k * kWave + lane] = top[k]; });
+                    else vec_each([&](int k) {
+                            const int j = k * kWave + lane;
+                            if (j <= last) cs[j] = top[k];
+                        });
+                }
+                code = ncode;
+                arg = narg;
+                cur = nx;
+            }
+            float y[kVecD];
+            load(a.X + (int64_t)a.nv * N + t0, y);
+            vec_each([&](int k) {
+                const bool ok = k * kWave + lane <= last && isfinite(top[k]);
+                const float diff = top[k] - y[k];
+                const float sq = diff * diff;
+                sum += ok ? (double)sq : 0.0;
+                cnt += ok ? 1 : 0;
+            });
+        }
+        sum = wave_sum(sum);
+        cnt = wave_sum(cnt);
+        if (lane == 0) {
+            const int slot = q * a.chunks_per_queue + chunk;
+            a.psum[(int64_t)e * a.P + slot] = sum;
+            a.pcnt[(int64_t)e * a.P + slot] = cnt;
+        }
+    }
+}
+
+}  // namespace sr

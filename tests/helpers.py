@@ -141,10 +141,12 @@ def check_population(pop, X, nv, N, m_gpu, v_gpu, pred_gpu=None, rtol=1e-5, floo
     """Conditioning-aware parity of a GPU evaluation (MSE, valid counts and
     optionally predictions [E, N]) with the oracle.
 
-    Every expression: the GPU MSE is NaN exactly where the oracle's is; the
-    valid count equals the oracle's wherever the expression's FP64 error bound
-    is finite (no point can change class); the MSE lies within twice its
-    bound of the FP64 MSE.  Expressions whose relative MSE error bound is
+    Every expression: the valid count differs from the oracle's by at most
+    the number of points whose FP64 error bound is infinite (points that can
+    change class within rounding), so it is equal where there are none; the
+    MSE is NaN exactly where the oracle's is unless no valid point is pinned
+    that way (an MSE is NaN only when its valid count is 0); the MSE lies
+    within twice its bound of the FP64 MSE.  Expressions whose relative MSE error bound is
     <= 1e-6 must match the oracle's MSE within rtol.  Predictions must pass
     point_parity.
 
@@ -155,7 +157,6 @@ def check_population(pop, X, nv, N, m_gpu, v_gpu, pred_gpu=None, rtol=1e-5, floo
     the FP64 evaluator are printed.  Returns those statistics (dict)."""
     m_gpu = np.asarray(m_gpu, np.float32)
     m_ref, v_ref = O.mse32(pop, X, nv, N, threads=8)
-    assert np.array_equal(np.isnan(m_gpu), np.isnan(m_ref)), np.flatnonzero(np.isnan(m_gpu) != np.isnan(m_ref))
     y = X[nv]
     strict = 0
     pts_ok = pts_n = 0
@@ -175,6 +176,9 @@ def check_population(pop, X, nv, N, m_gpu, v_gpu, pred_gpu=None, rtol=1e-5, floo
                 near = fin & (np.abs(g.astype(np.float64) - c32) <= 4 * ulp(c32))
             pts_ok += int((same_class | near).sum())
             pts_n += N
+        unstable = int((~np.isfinite(b64)).sum())
+        assert abs(int(v_gpu[e]) - int(v_ref[e])) <= unstable, (e, v_gpu[e], v_ref[e], unstable)
+        assert np.isnan(m_gpu[e]) == np.isnan(m_ref[e]) or unstable > 0, (e, m_gpu[e], m_ref[e])
         if np.isfinite(rb):
             assert v_gpu[e] == v_ref[e], (e, v_gpu[e], v_ref[e])
         if rb <= 1e-6:

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from symbolic_regression_gpu_amd import EVAL_BATCHED, EVAL_CACHED
-from symbolic_regression_gpu_amd.api import CACHE_DEVICE, CACHE_REFERENCE, GaParams, SelectParams
+from symbolic_regression_gpu_amd.api import CACHE_DEVICE, CACHE_REFERENCE, GaParams, SelectParams, Tuning
 from tests.helpers import GRAMMAR_BINARY, GRAMMAR_UNARY, uniform_data
 
 pytestmark = pytest.mark.gpu
@@ -43,9 +43,15 @@ def _same(h1, h2):
     return np.array_equal(np.asarray(h1).view(np.uint32), np.asarray(h2).view(np.uint32))
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_device_cache_equals_batched_over_generations(ctx, use_graph):
-    hb, sb, eb, hc, sc, ec = _runs(ctx, 3, 400, 8, 11, 0, 3, use_graph)
+@pytest.mark.parametrize("use_graph,cache_kernel", [(True, 0), (False, 0), (True, 1), (False, 1)])
+def test_device_cache_equals_batched_over_generations(ctx, use_graph, cache_kernel):
+    saved = ctx.tuning
+    try:
+        ctx.tuning = Tuning(saved.dpt, saved.chunk, saved.n_queues, saved.blocks_per_cu, saved.cache_dpt,
+                            saved.kernel, cache_kernel)
+        hb, sb, eb, hc, sc, ec = _runs(ctx, 3, 400, 8, 11, 0, 3, use_graph)
+    finally:
+        ctx.tuning = saved
     assert _same(hb, hc)
     assert np.array_equal(eb[2], ec[2])
     ex = sc["executed_node_evals"] / sc["logical_node_evals"]

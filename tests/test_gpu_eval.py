@@ -87,10 +87,12 @@ def test_batched_tunings_agree(ctx):
     saved = ctx.tuning
     by_shape = {}
     try:
-        for dpt in (1, 2, 3, 4):
+        # dpt 0: the vector interpreter
+        for dpt in (0, 1, 2, 3, 4):
             for nq in (1, 8):
                 for chunk in (512, 16384):
-                    ctx.tuning = Tuning(dpt=dpt, chunk=chunk, n_queues=nq, blocks_per_cu=0)
+                    ctx.tuning = Tuning(dpt=max(dpt, 1), chunk=chunk, n_queues=nq, blocks_per_cu=0,
+                                        kernel=int(dpt == 0))
                     m, v, _ = ctx.eval_mse(api, EVAL_BATCHED)
                     # same per-point values; only the fp64 partial-sum order moves
                     assert np.array_equal(v, vbase), (dpt, nq, chunk)
@@ -173,3 +175,42 @@ def test_autotune_picks_a_valid_shape_and_keeps_results(ctx):
         print(f"autotune: {best} ({ms:.3f} ms on the sample)")
     finally:
         ctx.tuning = saved
+
+
+def _deep(k, ops=(1, 2, 3, 4), var=0):
+    """A tree whose bytecode needs k + 1 stack slots: op(T(k-1), T(k-1))."""
+    if k == 0:
+        return [ops[0], -1, -1], [0.0, float(var), float(var + 1)]
+    lt, lv = _deep(k - 1, ops[1:] + ops[:1], var)
+    rt, rv = _deep(k - 1, ops[2:] + ops[:2], var + 1)
+    return [ops[k % len(ops)]] + lt + rt, [0.0] + lv + rv
+
+
+@pytest.mark.parametrize("nv,N", [(1, 20_003), (3, 100_000), (4, 4097), (10, 65_536), (32, 1000)])
+def test_vector_interpreter_equals_lane(ctx, nv, N):
+    """The vector interpreter (384 datapoints per instruction, stack in LDS
+    with deeper slots spilled) gives the lane interpreter's MSEs bit for bit:
+    same op functions, same operand order, same per-lane summation order."""
+    pop = random_population(150, 700 + nv, nv=nv, max_tokens=60, max_depth=7, unary=ALL_UNARY,
+                            binary=ALL_BINARY, const_min=-3, const_max=3)
+    exprs = [pop.expr(e) for e in range(pop.E)]
+    for k in (3, 5, 6, 7):  # stack needs 4, 6, 7, 8: slots 4..7 live in the spill area
+        t, v = _deep(k, var=0)
+        exprs.append((np.array(t, np.int32), np.array([x % max(nv, 1) if tt == -1 else x
+                                                      for tt, x in zip(t, v)], np.float64)))
+    pop = O.Population(exprs)
+    X = uniform_data(nv, N, nv + 3)
+    X[nv] = np.cos(X[0]) + 0.5
+    ctx.set_data(X)
+    api = to_api(pop)
+    saved = ctx.tuning
+    try:
+        ctx.tuning = Tuning(dpt=3, chunk=3072, n_queues=8, blocks_per_cu=0, kernel=0)
+        m_lane, v_lane, _ = ctx.eval_mse(api, EVAL_BATCHED)
+        ctx.tuning = Tuning(dpt=3, chunk=3072, n_queues=8, blocks_per_cu=0, kernel=1)
+        m_vec, v_vec, _ = ctx.eval_mse(api, EVAL_BATCHED)
+    finally:
+        ctx.tuning = saved
+    assert np.array_equal(m_vec.view(np.uint32), m_lane.view(np.uint32))
+    assert np.array_equal(v_vec, v_lane)
+    check_population(pop, X, nv, N, m_vec, v_vec, name=f"vector nv={nv}")
