@@ -17,9 +17,20 @@ also run with the uncached batched evaluation replayed as one hipGraph per
 generation and with the host-planned cache (their fitness vectors must be
 identical), and the plain one-launch-per-expression interpreter is timed on
 the initial and the final population.  Beside them:
-the reference-policy subtree cache, and the constant-fitting workload (4096
-expressions of 50-200 tokens, 2-8 free constants, 5 variables, 256k
-datapoints): adjoint vs central-difference gradient evaluations/s.
+  - single_expression: one tree (depth <= 6, <= 31 tokens, 3 variables) over
+    100k datapoints: per-point values and MSE, plain and batched;
+  - multi_expression: 100 trees of 5-60 tokens over 10 variables and 1M
+    datapoints: plain vs batched, and the hipRTC module's compile latency
+    (its device execution is switched off);
+  - mutant_cache: a 36-45 token base tree and 100 mutants (one depth 1-3
+    subtree replaced each), 1M datapoints: the reference-policy subtree cache
+    (hit rate, FLOPs saved, node-evals/s) against the uncached evaluation;
+  - constant_fit: 4096 expressions of 50-200 tokens over 3..8 of 8 variables,
+    2-8 free constants, 256k datapoints: 20 Adam steps with the adjoint
+    kernel vs central-difference gradient evaluations/s;
+  - fp64_error (with the CPU leg): the GPU results against the oracle's FP64
+    evaluator -- per-point ulps of the single expression, relative MSE errors
+    of the batches on their first 65536 datapoints.
 
 roofline: the skeleton interpreter kernel of the timed generations (HIP events
 around each launch on the context stream), counted in FP32 operator
@@ -123,11 +134,31 @@ def time_evals(ctx, pop, mode, reps):
             "executed_node_evals": executed // reps, "logical_node_evals": logical // reps}
 
 
+def cpu_cores():
+    """Threads for the CPU legs: the cores this process may run on, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(X, nv, pop_tok, pop_val, pop_off, pop_len, target_s=12.0):
     """Oracle evaluate_cpu_mse restatement on a sample of the population: all
     expressions x the first n datapoints, n sized for ~target_s seconds."""
     from oracle import oracle as O
-    cores = max(1, min(16, os.cpu_count() or 1))
+    cores = cpu_cores()
     N = X.shape[1]
     exprs = [(pop_tok[o:o + l], pop_val[o:o + l]) for o, l in zip(pop_off, pop_len)]
     pop = O.Population(exprs)
@@ -144,14 +175,15 @@ def cpu_baseline(X, nv, pop_tok, pop_val, pop_off, pop_len, target_s=12.0):
     n = int(min(N, max(2048, n * target_s / max(dt, 1e-3))))
     dt = run(n)
     rate = tokens * n / dt
-    return {"value": rate, "unit": "node-evals/s", "cores": cores, "kind": "port",
+    return {"value": rate, "unit": "node-evals/s", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
             "sample": f"all {len(exprs)} expressions ({tokens} tokens) x the first {n} of {N} "
                       f"datapoints, {dt:.1f} s, fp64-accumulated MSE as cl_cpu_eval.cpp:94-168"}
 
 
-def constant_fit_bench(ctx, seed, adj_steps, fd_steps, E=4096, N=262144, nv=5):
+def constant_fit_bench(ctx, seed, adj_steps, fd_steps, E=4096, N=262144, nv=8):
     """Adjoint vs central-difference gradient evaluations/s (one grad-eval =
-    dMSE/d(all free constants) of one expression over N points)."""
+    dMSE/d(all free constants) of one expression over N points).  The
+    expressions use 3..8 of the nv = 8 variables (one sixth each)."""
     import torch
     from symbolic_regression_gpu_amd import Population
     from symbolic_regression_gpu_amd.api import GaParams
@@ -164,17 +196,19 @@ def constant_fit_bench(ctx, seed, adj_steps, fd_steps, E=4096, N=262144, nv=5):
     X[nv] = np.nan_to_num(y, nan=0.0, posinf=0.0, neginf=0.0) + \
         rng.normal(0.0, 0.01, size=N).astype(np.float32)
     ctx.set_data(X)
-    params = GaParams.make(num_vars=nv, max_tokens=200, max_depth=10, const_min=-2.0, const_max=2.0,
-                           unary_ops=GRAMMAR_UNARY, binary_ops=GRAMMAR_BINARY)
     exprs, s = [], seed + 4
-    while len(exprs) < E:
-        ctx.ga_init(params, 8192, s)
-        tok, val, ln, cap = ctx.ga_population()
-        for i in range(8192):
-            if ln[i] >= 50 and len(exprs) < E:
-                exprs.append((tok[i * cap:i * cap + ln[i]].copy(),
-                              val[i * cap:i * cap + ln[i]].astype(np.float64)))
-        s += 1
+    for k, nv_e in enumerate(range(3, 9)):
+        params = GaParams.make(num_vars=nv_e, max_tokens=200, max_depth=10, const_min=-2.0, const_max=2.0,
+                               unary_ops=GRAMMAR_UNARY, binary_ops=GRAMMAR_BINARY)
+        want = E * (k + 1) // 6
+        while len(exprs) < want:
+            ctx.ga_init(params, 8192, s)
+            tok, val, ln, cap = ctx.ga_population()
+            for i in range(8192):
+                if ln[i] >= 50 and len(exprs) < want:
+                    exprs.append((tok[i * cap:i * cap + ln[i]].copy(),
+                                  val[i * cap:i * cap + ln[i]].astype(np.float64)))
+            s += 1
     slot_rows, vals = [], []
     for t, v in exprs:
         sl = np.full(len(t), -1, np.int32)
@@ -215,12 +249,198 @@ def constant_fit_bench(ctx, seed, adj_steps, fd_steps, E=4096, N=262144, nv=5):
     m = m.detach()
     fin = torch.isfinite(m0) & torch.isfinite(m)
     return {"workload": f"{E} expressions of 50-200 tokens ({fit.tokens} tokens), 2-8 free "
-                        f"constants, {nv} vars, {N} datapoints; Adam lr 1e-2",
+                        f"constants, 3-8 of {nv} vars, {N} datapoints; Adam lr 1e-2",
             "adjoint_grad_evals_per_s": E / t_adj, "fd_grad_evals_per_s": E / t_fd,
             "adjoint_over_fd": t_fd / t_adj, "ms_per_adam_step": 1e3 * t_adj,
             "ms_per_fd_gradient": 1e3 * t_fd, "adam_steps_timed": adj_steps,
             "median_mse_first_last": [float(m0[fin].median()), float(m[fin].median())],
             "tuning": {"forward_dpt": fwd_dpt, "adjoint_dpt": adj_dpt}}
+
+
+def random_exprs(ctx, nv, n, min_len, max_len, depth, seed, pool=4096):
+    """n seeded random trees of the grammar from the device generator, with
+    min_len <= length <= max_len (tokens, values as float64)."""
+    from symbolic_regression_gpu_amd.api import GaParams
+    p = GaParams.make(num_vars=nv, max_tokens=max_len, max_depth=depth, const_min=-2.0, const_max=2.0,
+                      unary_ops=GRAMMAR_UNARY, binary_ops=GRAMMAR_BINARY)
+    out, s = [], seed
+    while len(out) < n:
+        ctx.ga_init(p, pool, s)
+        tok, val, ln, cap = ctx.ga_population()
+        for i in range(pool):
+            if min_len <= ln[i] <= max_len and len(out) < n:
+                out.append((tok[i * cap:i * cap + ln[i]].copy(), val[i * cap:i * cap + ln[i]].astype(np.float64)))
+        s += 1
+    return out
+
+
+def subtree_sizes_depths(tok):
+    """Size (tokens) and depth of the subtree rooted at every prefix position."""
+    L = len(tok)
+    size = np.zeros(L, np.int64)
+    depth = np.zeros(L, np.int64)
+    for i in range(L - 1, -1, -1):
+        t = int(tok[i])
+        s, d, pos = 1, 1, i + 1
+        for _ in range(0 if t <= 0 else (2 if t < 10 else 1)):
+            s += size[pos]
+            d = max(d, 1 + depth[pos])
+            pos += size[pos]
+        size[i], depth[i] = s, d
+    return size, depth
+
+
+def timed(fn, reps):
+    """Mean wall ms of fn() over reps calls after one untimed call, and its last result."""
+    fn()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r = fn()
+    return 1e3 * (time.perf_counter() - t0) / reps, r
+
+
+def single_expression_bench(ctx, seed, reps=50):
+    """One seeded tree (depth <= 6, <= 31 tokens) over 3 variables ~ U[1,5] and
+    100k datapoints, target a second tree of the same grammar: per-point
+    values and MSE on the GPU (plain and batched interpreters)."""
+    from symbolic_regression_gpu_amd import EVAL_BATCHED, EVAL_PLAIN, Population
+    nv, N = 3, 100_000
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(1.0, 5.0, size=(nv + 1, N)).astype(np.float32)
+    ctx.set_data(X)
+    model, target = random_exprs(ctx, nv, 2, 15, 31, 6, seed)
+    X[nv] = np.nan_to_num(ctx.eval_predictions(Population.from_exprs([target]))[0], nan=0.0, posinf=0.0,
+                          neginf=0.0)
+    ctx.set_data(X)
+    pop = Population.from_exprs([model])
+    ms_pred, pred = timed(lambda: ctx.eval_predictions(pop), reps)
+    res = {"workload": f"one tree of {len(model[0])} tokens (depth <= 6), 3 vars, {N} datapoints",
+           "predictions_ms": ms_pred}
+    for name, mode in (("plain", EVAL_PLAIN), ("batched", EVAL_BATCHED)):
+        ms, (m, v, st) = timed(lambda: ctx.eval_mse(pop, mode), reps)
+        res[name] = {"ms_per_eval": ms, "node_evals_per_s": len(model[0]) * N / (ms * 1e-3),
+                     "interp_kernel_ms": st["interp_kernel_ms"], "mse": float(m[0]), "valid": int(v[0])}
+    return res, (X, nv, [model], pred[0])
+
+
+def multi_expression_bench(ctx, seed, reps=5):
+    """100 seeded trees of 5-60 tokens over 10 variables ~ U[1,5], 1M
+    datapoints: the plain one-launch-per-expression interpreter, the batched
+    interpreter, and the hipRTC backend's compile latency for the same batch."""
+    from symbolic_regression_gpu_amd import EVAL_BATCHED, EVAL_PLAIN, Population
+    from symbolic_regression_gpu_amd import api
+    nv, N = 10, 1 << 20
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(1.0, 5.0, size=(nv + 1, N)).astype(np.float32)
+    exprs = random_exprs(ctx, nv, 101, 5, 60, 7, seed)
+    ctx.set_data(X)
+    X[nv] = np.nan_to_num(ctx.eval_predictions(Population.from_exprs([exprs[100]]))[0], nan=0.0,
+                          posinf=0.0, neginf=0.0)
+    ctx.set_data(X)
+    exprs = exprs[:100]
+    pop = Population.from_exprs(exprs)
+    res = {"workload": f"100 trees of 5-60 tokens ({pop.total_tokens} tokens), 10 vars, {N} datapoints"}
+    for name, mode in (("plain", EVAL_PLAIN), ("batched", EVAL_BATCHED)):
+        res[name] = time_evals(ctx, pop, mode, reps)
+    res["batched_over_plain"] = res["batched"]["node_evals_per_s"] / res["plain"]["node_evals_per_s"]
+    t0 = time.perf_counter()
+    compile_ms, code_bytes = api.jit_compile(pop, nv, 4)
+    res["jit"] = {"compile_ms": compile_ms, "compile_wall_ms": 1e3 * (time.perf_counter() - t0),
+                  "code_bytes": code_bytes,
+                  "eval_ms": None,
+                  "note": "hipRTC module compiled on the host; its device execution is switched off "
+                          "(the first GPU run stopped making progress; cause not identified)"}
+    return res, (X, nv, exprs)
+
+
+def mutant_cache_bench(ctx, seed, reps=5):
+    """One seeded base tree (36-45 tokens, 5 variables) and 100 mutants, each
+    with one subtree of depth 1-3 replaced by a fresh random subtree of depth
+    1-3; 1M datapoints ~ U[1,5]: the reference-policy subtree cache (detect,
+    cache shared subtrees, skeletons) against the uncached batched evaluation."""
+    from symbolic_regression_gpu_amd import EVAL_BATCHED, EVAL_CACHED, Population
+    nv, N = 5, 1 << 20
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(1.0, 5.0, size=(nv + 1, N)).astype(np.float32)
+    base, target = random_exprs(ctx, nv, 2, 36, 45, 7, seed)
+    fresh = random_exprs(ctx, nv, 100, 2, 15, 3, seed + 1)
+    bt, bv = base
+    size, depth = subtree_sizes_depths(bt)
+    cand = np.flatnonzero((depth >= 1) & (depth <= 3) & (np.arange(len(bt)) > 0))
+    exprs = [base]
+    for k in range(100):
+        i = int(rng.choice(cand))
+        ft, fv = fresh[k]
+        exprs.append((np.concatenate([bt[:i], ft, bt[i + size[i]:]]),
+                      np.concatenate([bv[:i], fv, bv[i + size[i]:]])))
+    ctx.set_data(X)
+    X[nv] = np.nan_to_num(ctx.eval_predictions(Population.from_exprs([target]))[0], nan=0.0, posinf=0.0,
+                          neginf=0.0)
+    ctx.set_data(X)
+    pop = Population.from_exprs(exprs)
+    logical = pop.total_tokens * N
+    unc = time_evals(ctx, pop, EVAL_BATCHED, reps)
+    m_b, _, _ = ctx.eval_mse(pop, EVAL_BATCHED)
+    ctx.cache_config(2048, 3, 2)
+    ctx.cache_clear()
+    t0 = time.perf_counter()
+    m_c, _, st1 = ctx.eval_mse(pop, EVAL_CACHED)
+    first_ms = 1e3 * (time.perf_counter() - t0)
+    steady = time_evals(ctx, pop, EVAL_CACHED, reps)
+    ctx.cache_clear()
+    return {"workload": f"base tree of {len(bt)} tokens + 100 mutants ({pop.total_tokens} tokens), 5 vars, "
+                        f"{N} datapoints; reference-policy cache, min size 3, min frequency 2",
+            "uncached_batched": unc,
+            "cached_first_call": {"ms": first_ms, "subtrees_detected": st1["num_subtrees"],
+                                  "subtrees_cached": st1["num_cached"], "coverage": st1["coverage"],
+                                  "executed_over_logical": st1["executed_node_evals"] / logical},
+            "cached_steady": steady,
+            "cache_hit_rate": st1["coverage"],
+            "flops_saved": 1.0 - steady["executed_node_evals"] / logical,
+            "cached_over_uncached": steady["node_evals_per_s"] / unc["node_evals_per_s"],
+            "mse_identical_to_uncached": bool(np.array_equal(m_b.view(np.uint32), m_c.view(np.uint32)))}, \
+        (X, nv, exprs)
+
+
+def fp64_accuracy(ctx, cores, c1, c2, c3, evo, n_sample=1 << 16):
+    """Error of the GPU results against the oracle's FP64 evaluator, the
+    checker (sro_predict64 / sro_mse64 restating cl_cpu_eval.cpp in double):
+    per-point values of the single expression (all its datapoints) in fp32
+    ulps of the FP64 value, and relative MSE errors of the batches on their
+    first n_sample datapoints (the GPU evaluates the same sample)."""
+    from oracle import oracle as O
+    from symbolic_regression_gpu_amd import EVAL_BATCHED, Population
+    out = {}
+    X, nv, exprs, pred = c1
+    t, v = exprs[0]
+    N = X.shape[1]
+    v64 = O.predict64(t, v, X, nv, N)
+    c32 = O.predict32(t, v, X, nv, N)
+    g = np.asarray(pred, np.float64)
+    fin = np.isfinite(v64) & np.isfinite(g)
+    with np.errstate(invalid="ignore", over="ignore"):
+        ulps = np.abs(g[fin] - v64[fin]) / np.spacing(np.abs(v64[fin]).astype(np.float32)).astype(np.float64)
+        near32 = np.abs(g - c32) <= 4 * np.spacing(np.abs(c32).astype(np.float32)).astype(np.float64)
+    same_class = (np.isnan(g) == np.isnan(c32)) & (np.isinf(g) == np.isinf(c32))
+    out["single_expression_points"] = {
+        "points": int(N), "max_ulp_vs_fp64": float(ulps.max()) if ulps.size else 0.0,
+        "median_ulp_vs_fp64": float(np.median(ulps)) if ulps.size else 0.0,
+        "share_within_4ulp_of_oracle_fp32": float(np.mean(same_class & (near32 | ~np.isfinite(c32))))}
+    for name, (Xb, nvb, eb) in (("multi_expression", c2), ("mutant_cache", c3), ("evolution_final_population", evo)):
+        n = min(n_sample, Xb.shape[1])
+        Xs = np.ascontiguousarray(Xb[:, :n])
+        ctx.set_data(Xs)
+        m_gpu, _, _ = ctx.eval_mse(Population.from_exprs(eb), EVAL_BATCHED)
+        m64 = O.mse64(O.Population(eb), Xs, nvb, n, threads=cores)
+        m_gpu = np.asarray(m_gpu, np.float64)
+        both = np.isfinite(m_gpu) & np.isfinite(m64) & (m64 > 0)
+        rel = np.abs(m_gpu[both] - m64[both]) / m64[both]
+        out[name] = {"exprs": len(eb), "datapoints": n, "finite": int(both.sum()),
+                     "max_rel_mse_err": float(rel.max()) if rel.size else 0.0,
+                     "median_rel_mse_err": float(np.median(rel)) if rel.size else 0.0,
+                     "share_within_1e-5": float(np.mean(rel <= 1e-5)) if rel.size else 1.0,
+                     "nan_class_agrees": float(np.mean(np.isnan(m_gpu) == np.isnan(m64)))}
+    return out
 
 
 def copy_calibration(mib):
@@ -243,7 +463,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--N", type=int, default=1 << 20)
     ap.add_argument("--reps", type=int, default=5, help="timed batched/plain evaluations")
-    ap.add_argument("--adj-steps", type=int, default=5)
+    ap.add_argument("--adj-steps", type=int, default=20)
     ap.add_argument("--fd-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fit", action="store_true")
@@ -367,16 +587,26 @@ def main():
     log("[bench] plain and batched evaluation of the final population")
     plainK = time_evals(ctx, popK, EVAL_PLAIN, 2)
     batchedK = time_evals(ctx, popK, EVAL_BATCHED, 3)
+    evo = (X, nv, [popK.expr(e) for e in range(popK.num_exprs)])
+
+    log("[bench] single expression, 100k datapoints")
+    c1, c1_data = single_expression_bench(ctx, args.seed + 20)
+    log("[bench] 100 expressions over 10 variables, 1M datapoints: plain, batched, hipRTC compile")
+    c2, c2_data = multi_expression_bench(ctx, args.seed + 30)
+    log("[bench] base tree + 100 mutants, 1M datapoints: subtree cache vs uncached")
+    c3, c3_data = mutant_cache_bench(ctx, args.seed + 40)
 
     fit = None
     if not args.no_fit:
         log("[bench] constant fitting: adjoint vs finite differences")
         fit = constant_fit_bench(ctx, args.seed + 10, args.adj_steps, args.fd_steps)
 
-    cpu = None
+    cpu = acc = None
     if not args.no_cpu:
         log("[bench] CPU baseline")
         cpu = cpu_baseline(X, nv, pop0.tokens, pop0.values, pop0.offsets, pop0.lengths)
+        log("[bench] error against the FP64 evaluator")
+        acc = fp64_accuracy(ctx, cpu["cores"], c1_data, c2_data, c3_data, evo)
 
     # roofline of the dominant kernel of the timed steps: the skeleton
     # interpreter (MODE_MSE_REF), over every launch of the timed
@@ -491,7 +721,11 @@ def main():
         "batched_initial_population": batched0,
         "batched_over_plain_initial": batched0["node_evals_per_s"] / plain0["node_evals_per_s"],
         "reference_cache_initial_population": ref_cache,
+        "single_expression": c1,
+        "multi_expression": c2,
+        "mutant_cache": c3,
         "constant_fit": fit,
+        "fp64_error": acc,
     }
     print(json.dumps(out), flush=True)
 

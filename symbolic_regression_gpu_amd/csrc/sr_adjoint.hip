@@ -421,7 +421,8 @@ hipError_t launch_param_fwd(int dpt, int xc, const ParamArgs &a, int grid, hipSt
         hipLaunchKernelGGL((k_param_fwd<D, XCV>), dim3(grid), dim3(kBlock), 0, s, a);            \
         return hipGetLastError();                                                                \
     }
-    SR_PF(1, 4) SR_PF(2, 4) SR_PF(4, 4) SR_PF(1, 16) SR_PF(2, 16) SR_PF(4, 16) SR_PF(1, 32)
+    SR_PF(1, 4) SR_PF(2, 4) SR_PF(4, 4) SR_PF(1, 8) SR_PF(2, 8) SR_PF(4, 8) SR_PF(1, 16) SR_PF(2, 16)
+    SR_PF(4, 16) SR_PF(1, 32)
 #undef SR_PF
     return hipErrorInvalidValue;
 }
@@ -433,7 +434,8 @@ int param_fwd_occupancy(int dpt, int xc)
     if (dpt == D && xc == XCV)                                                                 \
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void *)k_param_fwd<D, XCV>, \
                                                             kBlock, 0) == hipSuccess ? b : 0;
-    SR_PF(1, 4) SR_PF(2, 4) SR_PF(4, 4) SR_PF(1, 16) SR_PF(2, 16) SR_PF(4, 16) SR_PF(1, 32)
+    SR_PF(1, 4) SR_PF(2, 4) SR_PF(4, 4) SR_PF(1, 8) SR_PF(2, 8) SR_PF(4, 8) SR_PF(1, 16) SR_PF(2, 16)
+    SR_PF(4, 16) SR_PF(1, 32)
 #undef SR_PF
     return 0;
 }
@@ -445,7 +447,7 @@ hipError_t launch_adjoint(int dpt, int xc, const ParamArgs &a, int grid, hipStre
         hipLaunchKernelGGL((k_adjoint<D, XCV>), dim3(grid), dim3(kBlock), 0, s, a);       \
         return hipGetLastError();                                                        \
     }
-    SR_AD(1, 4) SR_AD(2, 4) SR_AD(1, 16) SR_AD(2, 16) SR_AD(1, 32)
+    SR_AD(1, 4) SR_AD(2, 4) SR_AD(1, 8) SR_AD(2, 8) SR_AD(1, 16) SR_AD(2, 16) SR_AD(1, 32)
 #undef SR_AD
     return hipErrorInvalidValue;
 }
@@ -457,7 +459,7 @@ int adjoint_occupancy(int dpt, int xc)
     if (dpt == D && xc == XCV)                                                                    \
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void *)k_adjoint<D, XCV>,  \
                                                             kBlock, 0) == hipSuccess ? b : 0;
-    SR_AD(1, 4) SR_AD(2, 4) SR_AD(1, 16) SR_AD(2, 16) SR_AD(1, 32)
+    SR_AD(1, 4) SR_AD(2, 4) SR_AD(1, 8) SR_AD(2, 8) SR_AD(1, 16) SR_AD(2, 16) SR_AD(1, 32)
 #undef SR_AD
     return 0;
 }

@@ -25,7 +25,9 @@ struct ParamPlan {
 
 static int param_plan(Ctx &c, bool adjoint, ParamPlan &pp, int force_dpt = 0)
 {
-    pp.xc = pick_xc(c.nv);
+    // variable classes of the constant-fitting kernels: <= 4, <= 8 (nine
+    // registers, GPR-indexed), <= 16, <= 32
+    pp.xc = c.nv <= 8 && c.nv > 4 ? 8 : pick_xc(c.nv);
     const int max_dpt = adjoint ? (pp.xc == 32 ? 1 : 2) : (pp.xc == 32 ? 1 : 4);
     const int want = force_dpt ? force_dpt : (adjoint ? c.par.adj_dpt : c.par.fwd_dpt);
     if (force_dpt && force_dpt > max_dpt) return SR_ERR_UNSUPPORTED;

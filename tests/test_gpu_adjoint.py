@@ -101,6 +101,27 @@ def test_adjoint_matches_fp64_gradient_and_fd(ctx, min_len, max_tokens):
     assert covered >= 8 and fd_checked >= 10
 
 
+@pytest.mark.parametrize("nv", [6, 8, 12, 20])
+def test_adjoint_variable_classes(ctx, nv):
+    """Each variable-register class of the constant-fitting kernels (<= 8:
+    nine GPR-indexed registers, <= 16, <= 32): gradients within the FP64
+    bound, forward MSEs (adjoint pass and forward-only pass) equal the
+    batched evaluator's."""
+    N = 4096
+    pop, slot = c5_population(32, 40 + nv, nv, min_len=20, max_tokens=80)
+    X = uniform_data(nv, N, 50 + nv)
+    X[nv] = np.sin(X[0]) * X[nv - 1] + 0.5 * X[nv // 2]
+    covered, mse, g, c0, fit = _adjoint_vs_bound(ctx, pop, slot, X, nv)
+    assert covered >= pop.E // 4, covered
+    m_b, _, _ = ctx.eval_mse(to_api(pop), EVAL_BATCHED)
+    fin = np.isfinite(m_b)
+    assert np.array_equal(fin, np.isfinite(mse))
+    assert np.allclose(mse[fin], m_b[fin], rtol=1e-6)
+    m_f = fit.mse(torch.as_tensor(c0, device="cuda")).detach().cpu().numpy()
+    assert np.array_equal(np.isfinite(m_f), fin)
+    assert np.allclose(m_f[fin], m_b[fin], rtol=1e-6)
+
+
 def test_every_opcode_partials_on_gpu(ctx):
     from oracle.oracle import Population as OPop
     ops = sorted(UNARY_DOMAIN) + sorted(BINARY_DOMAIN)
