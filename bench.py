@@ -537,6 +537,9 @@ def main():
     ctx.ga_set_population(tok, val, ln, cap)
     log(f"[bench] cached generations (device-planned cache, hipGraph): {W} warm-up + {K} timed")
     ctx.ga_run(sel, EVAL_CACHED, W, 0, args.seed, True)
+    # skeleton kernel tuned on the last warm-up generation's plan
+    _, cache_tune_ms = ctx.autotune_cache()
+    tuned = ctx.tuning
     ctx.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -682,7 +685,8 @@ def main():
             "node_evals_per_generation": st_b["logical_node_evals"] // max(K, 1),
             "best_mse_last_generation": best,
             "tuning": {"kernel": mse_kernel, "cache_kernel": ref_kernel, "chunk": tuned.chunk,
-                       "n_queues": tuned.n_queues, "blocks_per_cu": tuned.blocks_per_cu},
+                       "n_queues": tuned.n_queues, "blocks_per_cu": tuned.blocks_per_cu,
+                       "cache_kernel_ms_on_tuning_plan": cache_tune_ms},
         },
         "roofline": {
             "bound": "valu",

@@ -279,12 +279,28 @@ int sr_jit_compile(const sr_population *pop, int32_t nv, int32_t dpt, char *log_
 int sr_jit_code(const sr_population *pop, int32_t nv, int32_t dpt, void *out, int64_t cap, int64_t *len);
 
 /* Autotuner (no counterpart in the reference): times every valid launch
- * shape of the batched interpreter (datapoints per lane, chunk, XCD queues,
+ * shape of the batched interpreter (lane interpreter with 1..4 datapoints per
+ * lane or vector interpreter, chunk = datapoints per work item, XCD queues,
  * workgroups per CU) on a length-stratified sample of at most max_sample
  * expressions of pop (0: 256), sets the fastest as the context tuning and
- * remembers it per (device, variable-register class, N).  best_ms: kernel ms
- * of the winner on the sample (0 when reused). */
+ * remembers it per (architecture, CU count, variable-register class, N).
+ * best_ms: kernel ms of the winner on the sample (0 when reused). */
 int sr_autotune(sr_ctx *ctx, const sr_population *pop, int32_t max_sample, sr_tuning *best, double *best_ms);
+/* Skeleton kernel of the device-planned subtree cache (cache_kernel,
+ * cache_dpt): re-runs both interpreter launches of the last cached
+ * generation (SR_CACHE_DEVICE; identical columns and partial sums) with every
+ * candidate, keeps the fastest in the context tuning and the remembered
+ * entry, and drops captured generation graphs.  best_ms: the winner's ms for
+ * both launches.  SR_ERR_INVALID_ARG before a cached generation ran on the
+ * current data. */
+int sr_autotune_cache(sr_ctx *ctx, sr_tuning *best, double *best_ms);
+/* The remembered tunings as text, one entry per line
+ * ("arch cus xc N dpt chunk n_queues blocks_per_cu cache_dpt kernel
+ * cache_kernel"); sr_tune_load adds a file's entries (all or none: a bad line
+ * fails the call with SR_ERR_INVALID_ARG), so a later sr_autotune of the same
+ * shape reuses them without timing.  entries may be NULL. */
+int sr_tune_save(const char *path);
+int sr_tune_load(const char *path, int32_t *entries);
 /* Same for the constant-fitting kernels of the prepared population: picks
  * datapoints per lane for the forward (finite differences) and adjoint
  * kernels at the given constants (device pointer [E*8]). */

@@ -111,6 +111,9 @@ _SIGNATURES = {
     "sr_eval_predictions": [_vp, C.POINTER(_Pop), _f32p],
     "sr_get_tuning": [_vp, C.POINTER(_Tuning)],
     "sr_autotune": [_vp, C.POINTER(_Pop), C.c_int32, C.POINTER(_Tuning), C.POINTER(C.c_double)],
+    "sr_autotune_cache": [_vp, C.POINTER(_Tuning), C.POINTER(C.c_double)],
+    "sr_tune_save": [C.c_char_p],
+    "sr_tune_load": [C.c_char_p, C.POINTER(C.c_int32)],
     "sr_param_autotune": [_vp, _vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "sr_set_tuning": [_vp, C.POINTER(_Tuning)],
     "sr_compile_expr": [_i32p, _f32p, C.c_int32, C.c_int32, _i32p, _i32p, C.POINTER(C.c_int32)],
@@ -230,6 +233,22 @@ def jit_source(pop, nv: int, dpt: int = 4) -> str:
     buf = C.create_string_buffer(n.value + 1)
     lib().sr_jit_source(C.byref(pop._c()), nv, dpt, buf, n.value + 1, C.byref(n))
     return buf.value.decode()
+
+
+def tune_save(path: str) -> None:
+    """Write the remembered tunings (every context of this process) to a text file."""
+    rc = lib().sr_tune_save(path.encode())
+    if rc != 0:
+        raise SrError(rc, f"cannot write tunings to {path}")
+
+
+def tune_load(path: str) -> int:
+    """Add the tunings of a file written by tune_save; returns the entry count."""
+    n = C.c_int32(0)
+    rc = lib().sr_tune_load(path.encode(), C.byref(n))
+    if rc != 0:
+        raise SrError(rc, f"cannot read tunings from {path}")
+    return n.value
 
 
 def jit_compile(pop, nv: int, dpt: int = 4):
@@ -541,6 +560,14 @@ class Context:
         t = _Tuning()
         ms = C.c_double()
         self._check(self._L.sr_autotune(self._h, C.byref(pop._c()), max_sample, C.byref(t), C.byref(ms)))
+        return Tuning(t.dpt, t.chunk, t.n_queues, t.blocks_per_cu, t.cache_dpt, t.kernel, t.cache_kernel), ms.value
+
+    def autotune_cache(self):
+        """Time the skeleton kernels of the device-planned cache on the last
+        cached generation's plan; keeps the fastest.  Returns (Tuning, ms)."""
+        t = _Tuning()
+        ms = C.c_double()
+        self._check(self._L.sr_autotune_cache(self._h, C.byref(t), C.byref(ms)))
         return Tuning(t.dpt, t.chunk, t.n_queues, t.blocks_per_cu, t.cache_dpt, t.kernel, t.cache_kernel), ms.value
 
     def param_autotune(self, d_cvals: int):

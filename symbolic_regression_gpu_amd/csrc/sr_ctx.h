@@ -134,6 +134,7 @@ struct MemoState {
 // a generation is one captured graph.
 struct DMemoState {
     bool ready = false;
+    bool planned = false;  // a generation was planned and launched since setup
     int C = 0;        // slots (value columns)
     int H = 0;        // lookup table entries (power of two >= 2C)
     int H2 = 0;       // per-generation count table entries (power of two >= 2 * E * cap)
@@ -215,6 +216,7 @@ struct Ctx {
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
     int num_cu = 0;
+    char arch[64] = {0};  // gcnArchName without target features (e.g. "gfx950")
     std::string err;
     std::recursive_mutex mu;
 
@@ -326,6 +328,9 @@ int validate_population(Ctx &c, const sr_population &pop);
 int dmemo_setup(Ctx &c, int capacity, int min_size);
 int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *ev = nullptr);
 int dmemo_stats(Ctx &c, long long out[9]);
+// The two interpreter launches of the last planned generation again (same
+// columns, same partial sums); ev: optional {start1, end1, start2, end2}.
+int dmemo_launch(Ctx &c, hipStream_t s, hipEvent_t *ev = nullptr);
 
 }  // namespace sr
 

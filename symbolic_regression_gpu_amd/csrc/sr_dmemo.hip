@@ -805,6 +805,7 @@ int dmemo_setup(Ctx &c, int capacity, int min_size)
     d.data_version = c.data_version;
     d.min_size = min_size;
     d.ready = false;
+    d.planned = false;
     hipError_t he = hipSuccess;
     auto need = [&](DevBuf &buf, size_t bytes) {
         if (he == hipSuccess) he = buf.ensure(bytes);
@@ -926,6 +927,22 @@ int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *
     if ((he = launch_order_by_len(d.p1_plen.as<int32_t>(), d.L1, d.p1_order.as<int32_t>(), s)) != hipSuccess ||
         (he = launch_order_by_len(d.p2_plen.as<int32_t>(), d.E, d.p2_order.as<int32_t>(), s)) != hipSuccess)
         return c.hip_fail(he, "device cache order");
+    int rc = dmemo_launch(c, s, ev);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_dm_gather, dim3(grid_of(std::max(d.L1, d.E))), dim3(kThreads), 0, s, a, cnt_out);
+    if ((he = hipGetLastError()) != hipSuccess) return c.hip_fail(he, "k_dm_gather");
+    d.planned = true;
+    return SR_OK;
+}
+
+// The two interpreter launches of the planned generation.  Running them
+// again recomputes the same columns and partial sums (columns hold exactly
+// what the interpreter computes, with every kernel variant), which the
+// skeleton-kernel tuner relies on.
+int dmemo_launch(Ctx &c, hipStream_t s, hipEvent_t *ev)
+{
+    DMemoState &d = c.dm;
+    if (!d.ready) return c.fail(SR_ERR_INVALID_ARG, "device cache not set up");
     LaunchPlan lp;
     int rc = plan_launch(c, MODE_MSE_REF, lp);
     if (rc) return rc;
@@ -953,8 +970,6 @@ int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *
                              ev ? ev[2 * launch + 1] : nullptr);
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_dm_gather, dim3(grid_of(std::max(d.L1, d.E))), dim3(kThreads), 0, s, a, cnt_out);
-    if ((he = hipGetLastError()) != hipSuccess) return c.hip_fail(he, "k_dm_gather");
     return SR_OK;
 }
 

@@ -388,6 +388,15 @@ int sr_ctx_create(int device, sr_ctx **out)
     Ctx &c = x->c;
     c.device = device;
     hipDeviceGetAttribute(&c.num_cu, hipDeviceAttributeMultiprocessorCount, device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        size_t k = 0;
+        while (k + 1 < sizeof c.arch && prop.gcnArchName[k] && prop.gcnArchName[k] != ':') {
+            c.arch[k] = prop.gcnArchName[k];
+            ++k;
+        }
+        c.arch[k] = '\0';
+    }
     if (hipStreamCreateWithFlags(&c.own_stream, hipStreamNonBlocking) != hipSuccess) {
         delete x;
         return SR_ERR_HIP;

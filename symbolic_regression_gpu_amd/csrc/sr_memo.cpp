@@ -595,6 +595,7 @@ int sr_cache_policy(sr_ctx *x, int32_t policy, int32_t capacity, int32_t min_siz
     m.planner.forget();
     m.cols.release();  // re-sized and re-configured at the next evaluation
     c.dm.ready = false;
+    c.dm.planned = false;
     c.ga.release_graphs();
     return SR_OK;
 }

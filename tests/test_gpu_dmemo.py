@@ -101,3 +101,41 @@ def test_device_cache_evaluate_twice_and_duplicates(ctx):
         ctx.cache_policy(CACHE_REFERENCE, 0, 3)
     assert _same(m1, m_ref) and _same(m2, m_ref)
     assert st2["executed_node_evals"] < st1["executed_node_evals"]
+
+
+def test_skeleton_tuner_keeps_results_and_persists(ctx, tmp_path):
+    """sr_autotune_cache refuses to run before a cached generation, then times
+    the skeleton kernels on the last plan; the generations that follow keep
+    the uncached run's fitness bit for bit, and the remembered tuning is
+    written to and read back from a file."""
+    from symbolic_regression_gpu_amd import SrError, api
+    nv, pop, gens, seed = 3, 300, 6, 77
+    X = uniform_data(nv, 20_011, seed)
+    X[nv] = np.sin(X[0]) * X[1] + 1.5
+    ctx.set_data(X)
+    ctx.ga_init(_params(nv), pop, seed)
+    start = ctx.ga_population()
+    sel = SelectParams(7, 0.1, 0.05)
+    saved = ctx.tuning
+    try:
+        ctx.cache_policy(CACHE_REFERENCE, 0, 3)
+        hist_b, _, _ = ctx.ga_run(sel, EVAL_BATCHED, gens, 0, seed, True)
+        ctx.ga_set_population(*start)
+        ctx.cache_policy(CACHE_DEVICE, 0, 3)
+        with pytest.raises(SrError):
+            ctx.autotune_cache()
+        h1, _, _ = ctx.ga_run(sel, EVAL_CACHED, 2, 0, seed, True)
+        t, ms = ctx.autotune_cache()
+        assert ms > 0 and t.cache_kernel in (0, 1) and 0 <= t.cache_dpt <= 4
+        assert ctx.tuning.cache_kernel == t.cache_kernel and ctx.tuning.cache_dpt == t.cache_dpt
+        h2, _, _ = ctx.ga_run(sel, EVAL_CACHED, gens - 2, 2, seed, True)
+        assert _same(np.concatenate([h1, h2]), hist_b)
+        path = tmp_path / "tune.txt"
+        api.tune_save(str(path))
+        rows = [line.split() for line in path.read_text().splitlines() if line and not line.startswith("#")]
+        mine = [r for r in rows if r[0].startswith("gfx") and int(r[2]) == 4 and int(r[3]) == 20_011]
+        assert mine and int(mine[0][9]) in (0, 1) and int(mine[0][10]) == t.cache_kernel
+        assert api.tune_load(str(path)) == len(rows)
+    finally:
+        ctx.cache_policy(CACHE_REFERENCE, 0, 3)
+        ctx.tuning = saved
