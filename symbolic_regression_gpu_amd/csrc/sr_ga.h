@@ -131,8 +131,10 @@ SR_HD int ga_subtree_size(const int32_t *tok, int len, int start)
 }
 
 // Grow generator (utils.cpp:995-1108) writing at most cap tokens into
-// tok/val.  Returns the length, or -1 if cap would be exceeded.
-SR_HD int ga_generate(const GaParams &p, int budget, Rng &r, int32_t *tok, float *val, int cap)
+// tok/val.  Returns the length, or -1 if cap would be exceeded.  With store
+// false it only draws (the lanes of a wave that run the same generator and
+// leave the writes to one lane).
+SR_HD int ga_generate(const GaParams &p, int budget, Rng &r, int32_t *tok, float *val, int cap, bool store = true)
 {
     if (budget <= 0) return 0;
     int left = p.max_tokens < budget ? p.max_tokens : budget;
@@ -149,8 +151,10 @@ SR_HD int ga_generate(const GaParams &p, int budget, Rng &r, int32_t *tok, float
         if (left <= 0) {
             if (!top) {  // budget exhausted under a pending child: fallback leaf
                 if (n >= cap) return -1;
-                tok[n] = TOK_VAR;
-                val[n] = 0.0f;
+                if (store) {
+                    tok[n] = TOK_VAR;
+                    val[n] = 0.0f;
+                }
                 ++n;
             }
             continue;
@@ -161,12 +165,10 @@ SR_HD int ga_generate(const GaParams &p, int budget, Rng &r, int32_t *tok, float
         if (leaf) {
             if (n >= cap) return -1;
             bool is_const = r.uniform(0.0f, 1.0f) < p.prob_const_leaf;
-            if (is_const) {
-                tok[n] = TOK_CONST;
-                val[n] = r.uniform(p.const_min, p.const_max);
-            } else {
-                tok[n] = TOK_VAR;
-                val[n] = (float)r.randint(0, nvars - 1);
+            const float lv = is_const ? r.uniform(p.const_min, p.const_max) : (float)r.randint(0, nvars - 1);
+            if (store) {
+                tok[n] = is_const ? TOK_CONST : TOK_VAR;
+                val[n] = lv;
             }
             ++n;
             --left;
@@ -184,16 +186,20 @@ SR_HD int ga_generate(const GaParams &p, int budget, Rng &r, int32_t *tok, float
             arity = 2;
         }
         if (n >= cap) return -1;
-        tok[n] = op;
-        val[n] = 0.0f;
+        if (store) {
+            tok[n] = op;
+            val[n] = 0.0f;
+        }
         ++n;
         --left;
         if (sp + arity > (int)(sizeof(stack) / sizeof(stack[0]))) return -1;
         for (int k = 0; k < arity; ++k) stack[sp++] = (int16_t)(d + 1);
     }
     if (n == 0) {
-        tok[0] = TOK_CONST;
-        val[0] = 0.0f;
+        if (store) {
+            tok[0] = TOK_CONST;
+            val[0] = 0.0f;
+        }
         n = 1;
     }
     return n;
@@ -293,6 +299,92 @@ SR_HD int ga_child(const GaParams &p, const SelectParams &s, int pop, int cap, c
     const int b2 = order[ga_tournament(r, fit, order, pop, s.tournament)];
     return ga_crossover(p, tok + (size_t)b1 * cap, val + (size_t)b1 * cap, len[b1],
                         tok + (size_t)b2 * cap, val + (size_t)b2 * cap, len[b2], r, ct, cv, cap);
+}
+
+// ---- one wave per child ----------------------------------------------------
+// ga_child's decisions (the same draws in the same order, every lane running
+// them alike), with the parents' tokens staged in LDS for the subtree scans
+// and the copies spread over the lanes; generated tokens are written by lane
+// 0.  Offspring are identical to ga_child's.
+__device__ inline void ga_wave_copy(const int32_t *st, const float *sv, int len, int32_t *dt, float *dv, int lane)
+{
+    for (int k = lane; k < len; k += 64) {
+        dt[k] = st[k];
+        dv[k] = sv[k];
+    }
+}
+
+__device__ inline void ga_wave_stage(const int32_t *src, int len, int32_t *lds, int lane)
+{
+    for (int k = lane; k < len; k += 64) lds[k] = src[k];
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's LDS writes done
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ inline int ga_child_wave(const GaParams &p, const SelectParams &s, int pop, int cap,
+                                    const int32_t *tok, const float *val, const int32_t *len, const float *fit,
+                                    const int32_t *order, uint64_t seed, uint32_t gen, int c, int32_t *ct,
+                                    float *cv, int32_t *l1, int32_t *l2, int lane)
+{
+    if (c < s.elites) {
+        const int e = order[c];
+        ga_wave_copy(tok + (size_t)e * cap, val + (size_t)e * cap, len[e], ct, cv, lane);
+        return len[e];
+    }
+    Rng r;
+    r.init(seed, (uint32_t)c, gen, GA_DOMAIN_NEXT);
+    const int b1 = order[ga_tournament(r, fit, order, pop, s.tournament)];
+    const int32_t *pt = tok + (size_t)b1 * cap;
+    const float *pv = val + (size_t)b1 * cap;
+    const int plen = len[b1];
+    ga_wave_stage(pt, plen, l1, lane);
+    if (r.uniform(0.0f, 1.0f) < s.p_mutation) {  // ga_mutate
+        if (plen <= 0) return ga_generate(p, p.max_tokens, r, ct, cv, cap, lane == 0);
+        const int idx = r.randint(0, plen - 1);
+        int old = ga_subtree_size(l1, plen, idx);
+        if (old <= 0) old = 1;
+        const int base = plen - old;
+        const int max_new = p.max_tokens - base;
+        if (max_new <= 0) {
+            ga_wave_copy(pt, pv, plen, ct, cv, lane);
+            return plen;
+        }
+        ga_wave_copy(pt, pv, idx, ct, cv, lane);
+        const int tail = plen - (idx + old);
+        const int room = cap - idx - tail;
+        const int nl = ga_generate(p, max_new, r, ct + idx, cv + idx, room, lane == 0);
+        if (nl < 0) return -1;
+        ga_wave_copy(pt + idx + old, pv + idx + old, tail, ct + idx + nl, cv + idx + nl, lane);
+        return idx + nl + tail;
+    }
+    const int b2 = order[ga_tournament(r, fit, order, pop, s.tournament)];
+    const int32_t *rt = tok + (size_t)b2 * cap;
+    const float *rv = val + (size_t)b2 * cap;
+    const int rlen = len[b2];
+    ga_wave_stage(rt, rlen, l2, lane);
+    // ga_crossover
+    if (plen <= 0 || rlen <= 0) {
+        ga_wave_copy(pt, pv, plen, ct, cv, lane);
+        return plen;
+    }
+    const int li = r.randint(0, plen - 1);
+    int lsub = ga_subtree_size(l1, plen, li);
+    if (lsub <= 0) lsub = 1;
+    const int attempts = rlen < 32 ? rlen : 32;
+    for (int a = 0; a < attempts; ++a) {
+        const int ri = r.randint(0, rlen - 1);
+        const int rsub = ga_subtree_size(l2, rlen, ri);
+        if (rsub <= 0) continue;
+        const int total = plen - lsub + rsub;
+        if (total > p.max_tokens) continue;
+        if (total > cap) return -1;
+        ga_wave_copy(pt, pv, li, ct, cv, lane);
+        ga_wave_copy(rt + ri, rv + ri, rsub, ct + li, cv + li, lane);
+        ga_wave_copy(pt + li + lsub, pv + li + lsub, plen - li - lsub, ct + li + rsub, cv + li + rsub, lane);
+        return total;
+    }
+    ga_wave_copy(pt, pv, plen, ct, cv, lane);
+    return plen;
 }
 
 }  // namespace sr

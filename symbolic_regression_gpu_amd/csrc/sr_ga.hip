@@ -28,42 +28,44 @@ __global__ void k_ga_init(GaParams p, int pop, int cap, uint64_t seed, int32_t *
 }
 
 // order[rank] = i under fitness_before (ascending fitness, NaN last, ties by
-// index).  O(n^2) comparisons; fitness staged through LDS in tiles.
+// index): one wave per individual, its lanes compare against n / 64 others
+// each and the counts are summed across the wave.
 __global__ __launch_bounds__(256) void k_ga_rank(const float *fit, int n, int32_t *order)
 {
-    __shared__ float tile[256];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const float fi = i < n ? fit[i] : 0.0f;
+    const int i = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) / 64);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const float fi = fit[i];
     int rank = 0;
-    for (int base = 0; base < n; base += 256) {
-        const int j = base + threadIdx.x;
-        tile[threadIdx.x] = j < n ? fit[j] : 0.0f;
-        __syncthreads();
-        const int m = n - base < 256 ? n - base : 256;
-        if (i < n)
-            for (int k = 0; k < m; ++k) rank += fitness_before(tile[k], base + k, fi, i) ? 1 : 0;
-        __syncthreads();
-    }
-    if (i < n) order[rank] = i;
+    for (int j = lane; j < n; j += 64) rank += fitness_before(fit[j], j, fi, i) ? 1 : 0;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) rank += __shfl_xor(rank, m, 64);
+    if (lane == 0) order[rank] = i;
 }
 
-// One thread per child of generation *gen.
-__global__ void k_ga_next(GaParams p, SelectParams s, int pop, int cap, const int32_t *tok,
-                          const float *val, const int32_t *len, const float *fit,
-                          const int32_t *order, uint64_t seed, const uint32_t *gen,
-                          int32_t *otok, float *oval, int32_t *olen, int32_t *status)
+// One wave per child of generation *gen (parents staged in LDS: two rows of
+// cap tokens per wave, dynamic shared memory).
+__global__ __launch_bounds__(256) void k_ga_next(GaParams p, SelectParams s, int pop, int cap, const int32_t *tok,
+                                                 const float *val, const int32_t *len, const float *fit,
+                                                 const int32_t *order, uint64_t seed, const uint32_t *gen,
+                                                 int32_t *otok, float *oval, int32_t *olen, int32_t *status)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    extern __shared__ int32_t ga_lds[];
+    const int w = threadIdx.x / 64, lane = threadIdx.x & 63;
+    const int c = (int)(blockIdx.x * (blockDim.x / 64)) + w;
     if (c >= pop) return;
-    int n = ga_child(p, s, pop, cap, tok, val, len, fit, order, seed, *gen, c,
-                     otok + (size_t)c * cap, oval + (size_t)c * cap);
-    if (n < 0) {
-        atomicOr((unsigned *)status, 2u);
-        otok[(size_t)c * cap] = TOK_CONST;
-        oval[(size_t)c * cap] = 0.0f;
-        n = 1;
+    int32_t *l1 = ga_lds + (size_t)w * 2 * cap, *l2 = l1 + cap;
+    int n = ga_child_wave(p, s, pop, cap, tok, val, len, fit, order, seed, *gen, c, otok + (size_t)c * cap,
+                          oval + (size_t)c * cap, l1, l2, lane);
+    if (lane == 0) {
+        if (n < 0) {
+            atomicOr((unsigned *)status, 2u);
+            otok[(size_t)c * cap] = TOK_CONST;
+            oval[(size_t)c * cap] = 0.0f;
+            n = 1;
+        }
+        olen[c] = n;
     }
-    olen[c] = n;
 }
 
 // One thread per expression: validate and compile into the batched
@@ -162,7 +164,7 @@ hipError_t launch_ga_init(const GaParams &p, int pop, int cap, uint64_t seed, in
 
 hipError_t launch_ga_rank(const float *fit, int n, int32_t *order, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ga_rank, dim3(blocks(n, 256)), dim3(256), 0, s, fit, n, order);
+    hipLaunchKernelGGL(k_ga_rank, dim3(blocks(n, 4)), dim3(256), 0, s, fit, n, order);
     return hipGetLastError();
 }
 
@@ -171,8 +173,8 @@ hipError_t launch_ga_next(const GaParams &p, const SelectParams &sp, int pop, in
                           const int32_t *order, uint64_t seed, const uint32_t *gen, int32_t *otok,
                           float *oval, int32_t *olen, int32_t *status, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ga_next, dim3(blocks(pop, 64)), dim3(64), 0, s, p, sp, pop, cap, tok, val,
-                       len, fit, order, seed, gen, otok, oval, olen, status);
+    hipLaunchKernelGGL(k_ga_next, dim3(blocks(pop, 4)), dim3(256), (size_t)4 * 2 * cap * sizeof(int32_t), s, p,
+                       sp, pop, cap, tok, val, len, fit, order, seed, gen, otok, oval, olen, status);
     return hipGetLastError();
 }
 
