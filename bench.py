@@ -60,6 +60,7 @@ GRAMMAR_UNARY = [10, 11, 29, 22, 27]
 GRAMMAR_BINARY = [1, 2, 3, 8]
 METRIC = "node-evals/s + ms/generation (1000 exprs × 1M dps), plain vs batched+adjoint"
 PEAK_FP32_TFLOPS = 157.3
+DEVICE_CACHE_MIN_SIZE = 2
 PEAK_HBM_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "traffic_k_batched_evolution_1000x1M.json")
 TRAFFIC_FILE_REF = os.path.join(HERE, "profiles", "traffic_k_batched_mseref_cached_evolution.json")
@@ -533,7 +534,9 @@ def main():
     # population) with the generational subtree cache planned on the device,
     # one captured graph per generation; its MSEs equal the batched ones bit
     # for bit, so selection and offspring are identical.
-    ctx.cache_policy(CACHE_DEVICE, 0, 3)
+    # subtrees of >= 2 tokens get columns (a unary op over a leaf included):
+    # measured the fastest minimum size on this workload (2..7)
+    ctx.cache_policy(CACHE_DEVICE, 0, DEVICE_CACHE_MIN_SIZE)
     ctx.ga_set_population(tok, val, ln, cap)
     log(f"[bench] cached generations (device-planned cache, hipGraph): {W} warm-up + {K} timed")
     ctx.ga_run(sel, EVAL_CACHED, W, 0, args.seed, True)
@@ -554,7 +557,7 @@ def main():
     # the same generations again without graphs: HIP events around each of
     # the two interpreter launches (the roofline's kernel time)
     log("[bench] the same cached generations, launches timed with events")
-    ctx.cache_policy(CACHE_DEVICE, 0, 3)
+    ctx.cache_policy(CACHE_DEVICE, 0, DEVICE_CACHE_MIN_SIZE)
     ctx.ga_set_population(tok, val, ln, cap)
     ctx.ga_run(sel, EVAL_CACHED, W, 0, args.seed, False)
     hist_t, _, st_t = ctx.ga_run(sel, EVAL_CACHED, K, W, args.seed, False)
