@@ -62,8 +62,8 @@ METRIC = "node-evals/s + ms/generation (1000 exprs × 1M dps), plain vs batched+
 PEAK_FP32_TFLOPS = 157.3
 DEVICE_CACHE_MIN_SIZE = 2
 PEAK_HBM_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(HERE, "profiles", "traffic_k_batched_evolution_1000x1M.json")
-TRAFFIC_FILE_REF = os.path.join(HERE, "profiles", "traffic_k_batched_mseref_cached_evolution.json")
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "traffic_k_vec_mse_evolution_1000x1M.json")
+TRAFFIC_FILE_REF = os.path.join(HERE, "profiles", "traffic_k_vec_mseref_cached_evolution.json")
 # VALU issue rate of both interpreter kernels from a counter run of this
 # script (SQ_INSTS_VALU over the kernels' time against one wave64 VALU
 # instruction per CU per cycle)
