@@ -48,6 +48,9 @@ static int ga_alloc(Ctx &c, int pop, int cap)
     if (he == hipSuccess) he = g.status.ensure(64);
     if (he == hipSuccess) he = g.max_need.ensure(64);
     if (he == hipSuccess) he = g.gen.ensure(64);
+    // the status word is read after every GA step: it starts cleared, not as
+    // whatever the allocation held
+    if (he == hipSuccess) he = hipMemsetAsync(g.status.p, 0, 4, c.stream);
     if (he != hipSuccess) return c.hip_fail(he, "hipMalloc GA buffers");
     g.pop = pop;
     g.cap = cap;

@@ -8,10 +8,10 @@
 // columns, prefetched one tile ahead, cannot hide their latency.  Here a
 // wave decodes an instruction once per 384 datapoints, loads the
 // instruction's variable or column operand straight from memory (six
-// independent coalesced loads per lane, with the next instruction's operand
-// loaded while the current one computes), keeps the stack in LDS (slots
-// beyond kVecLdsSlots in a per-wave spill area of global memory) and fetches
-// instructions with scalar loads.  Nothing depends on the number of
+// independent coalesced loads per lane, with the operands of the next two
+// instructions in flight while the current one computes), keeps the stack in
+// LDS (slots beyond kVecLdsSlots in a per-wave spill area of global memory)
+// and fetches instructions with scalar loads.  Nothing depends on the number of
 // variables, so one instantiation serves every nv.
 //
 // Work items, chunking, fused MSE partials and the per-lane summation order
@@ -152,22 +152,28 @@ __global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
                 if (has_ref(MODE) && src == SRC_REF) return a.cache + (int64_t)arg * N + t0;
                 return nullptr;
             };
-            float top[kVecD], o[kVecD], on[kVecD];
+            float top[kVecD], o[kVecD], y[kVecD];
+            load(a.X + (int64_t)a.nv * N + t0, y);  // the target, needed after the program
             vec_each([&](int k) { top[k] = 0.0f; });
             int sp = 0;
-            Ins cur = n > 0 ? fetch_ins(prog, 0) : Ins{0, 0};
-            int code = __builtin_amdgcn_readfirstlane(cur.code);
-            int arg = __builtin_amdgcn_readfirstlane(cur.arg);
-            if (const float *p = operand_ptr(code, arg)) load(p, on);
-            for (int i = 0; i < n; ++i) {
-                const Ins nx = i + 1 < n ? fetch_ins(prog, i + 1) : cur;
-                const int ncode = __builtin_amdgcn_readfirstlane(nx.code);
-                const int narg = __builtin_amdgcn_readfirstlane(nx.arg);
+            // operands two instructions ahead: instruction i reads buffer
+            // i % 2, which then receives the operand of instruction i + 2
+            float ba[kVecD], bb[kVecD];
+            int code_a = 0, arg_a = 0, code_b = 0, arg_b = 0;
+            auto fetch_to = [&](int i, int &c_, int &a_, float(&B)[kVecD]) {
+                if (i >= n) return;
+                const Ins x = fetch_ins(prog, i);
+                c_ = __builtin_amdgcn_readfirstlane(x.code);
+                a_ = __builtin_amdgcn_readfirstlane(x.arg);
+                if (const float *p = operand_ptr(c_, a_)) load(p, B);
+            };
+            fetch_to(0, code_a, arg_a, ba);
+            fetch_to(1, code_b, arg_b, bb);
+            auto step = [&](int i, int &code_r, int &arg_r, float(&B)[kVecD]) {
+                const int code = code_r, arg = arg_r;
                 const int src = bc_src(code);
-                if (src == SRC_VAR || (has_ref(MODE) && src == SRC_REF)) vec_each([&](int k) { o[k] = on[k]; });
-                // the next instruction's operand is in flight while this one computes
-                if (i + 1 < n)
-                    if (const float *p = operand_ptr(ncode, narg)) load(p, on);
+                if (src == SRC_VAR || (has_ref(MODE) && src == SRC_REF)) vec_each([&](int k) { o[k] = B[k]; });
+                fetch_to(i + 2, code_r, arg_r, B);
                 if (code & BC_PUSH) {
                     if (sp < kVecLdsSlots) vec_each([&](int k) { stk[sp][k][lane] = top[k]; });
                     else vec_each([&](int k) { spill[(sp - kVecLdsSlots) * kVecTile + k * kWave + lane] = top[k]; });
@@ -176,7 +182,13 @@ __global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
                 if (src == SRC_POP) {
                     --sp;
                     if (sp < kVecLdsSlots) vec_each([&](int k) { o[k] = stk[sp][k][lane]; });
-                    else vec_each([&](int k) { o[k] = spill[(sp - kVecLdsSlots) * kVecTile + k * kWave + lane]; });
+                    else {
+                        vec_each([&](int k) { o[k] = spill[(sp - kVecLdsSlots) * kVecTile + k * kWave + lane]; });
+                        // settle these loads here: left pending, they would make every
+                        // later write of the operand registers, on any path, wait for all
+                        // outstanding loads -- the prefetched operands included
+                        __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
+                    }
                 } else if (src == SRC_CONST) {
                     const float cv = __int_as_float(arg);
                     vec_each([&](int k) { o[k] = cv; });
@@ -198,12 +210,11 @@ __global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
                             if (j <= last) cs[j] = top[k];
                         });
                 }
-                code = ncode;
-                arg = narg;
-                cur = nx;
+            };
+            for (int i = 0; i < n; i += 2) {
+                step(i, code_a, arg_a, ba);
+                if (i + 1 < n) step(i + 1, code_b, arg_b, bb);
             }
-            float y[kVecD];
-            load(a.X + (int64_t)a.nv * N + t0, y);
             vec_each([&](int k) {
                 const bool ok = k * kWave + lane <= last && isfinite(top[k]);
                 const float diff = top[k] - y[k];

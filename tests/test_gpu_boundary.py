@@ -45,6 +45,38 @@ def test_evolve_step_matches_oracle(ctx):
     assert e.value.code == -3
 
 
+def test_evolve_step_first_on_a_context_after_other_work():
+    """sr_evolve_step as the first GA call of a context whose device memory
+    was used before (evaluation, constant fitting): the GA status word starts
+    cleared, so the step succeeds and matches the oracle."""
+    from symbolic_regression_gpu_amd import EVAL_BATCHED, Context
+    c = Context(0)
+    try:
+        nv, N = 4, 4096
+        pop, slot = c5_population(64, 17, nv, min_len=20, max_tokens=120)
+        X = uniform_data(nv, N, 6)
+        c.set_data(X)
+        api = to_api(pop)
+        c.eval_mse(api, EVAL_BATCHED)
+        fit = ConstantFit(c, api, slot)
+        fit.adjoint_grad(fit.initial())
+        torch.cuda.synchronize()
+        gp, op = GaParams.make(**KW), O.ga_params(**KW)
+        cap = KW["max_tokens"] + KW["max_depth"] + 2
+        n = 300
+        rc, tok, val, ln = O.ga_init_population(op, n, cap, 43)
+        f = np.random.default_rng(4).uniform(0, 10, n).astype(np.float32)
+        t2, v2, l2, _ = c.evolve_step(Population.from_strided(tok, val, ln, cap), f, gp,
+                                      SelectParams(7, 0.1, 0.05), 5, 1)
+        rc, wt, wv, wl = O.ga_next_generation(op, 7, 0.1, 0.05, n, cap, tok, val, ln, f, 5, 1)
+        assert rc == 0 and np.array_equal(l2, wl)
+        for i in range(n):
+            s = slice(i * cap, i * cap + wl[i])
+            assert np.array_equal(t2[s], wt[s]) and np.array_equal(v2[s].view(np.int32), wv[s].view(np.int32)), i
+    finally:
+        c.close()
+
+
 def test_grad_consts_matches_fp64_bound_and_fit_path(ctx):
     nv, N = 4, 4096
     pop, slot = c5_population(64, 13, nv, min_len=20, max_tokens=120)
