@@ -41,7 +41,15 @@ struct ParamArgs {
     int replicas;           // forward: 1 = plain, 2*MAX_SLOTS = central differences
     float fd_h;
     int tape_stride;        // tape rows per wave
+    int prog_lds;           // instructions per wave in the program LDS (longest program + 1)
 };
+
+// Dynamic LDS of the parametric kernels: each wave's program (Ins) and, for
+// the adjoint, its aux words.
+inline size_t param_lds_bytes(int prog_lds, bool adjoint)
+{
+    return (size_t)(kBlock / kWave) * prog_lds * (sizeof(Ins) + (adjoint ? sizeof(int32_t) : 0));
+}
 
 // d f(u) / du for unary op, given input u and output y.
 SR_DEV float d_unary(int op, float u, float y)

@@ -37,9 +37,9 @@ SR_DEV void stage_param_program(const ParamArgs &a, int e, int n, int lane, int 
 template <int DPT, int XC>
 __global__ __launch_bounds__(kBlock) void k_param_fwd(ParamArgs a)
 {
-    __shared__ Ins s_prog[kBlock / kWave][kProgLds];
+    extern __shared__ Ins s_prog[];  // [waves][prog_lds]
     const int lane = threadIdx.x & (kWave - 1);
-    Ins *lp = s_prog[threadIdx.x / kWave];
+    Ins *lp = s_prog + (threadIdx.x / kWave) * a.prog_lds;
     const int q = a.n_queues > 1 ? (int)(blockIdx.x % (unsigned)a.n_queues) : 0;
     const int per_rep = a.n_exprs * a.chunks_per_queue;
     const int items = a.replicas * per_rep;
@@ -141,8 +141,7 @@ struct AdjLane {
     float &top;
     float &o;
     float (&astk)[kAdjRegs];
-    float (&tg)[kAdjRegs];
-    float (&gacc)[MAX_SLOTS];
+    float (&tg)[kAdjRegs];  // per-slot gradient sums of the work item (masked per datapoint)
     float &at;
 };
 
@@ -156,8 +155,7 @@ SR_DEV void with_adj_lanes(F &&f, L &...ls)
         float x[kXRegs<XC>];
         float top = 0.0f, o = 0.0f, at = 0.0f;
         float astk[kAdjRegs], tg[kAdjRegs];
-        float gacc[MAX_SLOTS];
-        AdjLane<XC> l{stk, x, top, o, astk, tg, gacc, at};
+        AdjLane<XC> l{stk, x, top, o, astk, tg, at};
         with_adj_lanes<DPT, XC>(f, ls..., l);
     }
 }
@@ -209,15 +207,16 @@ struct WaveTape {
     }
 };
 
+// At least 3 waves per SIMD: the 2-datapoint variants otherwise take 172-181
+// VGPRs (2 waves), and the reverse sweep is latency-bound.
 template <int DPT, int XC>
-__global__ __launch_bounds__(kBlock) void k_adjoint(ParamArgs a)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_adjoint(ParamArgs a)
 {
-    __shared__ Ins s_prog[kBlock / kWave][kProgLds];
-    __shared__ int32_t s_aux[kBlock / kWave][kProgLds];
+    extern __shared__ Ins s_prog[];  // [waves][prog_lds] programs, then [waves][prog_lds] aux words
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
-    Ins *lp = s_prog[w];
-    int32_t *la = s_aux[w];
+    Ins *lp = s_prog + w * a.prog_lds;
+    int32_t *la = reinterpret_cast<int32_t *>(s_prog + (kBlock / kWave) * a.prog_lds) + w * a.prog_lds;
     const int q = a.n_queues > 1 ? (int)(blockIdx.x % (unsigned)a.n_queues) : 0;
     const int64_t wave = (int64_t)blockIdx.x * (kBlock / kWave) + w;
     const WaveTape<DPT> tape{a.tape + wave * (int64_t)a.tape_stride * kWave * DPT, lane};
@@ -245,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(ParamArgs a)
             for_each_lane(
                 [&](int, auto &l) {
 #pragma unroll
-                    for (int k = 0; k < MAX_SLOTS; ++k) l.gacc[k] = 0.0f;
+                    for (int k = 0; k < kAdjRegs; ++k) l.tg[k] = 0.0f;
                 },
                 L...);
             auto noref = [](auto &...) {};
@@ -284,8 +283,6 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(ParamArgs a)
                         sum += okv[d] ? (double)sq : 0.0;
                         cnt += okv[d] ? 1 : 0;
                         l.at = okv[d] ? diff : 0.0f;
-#pragma unroll
-                        for (int k = 0; k < kAdjRegs; ++k) l.tg[k] = 0.0f;
                         any = any || okv[d];
                     },
                     L...);
@@ -345,7 +342,8 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(ParamArgs a)
                         },
                         L...);
                     if (slot >= 0) {
-                        ((L.tg[slot] += L.o), ...);
+                        // a point outside the MSE adds nothing, not 0 * inf
+                        for_each_lane([&](int d, auto &l) { l.tg[slot] += okv[d] ? l.o : 0.0f; }, L...);
                     } else if (src == SRC_POP) {
                         ((L.astk[asp] = L.o), ...);
                         ++asp;
@@ -356,12 +354,6 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(ParamArgs a)
                     }
                     --i;
                 }
-                for_each_lane(
-                    [&](int d, auto &l) {
-#pragma unroll
-                        for (int k = 0; k < MAX_SLOTS; ++k) l.gacc[k] += okv[d] ? l.tg[k] : 0.0f;
-                    },
-                    L...);
             }
             sum = wave_sum(sum);
             cnt = wave_sum(cnt);
@@ -369,7 +361,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(ParamArgs a)
 #pragma unroll
             for (int k = 0; k < MAX_SLOTS; ++k) {
                 double t = 0.0;
-                for_each_lane([&](int, auto &l) { t += (double)l.gacc[k]; }, L...);
+                for_each_lane([&](int, auto &l) { t += (double)l.tg[k]; }, L...);
                 g[k] = wave_sum(t);
             }
             if (lane == 0) {
@@ -418,7 +410,7 @@ hipError_t launch_param_fwd(int dpt, int xc, const ParamArgs &a, int grid, hipSt
 {
 #define SR_PF(D, XCV)                                                                            \
     if (dpt == D && xc == XCV) {                                                                \
-        hipLaunchKernelGGL((k_param_fwd<D, XCV>), dim3(grid), dim3(kBlock), 0, s, a);            \
+        hipLaunchKernelGGL((k_param_fwd<D, XCV>), dim3(grid), dim3(kBlock), param_lds_bytes(a.prog_lds, false), s, a); \
         return hipGetLastError();                                                                \
     }
     SR_PF(1, 4) SR_PF(2, 4) SR_PF(4, 4) SR_PF(1, 8) SR_PF(2, 8) SR_PF(4, 8) SR_PF(1, 16) SR_PF(2, 16)
@@ -427,13 +419,13 @@ hipError_t launch_param_fwd(int dpt, int xc, const ParamArgs &a, int grid, hipSt
     return hipErrorInvalidValue;
 }
 
-int param_fwd_occupancy(int dpt, int xc)
+int param_fwd_occupancy(int dpt, int xc, int prog_lds)
 {
     int b = 0;
 #define SR_PF(D, XCV)                                                                          \
     if (dpt == D && xc == XCV)                                                                 \
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void *)k_param_fwd<D, XCV>, \
-                                                            kBlock, 0) == hipSuccess ? b : 0;
+                                                            kBlock, param_lds_bytes(prog_lds, false)) == hipSuccess ? b : 0;
     SR_PF(1, 4) SR_PF(2, 4) SR_PF(4, 4) SR_PF(1, 8) SR_PF(2, 8) SR_PF(4, 8) SR_PF(1, 16) SR_PF(2, 16)
     SR_PF(4, 16) SR_PF(1, 32)
 #undef SR_PF
@@ -444,7 +436,7 @@ hipError_t launch_adjoint(int dpt, int xc, const ParamArgs &a, int grid, hipStre
 {
 #define SR_AD(D, XCV)                                                                    \
     if (dpt == D && xc == XCV) {                                                         \
-        hipLaunchKernelGGL((k_adjoint<D, XCV>), dim3(grid), dim3(kBlock), 0, s, a);       \
+        hipLaunchKernelGGL((k_adjoint<D, XCV>), dim3(grid), dim3(kBlock), param_lds_bytes(a.prog_lds, true), s, a); \
         return hipGetLastError();                                                        \
     }
     SR_AD(1, 4) SR_AD(2, 4) SR_AD(1, 8) SR_AD(2, 8) SR_AD(1, 16) SR_AD(2, 16) SR_AD(1, 32)
@@ -452,13 +444,13 @@ hipError_t launch_adjoint(int dpt, int xc, const ParamArgs &a, int grid, hipStre
     return hipErrorInvalidValue;
 }
 
-int adjoint_occupancy(int dpt, int xc)
+int adjoint_occupancy(int dpt, int xc, int prog_lds)
 {
     int b = 0;
 #define SR_AD(D, XCV)                                                                             \
     if (dpt == D && xc == XCV)                                                                    \
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void *)k_adjoint<D, XCV>,  \
-                                                            kBlock, 0) == hipSuccess ? b : 0;
+                                                            kBlock, param_lds_bytes(prog_lds, true)) == hipSuccess ? b : 0;
     SR_AD(1, 4) SR_AD(2, 4) SR_AD(1, 8) SR_AD(2, 8) SR_AD(1, 16) SR_AD(2, 16) SR_AD(1, 32)
 #undef SR_AD
     return 0;

@@ -40,7 +40,8 @@ static int param_plan(Ctx &c, bool adjoint, ParamPlan &pp, int force_dpt = 0)
     if (pp.chunk > pp.slice) pp.chunk = pp.slice;
     pp.cpq = (int)((pp.slice + pp.chunk - 1) / pp.chunk);
     pp.P = pp.nq * pp.cpq;
-    int occ = adjoint ? adjoint_occupancy(pp.dpt, pp.xc) : param_fwd_occupancy(pp.dpt, pp.xc);
+    const int prog_lds = c.par.max_len + 1;
+    int occ = adjoint ? adjoint_occupancy(pp.dpt, pp.xc, prog_lds) : param_fwd_occupancy(pp.dpt, pp.xc, prog_lds);
     if (occ <= 0) return c.fail(SR_ERR_UNSUPPORTED, "no constant-fitting kernel for this variable count");
     pp.grid = (int)rup((int64_t)c.num_cu * occ, pp.nq);
     return SR_OK;
@@ -75,6 +76,7 @@ static ParamArgs base_args(Ctx &c, const ParamPlan &pp, const float *cvals)
     a.chunks_per_queue = pp.cpq;
     a.P = pp.P;
     a.replicas = 1;
+    a.prog_lds = s.max_len + 1;
     return a;
 }
 
