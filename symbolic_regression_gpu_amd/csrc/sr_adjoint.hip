@@ -207,10 +207,12 @@ struct WaveTape {
     }
 };
 
-// At least 3 waves per SIMD: the 2-datapoint variants otherwise take 172-181
-// VGPRs (2 waves), and the reverse sweep is latency-bound.
+// At least 4 waves per SIMD (<= 128 VGPRs): the 2-datapoint variants
+// otherwise take 172-181 VGPRs (2 waves); the kernel is latency-bound, and a
+// few spilled registers cost less than the lost occupancy (measured on 4096
+// expressions x 256k: 253 ms per gradient at 2 waves, 183 at 3, 153 at 4).
 template <int DPT, int XC>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_adjoint(ParamArgs a)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint(ParamArgs a)
 {
     extern __shared__ Ins s_prog[];  // [waves][prog_lds] programs, then [waves][prog_lds] aux words
     const int lane = threadIdx.x & (kWave - 1);

@@ -116,14 +116,29 @@ __global__ __launch_bounds__(1024) void k_order_by_len(const int32_t *plen, int 
         atomicAdd(&hist[B - 1 - l], 1);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int k = 0; k < B; ++k) {
-            int h = hist[k];
-            hist[k] = acc;
-            acc += h;
-        }
+    // exclusive scan of the histogram: 4 bins per thread, then the threads'
+    // sums across the wave (shuffles) and across the 16 waves (LDS)
+    __shared__ int wsum[1024 / 64];
+    constexpr int per = B / 1024;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int loc[per], run = 0;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        loc[k] = run;
+        run += hist[t * per + k];
     }
+    int inc = run;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += v;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int base = inc - run;
+    for (int k = 0; k < w; ++k) base += wsum[k];
+#pragma unroll
+    for (int k = 0; k < per; ++k) hist[t * per + k] = base + loc[k];
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         int l = plen[i];
