@@ -107,6 +107,12 @@ typedef struct {
 int sr_ctx_create(int device, sr_ctx **out);
 int sr_ctx_destroy(sr_ctx *ctx);
 const char *sr_last_error(const sr_ctx *ctx);
+/* HIP versions this library was built against (HIP_VERSION of its headers)
+ * and runs on (hipRuntimeGetVersion / hipDriverGetVersion of the runtime the
+ * process loaded; one libamdhip64.so.<major> serves the whole process, torch's
+ * included).  SR_ERR_UNSUPPORTED when the runtime's major version differs
+ * from the build's.  No counterpart in the reference. */
+int sr_runtime_info(int32_t *built, int32_t *runtime, int32_t *driver);
 /* Run on a caller-owned hipStream_t; NULL is the device's null (default)
  * stream, which is what torch's default stream reports as its handle. */
 int sr_set_stream(sr_ctx *ctx, void *hip_stream);

@@ -113,6 +113,7 @@ _SIGNATURES = {
     "sr_autotune": [_vp, C.POINTER(_Pop), C.c_int32, C.POINTER(_Tuning), C.POINTER(C.c_double)],
     "sr_autotune_cache": [_vp, C.POINTER(_Tuning), C.POINTER(C.c_double)],
     "sr_tune_save": [C.c_char_p],
+    "sr_runtime_info": [C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "sr_tune_load": [C.c_char_p, C.POINTER(C.c_int32)],
     "sr_param_autotune": [_vp, _vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "sr_set_tuning": [_vp, C.POINTER(_Tuning)],
@@ -233,6 +234,13 @@ def jit_source(pop, nv: int, dpt: int = 4) -> str:
     buf = C.create_string_buffer(n.value + 1)
     lib().sr_jit_source(C.byref(pop._c()), nv, dpt, buf, n.value + 1, C.byref(n))
     return buf.value.decode()
+
+
+def runtime_info():
+    """(HIP version built against, runtime version loaded, driver version, same major)."""
+    b, r, d = C.c_int32(), C.c_int32(), C.c_int32()
+    rc = lib().sr_runtime_info(C.byref(b), C.byref(r), C.byref(d))
+    return b.value, r.value, d.value, rc == 0
 
 
 def tune_save(path: str) -> None:

@@ -429,6 +429,18 @@ int sr_ctx_destroy(sr_ctx *x)
 
 const char *sr_last_error(const sr_ctx *x) { return x ? x->c.err.c_str() : "null context"; }
 
+int sr_runtime_info(int32_t *built, int32_t *runtime, int32_t *driver)
+{
+    int rt = 0, drv = 0;
+    if (hipRuntimeGetVersion(&rt) != hipSuccess) rt = 0;
+    if (hipDriverGetVersion(&drv) != hipSuccess) drv = 0;
+    if (built) *built = HIP_VERSION;
+    if (runtime) *runtime = rt;
+    if (driver) *driver = drv;
+    // HIP_VERSION = major * 10^7 + minor * 10^5 + patch
+    return rt / 10000000 == HIP_VERSION / 10000000 ? SR_OK : SR_ERR_UNSUPPORTED;
+}
+
 int sr_set_stream(sr_ctx *x, void *stream)
 {
     if (!x) return SR_ERR_INVALID_ARG;

@@ -45,3 +45,22 @@ def test_no_device_is_a_status_code():
 def test_compile_hook_runs_without_gpu():
     code, arg, need = api.compile_expr([1, -1, 0], [0, 0, 2.0], nv=1)
     assert len(code) == 2 and need == 0
+
+
+def test_one_hip_runtime_per_process():
+    """A process that loads the library through the package runs one HIP
+    runtime (torch's libamdhip64.so.7, which the package loads first, serves
+    the library too), of the same major version as the headers the library
+    was built with.  Checked in a fresh interpreter: loading libsrhip.so
+    directly before torch maps the one its rpath names as well."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from symbolic_regression_gpu_amd import api\n"
+            "api.lib()\n"
+            "paths = sorted({l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l})\n"
+            "print(len(paths), *api.runtime_info())\n" % ROOT)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-500:]
+    n, built, runtime, driver, same = r.stdout.split()[-5:]
+    assert n == "1" and same == "True" and int(built) // 10_000_000 == 7, r.stdout
