@@ -408,7 +408,10 @@ def fp64_accuracy(ctx, cores, c1, c2, c3, evo, n_sample=1 << 16):
     checker (sro_predict64 / sro_mse64 restating cl_cpu_eval.cpp in double):
     per-point values of the single expression (all its datapoints) in fp32
     ulps of the FP64 value, and relative MSE errors of the batches on their
-    first n_sample datapoints (the GPU evaluates the same sample)."""
+    first n_sample datapoints (the GPU evaluates the same sample), with the
+    share within 1e-5 of the FP64 MSE and of the oracle's fp32 MSE (the
+    reference CPU evaluator's arithmetic; an ill-conditioned expression is
+    off FP64 in both)."""
     from oracle import oracle as O
     from symbolic_regression_gpu_amd import EVAL_BATCHED, Population
     out = {}
@@ -432,14 +435,20 @@ def fp64_accuracy(ctx, cores, c1, c2, c3, evo, n_sample=1 << 16):
         Xs = np.ascontiguousarray(Xb[:, :n])
         ctx.set_data(Xs)
         m_gpu, _, _ = ctx.eval_mse(Population.from_exprs(eb), EVAL_BATCHED)
-        m64 = O.mse64(O.Population(eb), Xs, nvb, n, threads=cores)
+        opop = O.Population(eb)
+        m64 = O.mse64(opop, Xs, nvb, n, threads=cores)
+        m32, _ = O.mse32(opop, Xs, nvb, n, threads=cores)
         m_gpu = np.asarray(m_gpu, np.float64)
+        m32 = np.asarray(m32, np.float64)
         both = np.isfinite(m_gpu) & np.isfinite(m64) & (m64 > 0)
         rel = np.abs(m_gpu[both] - m64[both]) / m64[both]
+        b32 = np.isfinite(m_gpu) & np.isfinite(m32) & (m32 > 0)
+        rel32 = np.abs(m_gpu[b32] - m32[b32]) / m32[b32]
         out[name] = {"exprs": len(eb), "datapoints": n, "finite": int(both.sum()),
                      "max_rel_mse_err": float(rel.max()) if rel.size else 0.0,
                      "median_rel_mse_err": float(np.median(rel)) if rel.size else 0.0,
                      "share_within_1e-5": float(np.mean(rel <= 1e-5)) if rel.size else 1.0,
+                     "share_within_1e-5_of_oracle_fp32": float(np.mean(rel32 <= 1e-5)) if rel32.size else 1.0,
                      "nan_class_agrees": float(np.mean(np.isnan(m_gpu) == np.isnan(m64)))}
     return out
 
