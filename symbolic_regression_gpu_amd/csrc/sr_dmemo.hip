@@ -83,7 +83,7 @@ struct DmArgs {
     int32_t *ct_cnt, *ct_rep, *ct_root, *ct_slot, *ct_wr, *ct_off, *ct_prog;
     int32_t *p1_tok;
     float *p1_val;
-    int32_t *p1_len, *p1_slot, *p1_pos;
+    int32_t *p1_len, *p1_slot, *p1_pos, *p1_need;
     int32_t *p2_tok;
     float *p2_val;
     int32_t *p2_len, *e_src;
@@ -138,6 +138,7 @@ __global__ void k_dm_clear_pos(DmArgs a)
     if (p < a.L1) {
         a.p1_len[p] = 0;
         a.p1_plen[p] = 0;
+        a.p1_need[p] = 0;
     }
     if (p < a.E) {
         a.p2_len[p] = 0;
@@ -544,7 +545,11 @@ __global__ __launch_bounds__(64) void k_dm_emit2(DmArgs a)
         return;
     }
     if (a.sh[b] && a.pslot[b] >= 0 && a.ct_prog[a.cidx[b]] >= 0) {
-        if (lane == 0) a.e_src[e] = kFromLaunch1 | a.ct_prog[a.cidx[b]];
+        if (lane == 0) {
+            const int r = a.ct_prog[a.cidx[b]];
+            a.e_src[e] = kFromLaunch1 | r;
+            a.p1_need[r] = 1;  // this root's MSE is an expression's fitness
+        }
         return;
     }
     for (int k = lane; k < L; k += 64) {
@@ -642,7 +647,8 @@ __global__ __launch_bounds__(64) void k_dm_compile(DmArgs a, int nv)
 __global__ void k_dm_gather(DmArgs a, long long *cnt_out)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < a.L1 && e < a.ctr[C_L1]) {
+    if (e < a.L1 && e < a.ctr[C_L1] && a.p1_need[e]) {
+        // only roots that were whole expressions had their MSE computed
         const int s = a.p1_slot[e];
         a.s_mse[s] = a.mse1[e];
         a.s_cnt[s] = a.cnt1[e];
@@ -746,6 +752,7 @@ DmArgs dm_args(Ctx &c, int b)
     a.p1_len = d.p1_len.as<int32_t>();
     a.p1_slot = d.p1_slot.as<int32_t>();
     a.p1_pos = d.p1_pos.as<int32_t>();
+    a.p1_need = d.p1_need.as<int32_t>();
     a.p2_tok = d.p2_tok.as<int32_t>();
     a.p2_val = d.p2_val.as<float>();
     a.p2_len = d.p2_len.as<int32_t>();
@@ -837,7 +844,7 @@ int dmemo_setup(Ctx &c, int capacity, int min_size)
     need(d.p2_tok, P2 * 4);
     need(d.p2_val, P2 * 4);
     need(d.p2_ins, P2 * sizeof(Ins));
-    for (DevBuf *bb : {&d.p1_len, &d.p1_slot, &d.p1_pos, &d.p1_plen, &d.p1_order, &d.mse1})
+    for (DevBuf *bb : {&d.p1_len, &d.p1_slot, &d.p1_pos, &d.p1_need, &d.p1_plen, &d.p1_order, &d.mse1})
         need(*bb, (size_t)d.L1 * 4);
     need(d.cnt1, (size_t)d.L1 * 8);
     need(d.p1_pf, (size_t)d.L1 * kRefPrefetch * 4);
@@ -961,6 +968,7 @@ int dmemo_launch(Ctx &c, hipStream_t s, hipEvent_t *ev)
         ba.cache = cols;
         ba.out = cols;
         ba.n_exprs_dev = d.ctr.as<int32_t>() + (one ? C_L1 : C_L2);
+        ba.need_mse = one ? d.p1_need.as<int32_t>() : nullptr;
         EvalBufs bufs{c.d_psum.as<double>(), c.d_pcnt.as<long long>(), c.d_counters.as<uint32_t>(),
                       one ? d.mse1.as<float>() : d.mse2.as<float>(),
                       one ? d.cnt1.as<long long>() : d.cnt2.as<long long>()};

@@ -50,6 +50,10 @@ struct BatchArgs {
     int P;                    // partial slots per expression = n_queues*chunks_per_queue
     int prog_stride;          // programs at e*prog_stride when prog_off is null
     float *spill;             // vector interpreter: per-wave stack spill area (k_vec)
+    // MODE_MSE_REF with k_vec (may be null = every program): need_mse[e] == 0
+    // marks a program that only writes columns; it loads no target, computes
+    // no squared errors and leaves zero partials (MSE NaN, count 0).
+    const int32_t *need_mse;
 };
 
 // ---------------------------------------------------------------------------

@@ -129,6 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
         const int64_t q1 = q0 + a.slice < N ? q0 + a.slice : N;
         const int64_t c0 = q0 + (int64_t)chunk * a.chunk;
         const int64_t c1 = c0 + a.chunk < q1 ? c0 + a.chunk : q1;
+        const bool need = !has_ref(MODE) || !a.need_mse || a.need_mse[e] != 0;
         double sum = 0.0;
         long long cnt = 0;
 
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
                 step(i, code_a, arg_a, ba);
                 if (i + 1 < n) step(i + 1, code_b, arg_b, bb);
             }
-            vec_each([&](int k) {
+            if (need) vec_each([&](int k) {
                 const bool ok = k * kWave + lane <= last && isfinite(top[k]);
                 const float diff = top[k] - y[k];
                 const float sq = diff * diff;
@@ -223,8 +224,10 @@ __global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
                 cnt += ok ? 1 : 0;
             });
         }
+        if (need) {
         sum = wave_sum(sum);
         cnt = wave_sum(cnt);
+        }
         if (lane == 0) {
             const int slot = q * a.chunks_per_queue + chunk;
             a.psum[(int64_t)e * a.P + slot] = sum;
