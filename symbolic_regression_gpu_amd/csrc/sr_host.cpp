@@ -117,8 +117,10 @@ static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 int plan_launch(Ctx &c, int mode, LaunchPlan &lp, bool allow_vec)
 {
     lp.kind = KERNEL_LANE;
-    if (allow_vec && ((mode == MODE_MSE && c.tune.kernel == KERNEL_VEC) ||
-                      (mode == MODE_MSE_REF && c.tune.cache_kernel == KERNEL_VEC)))
+    // (the vector interpreter addresses columns with 32-bit element offsets)
+    if (allow_vec && c.N <= (int64_t)UINT32_MAX &&
+        ((mode == MODE_MSE && c.tune.kernel == KERNEL_VEC) ||
+         (mode == MODE_MSE_REF && c.tune.cache_kernel == KERNEL_VEC)))
         lp.kind = KERNEL_VEC;
     lp.xc = pick_xc(c.nv);
     lp.dpt = mode == MODE_MSE_REF ? (c.tune.cache_dpt > 0 ? c.tune.cache_dpt : 3) : c.tune.dpt;

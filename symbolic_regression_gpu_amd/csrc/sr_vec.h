@@ -38,6 +38,15 @@ SR_DEV Ins fetch_ins(ConstIns *p, int i)
     return Ins{p[i].code, p[i].arg};
 }
 
+// A raw buffer resource over the memory at p (scalar registers): a load is
+// then one buffer instruction from a 32-bit per-lane byte offset.  Word 3 is
+// the gfx9-family raw-buffer format word; the record count is the maximum,
+// so no offset this kernel forms is clamped.
+SR_DEV __amdgpu_buffer_rsrc_t col_rsrc(const float *p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)0xffffffff, 0x00020000);
+}
+
 template <typename F>
 SR_DEV void vec_each(F &&f)
 {
@@ -121,15 +130,19 @@ __global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
         if (item >= items) break;
         const int chunk = item / n_exprs;
         const int rank = item - chunk * n_exprs;
-        const int e = a.order[rank];
-        ConstIns *prog =
-            (ConstIns *)(a.progs + (a.prog_off ? (int64_t)a.prog_off[e] : (int64_t)e * a.prog_stride));
-        const int n = a.prog_len[e];
+        // the item's expression, program and length are wave-uniform: held
+        // in scalar registers, so each instruction fetch is a scalar load
+        // from a scalar base
+        const int e = __builtin_amdgcn_readfirstlane(a.order[rank]);
+        const int64_t poff =
+            a.prog_off ? (int64_t)__builtin_amdgcn_readfirstlane(a.prog_off[e]) : (int64_t)e * a.prog_stride;
+        ConstIns *prog = (ConstIns *)(a.progs + poff);
+        const int n = __builtin_amdgcn_readfirstlane(a.prog_len[e]);
         const int64_t q0 = (int64_t)q * a.slice;
         const int64_t q1 = q0 + a.slice < N ? q0 + a.slice : N;
         const int64_t c0 = q0 + (int64_t)chunk * a.chunk;
         const int64_t c1 = c0 + a.chunk < q1 ? c0 + a.chunk : q1;
-        const bool need = !has_ref(MODE) || !a.need_mse || a.need_mse[e] != 0;
+        const bool need = !has_ref(MODE) || !a.need_mse || __builtin_amdgcn_readfirstlane(a.need_mse[e]) != 0;
         double sum = 0.0;
         long long cnt = 0;
 
@@ -137,20 +150,28 @@ __global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
             const int last = (int)((c1 - t0 < kVecTile ? c1 - t0 : kVecTile) - 1);
             const bool full = last == kVecTile - 1;
             // six coalesced loads per lane from a uniform base (datapoint
-            // t0 + 64 k + lane), clamped to the tile's last datapoint
+            // t0 + 64 k + lane), clamped to the tile's last datapoint: the
+            // byte offsets are set once per tile, so a load is one
+            // instruction (scalar base + vector offset) on either kind of tile
+            uint32_t offk[kVecD];
+            vec_each([&](int k) { offk[k] = (uint32_t)min(k * kWave + lane, last) * 4u; });
             auto load = [&](const float *base, float(&dst)[kVecD]) {
-                if (full) {
-                    vec_each([&](int k) { dst[k] = base[k * kWave + lane]; });
-                } else {
-                    vec_each([&](int k) { dst[k] = base[min(k * kWave + lane, last)]; });
-                }
+                const __amdgpu_buffer_rsrc_t r = col_rsrc(base);
+                vec_each([&](int k) {
+                    dst[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)offk[k], 0, 0));
+                });
             };
             // operand of an instruction: its variable / column, a constant,
-            // or nothing (a popped operand is read when the instruction runs)
+            // or nothing (a popped operand is read when the instruction runs);
+            // column arg starts arg * N elements past the tile's base (an
+            // unsigned 32 x 32-bit product: N < 2^32 for this kernel)
+            const float *xt = a.X + t0;
+            const float *ct = a.cache + t0;
             auto operand_ptr = [&](int code, int arg) -> const float * {
                 const int src = bc_src(code);
-                if (src == SRC_VAR) return a.X + (int64_t)arg * N + t0;
-                if (has_ref(MODE) && src == SRC_REF) return a.cache + (int64_t)arg * N + t0;
+                const uint64_t off = (uint64_t)(uint32_t)arg * (uint32_t)N;
+                if (src == SRC_VAR) return xt + off;
+                if (has_ref(MODE) && src == SRC_REF) return ct + off;
                 return nullptr;
             };
             float top[kVecD], o[kVecD], y[kVecD];
@@ -225,8 +246,8 @@ __global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
             });
         }
         if (need) {
-        sum = wave_sum(sum);
-        cnt = wave_sum(cnt);
+            sum = wave_sum(sum);
+            cnt = wave_sum(cnt);
         }
         if (lane == 0) {
             const int slot = q * a.chunks_per_queue + chunk;
