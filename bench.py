@@ -665,6 +665,8 @@ def main():
                 continue
             issue = {"achieved": v["valu_instr_per_s"], "peak": v["valu_issue_peak_per_s"],
                      "unit": "VALU wave64 instructions/s", "frac": v["valu_issue_frac"],
+                     # the scalar unit of a CU issues at most one instruction per cycle too
+                     "salu_frac": v.get("salu_issue_frac"),
                      "source": "profiles/valu_issue_bench_default.json (counter run)"}
             # a counter run describes the kernel it names, nothing else
             kk = k.replace(" ", "")
