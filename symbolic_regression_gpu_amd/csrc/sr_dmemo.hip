@@ -97,12 +97,58 @@ struct DmArgs {
     long long *stats;
 };
 
+// Token bodies compared / copied 8 tokens at a time: the loads of a chunk are
+// independent (indexes clamped to the body), so a long body costs one memory
+// latency per chunk instead of one per token.
+constexpr int kBodyChunk = 8;
+
+SR_DEV bool same_tokens(const int32_t *__restrict__ t1, const float *__restrict__ v1,
+                        const int32_t *__restrict__ t2, const float *__restrict__ v2, int n)
+{
+    bool same = true;
+    for (int j0 = 0; j0 < n && same; j0 += kBodyChunk) {
+        const int last = n - 1 - j0;
+        int32_t x[kBodyChunk], y[kBodyChunk];
+        uint32_t u[kBodyChunk], w[kBodyChunk];
+#pragma unroll
+        for (int k = 0; k < kBodyChunk; ++k) {
+            const int j = j0 + min(k, last);
+            x[k] = t1[j];
+            y[k] = t2[j];
+            u[k] = __float_as_uint(v1[j]);
+            w[k] = __float_as_uint(v2[j]);
+        }
+#pragma unroll
+        for (int k = 0; k < kBodyChunk; ++k) same = same && x[k] == y[k] && u[k] == w[k];
+    }
+    return same;
+}
+
+SR_DEV void copy_tokens(int32_t *__restrict__ td, float *__restrict__ vd, const int32_t *__restrict__ ts,
+                        const float *__restrict__ vs, int n)
+{
+    for (int j0 = 0; j0 < n; j0 += kBodyChunk) {
+        const int m = min(kBodyChunk, n - j0);
+        int32_t x[kBodyChunk];
+        float u[kBodyChunk];
+#pragma unroll
+        for (int k = 0; k < kBodyChunk; ++k) {
+            const int j = j0 + min(k, m - 1);
+            x[k] = ts[j];
+            u[k] = vs[j];
+        }
+#pragma unroll
+        for (int k = 0; k < kBodyChunk; ++k)
+            if (k < m) {
+                td[j0 + k] = x[k];
+                vd[j0 + k] = u[k];
+            }
+    }
+}
+
 SR_DEV bool same_body(const DmArgs &a, int64_t p, int64_t r, int n)
 {
-    for (int k = 0; k < n; ++k)
-        if (a.tok[p + k] != a.tok[r + k] || __float_as_uint(a.val[p + k]) != __float_as_uint(a.val[r + k]))
-            return false;
-    return true;
+    return same_tokens(a.tok + p, a.val + p, a.tok + r, a.val + r, n);
 }
 
 SR_DEV bool executed(const DmArgs &a, int64_t p)
@@ -160,45 +206,104 @@ __global__ void k_dm_clear_ct(DmArgs a)
     a.ct_prog[i] = -1;
 }
 
-// ---- 1. sizes and structural hashes (one wave per expression) ---------------
-// A non-commutative chain over (token, value bits, child hashes) through a
-// full 64-bit finaliser; 0 is reserved for empty table entries.  The stream
-// is staged in LDS and hashed right to left by lane 0 (the recursion is
-// serial), then written back by the whole wave.
+// ---- 1. sizes and subtree keys (one wave per expression) ---------------------
+// A subtree is the token span [i, i + size) of the prefix stream.  With
+// S[k] = sum over tokens before k of (1 - arity), its end is the first
+// j > i with S[j] = S[i] + 1; S rises by at most 1 per token, so a search
+// from S[j] < target may skip target - S[j] - 1 positions.  Its key is a
+// polynomial hash of the span: every token gets a 64-bit code, weighted by
+// Binv^k (B odd, Binv its inverse mod 2^64); the prefix sums U of the
+// weighted codes give sum_k code_k Binv^(k-i) = (U[end] - U[i]) * B^i,
+// which depends only on the span's contents, finalised with its length.
+// 0 is reserved for empty table entries.  Both prefixes are wave scans, so
+// nothing here is serial over the tokens.
+struct HashPowers {
+    uint64_t pw[MAX_EXPR_LEN + 1], ipw[MAX_EXPR_LEN + 1];
+};
+constexpr uint64_t kHashBase = 0xd6e8feb86659fd93ULL;  // odd
+
+constexpr HashPowers make_hash_powers()
+{
+    HashPowers h{};
+    uint64_t inv = kHashBase;  // Newton: each step doubles the correct low bits
+    for (int k = 0; k < 6; ++k) inv *= 2 - kHashBase * inv;
+    uint64_t p = 1, q = 1;
+    for (int k = 0; k <= MAX_EXPR_LEN; ++k) {
+        h.pw[k] = p;
+        h.ipw[k] = q;
+        p *= kHashBase;
+        q *= inv;
+    }
+    return h;
+}
+static_assert(make_hash_powers().pw[5] * make_hash_powers().ipw[5] == 1, "hash base inverse");
+__constant__ HashPowers kHashPowers = make_hash_powers();
+
+SR_DEV uint64_t token_code(int t, uint32_t v)
+{
+    return mix64((uint64_t)(uint32_t)t * 0x9e3779b97f4a7c15ULL ^ mix64((uint64_t)v + 0x632be59bd9b4e019ULL));
+}
+
+template <class T>
+SR_DEV T wave_excl_scan(T v, int lane)
+{
+    T inc = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const T u = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += u;
+    }
+    return inc - v;
+}
+
 __global__ __launch_bounds__(64) void k_dm_hash(DmArgs a)
 {
     extern __shared__ unsigned char lds[];
-    uint64_t *lh = (uint64_t *)lds;
-    int32_t *lt = (int32_t *)(lh + a.cap);
-    uint32_t *lv = (uint32_t *)(lt + a.cap);
-    int32_t *ls = (int32_t *)(lv + a.cap);
+    uint64_t *lu = (uint64_t *)lds;            // [cap + 1] U prefix
+    int32_t *ls = (int32_t *)(lu + a.cap + 1);  // [cap + 1] S prefix
     const int e = blockIdx.x, lane = threadIdx.x;
     const int64_t b = (int64_t)e * a.cap;
-    const int L = min(a.len[e], a.cap);
-    for (int k = lane; k < L; k += 64) {
-        lt[k] = a.tok[b + k];
-        lv[k] = __float_as_uint(a.val[b + k]);
+    const int L = max(0, min(a.len[e], a.cap));
+    // contiguous blocks per lane for the prefixes
+    const int per = (L + 63) / 64;
+    const int lo = min(L, lane * per), hi = min(L, lo + per);
+    int ds = 0;
+    uint64_t us = 0;
+    for (int k = lo; k < hi; ++k) {
+        const int t = a.tok[b + k];
+        const uint64_t w = token_code(t, __float_as_uint(a.val[b + k])) * kHashPowers.ipw[k];
+        ls[k] = 1 - tok_arity(t);
+        lu[k] = w;
+        ds += 1 - tok_arity(t);
+        us += w;
+    }
+    int dx = wave_excl_scan(ds, lane);
+    uint64_t ux = wave_excl_scan(us, lane);
+    for (int k = lo; k < hi; ++k) {
+        const int d = ls[k];
+        const uint64_t w = lu[k];
+        ls[k] = dx;
+        lu[k] = ux;
+        dx += d;
+        ux += w;
+    }
+    if (lane == 63) {  // the last block's running sums are the totals
+        ls[L] = dx;
+        lu[L] = ux;
     }
     __syncthreads();
-    if (lane == 0) {
-        for (int i = L - 1; i >= 0; --i) {
-            const int t = lt[i];
-            uint64_t x = mix64((uint64_t)(uint32_t)t * 0x9e3779b97f4a7c15ULL ^ mix64((uint64_t)lv[i] + 0x632be59bd9b4e019ULL));
-            int sz = 1, pos = i + 1;
-            for (int k = 0; k < tok_arity(t); ++k) {
-                if (pos >= L) break;
-                x = mix64(x * 0xbf58476d1ce4e5b9ULL + lh[pos] + (uint64_t)(k + 1));
-                sz += ls[pos];
-                pos += ls[pos];
-            }
-            lh[i] = x ? x : 1;
-            ls[i] = sz;
+    for (int i = lane; i < L; i += 64) {
+        const int target = ls[i] + 1;
+        int j = i + 1;
+        while (j < L) {
+            const int s = ls[j];
+            if (s >= target) break;
+            j += target - s;
         }
-    }
-    __syncthreads();
-    for (int k = lane; k < L; k += 64) {
-        a.hh[b + k] = lh[k];
-        a.sz[b + k] = ls[k];
+        j = min(j, L);
+        const int n = j - i;
+        const uint64_t x = mix64((lu[j] - lu[i]) * kHashPowers.pw[i] + (uint64_t)n * 0xbf58476d1ce4e5b9ULL);
+        a.hh[b + i] = x ? x : 1;
+        a.sz[b + i] = n;
     }
 }
 
@@ -219,10 +324,8 @@ __global__ void k_dm_lookup(DmArgs a)
         }
         if (k != h) continue;
         const int s = a.ht_slot[i];
-        bool same = a.s_len[s] == n;
         const int64_t sb = (int64_t)s * a.cap;
-        for (int j = 0; same && j < n; ++j)
-            same = a.s_tok[sb + j] == a.tok[p + j] && __float_as_uint(a.s_val[sb + j]) == __float_as_uint(a.val[p + j]);
+        const bool same = a.s_len[s] == n && same_tokens(a.s_tok + sb, a.s_val + sb, a.tok + p, a.val + p, n);
         if (same) {
             a.hit[p] = s;
             a.s_seen[s] = (uint32_t)a.ctr[C_GEN];
@@ -280,8 +383,16 @@ __global__ void k_dm_outer(DmArgs a)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (int64_t)a.E * a.cap || !a.sh[p]) return;
-    for (int k = 1; k < a.sz[p]; ++k)
-        if (a.sh[p + k]) atomicMin(&a.anc[p + k], (int32_t)p);
+    const int n = a.sz[p];
+    for (int k0 = 1; k0 < n; k0 += kBodyChunk) {
+        const int last = n - 1;
+        int32_t f[kBodyChunk];
+#pragma unroll
+        for (int k = 0; k < kBodyChunk; ++k) f[k] = a.sh[p + min(k0 + k, last)];
+#pragma unroll
+        for (int k = 0; k < kBodyChunk; ++k)
+            if (k0 + k < n && f[k]) atomicMin(&a.anc[p + k0 + k], (int32_t)p);
+    }
 }
 
 __global__ void k_dm_anc(DmArgs a)
@@ -402,10 +513,7 @@ SR_DEV void claim_slot(const DmArgs &a, int i, int slot, int64_t body)
     a.ct_slot[i] = slot;
     a.s_key[slot] = a.ct_key[i];
     a.s_len[slot] = n;
-    for (int k = 0; k < n; ++k) {
-        a.s_tok[(int64_t)slot * a.cap + k] = a.tok[body + k];
-        a.s_val[(int64_t)slot * a.cap + k] = a.val[body + k];
-    }
+    copy_tokens(a.s_tok + (int64_t)slot * a.cap, a.s_val + (int64_t)slot * a.cap, a.tok + body, a.val + body, n);
     a.s_live[slot] = 1;
     a.s_has[slot] = 0;
     a.s_seen[slot] = (uint32_t)a.ctr[C_GEN];
@@ -906,7 +1014,7 @@ int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *
     hipLaunchKernelGGL(k_dm_begin, dim3(1), dim3(64), 0, s, a);
     DM_LAUNCH(k_dm_clear_pos, big);
     DM_LAUNCH(k_dm_clear_ct, d.H2);
-    DM_WAVES(k_dm_hash, d.E, 5 * cap4);
+    DM_WAVES(k_dm_hash, d.E, 12 * ((size_t)d.cap + 1));
     DM_LAUNCH(k_dm_lookup, M);
     DM_LAUNCH(k_dm_cover, M);
     DM_LAUNCH(k_dm_count, M);
