@@ -259,6 +259,15 @@ int sr_grad_consts(sr_ctx *ctx, const sr_population *pop, const uint8_t *const_m
  * A test and inspection hook; no counterpart in the reference. */
 int sr_compile_expr(const int32_t *tokens, const float *values, int32_t len, int32_t nv,
                     int32_t *out_code, int32_t *out_arg, int32_t *stack_need);
+/* Compile one cache skeleton (OP_REF = -2 and OP_STORE = -3 tokens accepted,
+ * OP_STORE's value = destination column) with algorithm 0 = the serial
+ * compiler of sr_compile_expr, 1 = the wave-parallel compiler the
+ * device-planned cache runs on the GPU, here executed on the host lane by
+ * lane.  Both give the same bytecode.  Returns the instruction count or a
+ * negative sr_status.  A test and inspection hook; no counterpart in the
+ * reference. */
+int sr_compile_skeleton(const int32_t *tokens, const float *values, int32_t len, int32_t nv, int32_t algorithm,
+                        int32_t *out_code, int32_t *out_arg, int32_t *stack_need);
 /* Same, as the parametric program the adjoint pass runs: const_slot [len] is
  * -1 or the free-constant slot (0..7) of each token.  Code bit 15 marks
  * instructions whose value depends on a free constant, bits 20..30 hold 1 +

@@ -118,6 +118,7 @@ _SIGNATURES = {
     "sr_param_autotune": [_vp, _vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "sr_set_tuning": [_vp, C.POINTER(_Tuning)],
     "sr_compile_expr": [_i32p, _f32p, C.c_int32, C.c_int32, _i32p, _i32p, C.POINTER(C.c_int32)],
+    "sr_compile_skeleton": [_i32p, _f32p, C.c_int32, C.c_int32, C.c_int32, _i32p, _i32p, C.POINTER(C.c_int32)],
     "sr_compile_param_expr": [_i32p, _f32p, _i32p, C.c_int32, C.c_int32, _i32p, _i32p, _vp,
                               C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "sr_jit_source": [C.POINTER(_Pop), C.c_int32, C.c_int32, _vp, C.c_int64, C.POINTER(C.c_int64)],
@@ -223,6 +224,22 @@ def compile_expr(tokens, values, nv, const_slot=None):
     if n < 0:
         raise SrError(n, "compile failed")
     return code[:n], arg[:n], need.value, aux[:n], rows.value
+
+
+def compile_skeleton(tokens, values, nv, algorithm=1):
+    """Bytecode of one cache skeleton (OP_REF / OP_STORE tokens allowed):
+    (code int32[n], arg int32[n], stack_need).  algorithm 0 = the serial
+    compiler, 1 = the wave-parallel compiler of the device-planned cache (run
+    on the host)."""
+    t = np.ascontiguousarray(tokens, np.int32)
+    v = np.ascontiguousarray(values, np.float32)
+    code = np.empty(max(len(t), 1), np.int32)
+    arg = np.empty(max(len(t), 1), np.int32)
+    need = C.c_int32()
+    n = lib().sr_compile_skeleton(t, v, len(t), nv, algorithm, code, arg, C.byref(need))
+    if n < 0:
+        raise SrError(n, "compile failed")
+    return code[:n], arg[:n], need.value
 
 
 def jit_source(pop, nv: int, dpt: int = 4) -> str:

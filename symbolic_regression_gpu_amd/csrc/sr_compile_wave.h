@@ -1,0 +1,316 @@
+// sr_compile_wave.h -- the bytecode compiler of sr_format.h (compile_expr)
+// restated for one 64-lane wave: no step is serial over the tokens, so a
+// program of n tokens compiles in a few dozen dependent LDS round trips
+// instead of several per token.  The device cache planner compiles every
+// skeleton with it; the host runs the same code with the lanes taken one
+// after the other (the CPU tests compare it with compile_expr).
+//
+// The algorithm (every per-node step runs on all nodes at once):
+//   1. S[k] = sum over tokens before k of (1 - arity); the stream is valid iff
+//      every token is valid, S[n] = 1 and S[k] <= 0 for 0 < k < n.
+//   2. subtree size: the end of node i is the first j > i with
+//      S[j] = S[i] + 1 (S rises by at most 1 per token, so the search skips).
+//   3. stack need (Sethi-Ullman, as compile_expr): rounds over the nodes
+//      until every node whose children are known is known; rounds <= height.
+//   4. instruction count of a subtree = sum over its span of w (1 per
+//      operator, 2 for a binary operator of two leaves), a prefix sum W;
+//      the children evaluated first / second follow compile_expr's rule, the
+//      second one starts after the first one's instructions.
+//   5. start of every subtree's instructions = sum of those offsets along
+//      the path to the root, by pointer jumping (log2(height) rounds).
+//   6. each operator writes its instruction at start + count - 1 (a binary
+//      operator of two leaves also its LOAD at start); the first instruction
+//      of every second-evaluated subtree takes BC_PUSH.
+// The bytecode equals compile_expr's, instruction for instruction.
+#pragma once
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#endif
+
+#include "sr_format.h"
+
+namespace sr {
+
+constexpr int kWaveLanes = 64;
+
+// LDS (or host) scratch for programs of up to `cap` tokens.
+struct WaveCompileScratch {
+    int32_t *S;                        // [cap + 1]
+    int32_t *size, *need;              // [cap]
+    int32_t *W;                        // [cap + 1]
+    int32_t *acc0, *acc1, *up0, *up1;  // [cap]
+    int32_t *part;                     // [kWaveLanes]
+    int32_t *flag;                     // [4]
+};
+
+// Scratch ints needed for programs of up to cap tokens.
+SR_HD int wave_compile_scratch_ints(int cap) { return 8 * cap + 2 + kWaveLanes + 4; }
+
+SR_HD WaveCompileScratch wave_compile_scratch(int32_t *base, int cap)
+{
+    WaveCompileScratch s;
+    s.S = base;
+    s.size = s.S + cap + 1;
+    s.need = s.size + cap;
+    s.W = s.need + cap;
+    s.acc0 = s.W + cap + 1;
+    s.acc1 = s.acc0 + cap;
+    s.up0 = s.acc1 + cap;
+    s.up1 = s.up0 + cap;
+    s.part = s.up1 + cap;
+    s.flag = s.part + kWaveLanes;
+    return s;
+}
+
+// Lanes of one wave.  each(f) runs f(lane) on every lane, then makes every
+// lane's writes visible to all (on the device a workgroup barrier; on the
+// host the lanes simply run in order).  State carried from one step to the
+// next lives in the scratch arrays; values read outside each() are uniform.
+#if defined(__HIPCC__)
+struct DeviceWave {
+    template <class F>
+    __device__ void each(F &&f)
+    {
+        f((int)threadIdx.x);
+        __syncthreads();
+    }
+};
+#endif
+struct HostWave {
+    template <class F>
+    void each(F &&f)
+    {
+        for (int l = 0; l < kWaveLanes; ++l) f(l);
+    }
+};
+
+SR_HD int wave_block_lo(int n, int lane)
+{
+    const int per = (n + kWaveLanes - 1) / kWaveLanes;
+    return lane * per < n ? lane * per : n;
+}
+SR_HD int wave_block_hi(int n, int lane)
+{
+    const int per = (n + kWaveLanes - 1) / kWaveLanes;
+    const int hi = (lane + 1) * per;
+    return hi < n ? hi : n;
+}
+
+SR_HD int tok_delta(int t) { return is_leaf(t) ? 1 : (is_binary(t) ? -1 : 0); }
+
+// Compile tok/val[0..n) (OP_REF and OP_STORE accepted); out: [n].  Returns
+// the instruction count (uniform), or -1 for a malformed stream, and sets
+// *stack_need.  Same result as raw_stack_depth(tok, n, true) >= 0 followed
+// by compile_expr.
+template <class Wave>
+SR_HD int compile_wave(Wave &w, const int32_t *tok, const float *val, int n, int nv, WaveCompileScratch s,
+                       Ins *out, int *stack_need)
+{
+    if (n <= 0 || n > MAX_EXPR_LEN) return -1;
+    int32_t *S = s.S, *size = s.size, *need = s.need, *W = s.W, *part = s.part, *flag = s.flag;
+    // 1. validity and S
+    w.each([&](int lane) {
+        if (lane < 4) flag[lane] = 0;
+        int d = 0;
+        for (int k = wave_block_lo(n, lane); k < wave_block_hi(n, lane); ++k) d += tok_delta(tok[k]);
+        part[lane] = d;
+    });
+    w.each([&](int lane) {
+        int x = 0;
+        for (int l = 0; l < lane; ++l) x += part[l];
+        bool bad = false;
+        for (int k = wave_block_lo(n, lane); k < wave_block_hi(n, lane); ++k) {
+            const int t = tok[k];
+            S[k] = x;
+            bad = bad || (k > 0 && x > 0) || !(is_valid_token(t) || t == OP_STORE);
+            x += tok_delta(t);
+        }
+        if (lane == kWaveLanes - 1) {
+            S[n] = x;
+            bad = bad || x != 1;
+        }
+        if (bad) flag[0] = 1;
+    });
+    if (flag[0]) return -1;
+    // 2. sizes; leaves know their need
+    w.each([&](int lane) {
+        for (int i = lane; i < n; i += kWaveLanes) {
+            const int target = S[i] + 1;
+            int j = i + 1;
+            while (j < n) {
+                const int v = S[j];
+                if (v >= target) break;
+                j += target - v;
+            }
+            size[i] = (j < n ? j : n) - i;
+            need[i] = is_leaf(tok[i]) ? 0 : -1;
+        }
+    });
+    // 3. stack need, contiguous blocks from the right (a child follows its
+    // parent, so a block's chains settle in one round)
+    for (int round = 0; round <= n; ++round) {
+        const int f = 1 + (round & 1);
+        w.each([&](int lane) {
+            if (lane == 0) flag[3 - f] = 0;
+            bool pending = false;
+            for (int i = wave_block_hi(n, lane) - 1; i >= wave_block_lo(n, lane); --i) {
+                if (need[i] >= 0) continue;
+                const int t = tok[i];
+                const int a = i + 1;
+                int r = -1;
+                if (!is_binary(t)) {
+                    r = need[a];
+                } else {
+                    const int b = a + size[a];
+                    const bool la = is_leaf(tok[a]), lb = is_leaf(tok[b]);
+                    const int na = need[a], nb = need[b];
+                    if (la && lb) r = 0;
+                    else if (la) r = nb;
+                    else if (lb) r = na;
+                    else if (na >= 0 && nb >= 0) r = na != nb ? (na > nb ? na : nb) : na + 1;
+                }
+                if (r >= 0) need[i] = r;
+                else pending = true;
+            }
+            if (pending) flag[f] = 1;
+        });
+        if (!flag[f]) break;
+    }
+    *stack_need = need[0];
+    if (is_leaf(tok[0])) {  // n == 1
+        int src;
+        int32_t arg;
+        leaf_src(tok[0], val[0], nv, src, arg);
+        out[0] = {0 | (src << 8) | (CL_LOAD << BC_CLS_SHIFT), arg};
+        return 1;
+    }
+    // 4. instruction counts (prefix W); pointer-jumping state of every node
+    auto weight = [&](int i) -> int {
+        const int t = tok[i];
+        if (is_leaf(t)) return 0;
+        if (!is_binary(t)) return 1;
+        const int a = i + 1, b = a + size[a];
+        return is_leaf(tok[a]) && is_leaf(tok[b]) ? 2 : 1;
+    };
+    w.each([&](int lane) {
+        int x = 0;
+        for (int k = wave_block_lo(n, lane); k < wave_block_hi(n, lane); ++k) {
+            x += weight(k);
+            s.up0[k] = -1;
+            s.acc0[k] = 0;
+        }
+        part[lane] = x;
+    });
+    w.each([&](int lane) {
+        int x = 0;
+        for (int l = 0; l < lane; ++l) x += part[l];
+        for (int k = wave_block_lo(n, lane); k < wave_block_hi(n, lane); ++k) {
+            W[k] = x;
+            x += weight(k);
+        }
+        if (lane == kWaveLanes - 1) W[n] = x;
+    });
+    auto count = [&](int i) { return W[i + size[i]] - W[i]; };
+    // first / second evaluated child of node i (-1 when none)
+    auto children = [&](int i, int &first, int &second) {
+        first = second = -1;
+        const int t = tok[i], a = i + 1;
+        if (!is_binary(t)) {
+            if (!is_leaf(tok[a])) first = a;
+            return;
+        }
+        const int b = a + size[a];
+        const bool la = is_leaf(tok[a]), lb = is_leaf(tok[b]);
+        if (la && lb) return;
+        if (la) first = b;
+        else if (lb) first = a;
+        else {
+            const bool a_first = need[a] > need[b];
+            first = a_first ? a : b;
+            second = a_first ? b : a;
+        }
+    };
+    w.each([&](int lane) {
+        for (int i = lane; i < n; i += kWaveLanes) {
+            const int t = tok[i];
+            if (is_leaf(t)) continue;
+            if (t == OP_STORE && is_leaf(tok[i + 1])) flag[0] = 1;
+            int f, g;
+            children(i, f, g);
+            if (f >= 0) s.up0[f] = i;  // acc0[f] stays 0
+            if (g >= 0) {
+                s.up0[g] = i;
+                s.acc0[g] = count(f);
+            }
+        }
+    });
+    if (flag[0]) return -1;
+    // 5. subtree starts by pointer jumping
+    int32_t *acc = s.acc0, *up = s.up0, *acc2 = s.acc1, *up2 = s.up1;
+    for (int span = 1; span < n; span <<= 1) {
+        w.each([&](int lane) {
+            for (int i = lane; i < n; i += kWaveLanes) {
+                const int u = up[i];
+                acc2[i] = u >= 0 ? acc[i] + acc[u] : acc[i];
+                up2[i] = u >= 0 ? up[u] : -1;
+            }
+        });
+        int32_t *t0 = acc, *t1 = up;
+        acc = acc2;
+        up = up2;
+        acc2 = t0;
+        up2 = t1;
+    }
+    // 6. instructions, then the push bits
+    w.each([&](int lane) {
+        for (int i = lane; i < n; i += kWaveLanes) {
+            const int t = tok[i];
+            if (is_leaf(t)) continue;
+            const int start = acc[i], at = start + count(i) - 1;
+            const int a = i + 1;
+            int src;
+            int32_t arg;
+            if (!is_binary(t)) {
+                if (is_leaf(tok[a])) {
+                    leaf_src(tok[a], val[a], nv, src, arg);
+                    out[at] = {t | (src << 8) | (CL_UN << BC_CLS_SHIFT), arg};
+                } else if (t == OP_STORE) {
+                    out[at] = {STORE_CODE, (int32_t)val[i]};
+                } else {
+                    out[at] = {t | (SRC_NONE << 8) | (CL_UN << BC_CLS_SHIFT), 0};
+                }
+                continue;
+            }
+            const int b = a + size[a];
+            const bool la = is_leaf(tok[a]), lb = is_leaf(tok[b]);
+            const int bin = CL_BIN << BC_CLS_SHIFT;
+            if (la && lb) {
+                leaf_src(tok[a], val[a], nv, src, arg);
+                out[start] = {0 | (src << 8) | (CL_LOAD << BC_CLS_SHIFT), arg};
+                leaf_src(tok[b], val[b], nv, src, arg);
+                out[at] = {t | (src << 8) | bin, arg};
+            } else if (la) {
+                leaf_src(tok[a], val[a], nv, src, arg);
+                out[at] = {t | (src << 8) | (is_commutative(t) ? 0 : BC_SWAP) | bin, arg};
+            } else if (lb) {
+                leaf_src(tok[b], val[b], nv, src, arg);
+                out[at] = {t | (src << 8) | bin, arg};
+            } else {
+                const bool a_first = need[a] > need[b];
+                out[at] = {t | (SRC_POP << 8) | ((a_first && !is_commutative(t)) ? BC_SWAP : 0) | bin, 0};
+            }
+        }
+    });
+    w.each([&](int lane) {
+        for (int i = lane; i < n; i += kWaveLanes) {
+            if (!is_binary(tok[i])) continue;
+            int f, g;
+            children(i, f, g);
+            if (g >= 0) out[acc[g]].code |= BC_PUSH;
+        }
+    });
+    return W[n];
+}
+
+}  // namespace sr

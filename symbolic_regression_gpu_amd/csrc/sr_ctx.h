@@ -147,9 +147,8 @@ struct DMemoState {
     DevBuf ht_key, ht_slot;                                           // key -> slot
     DevBuf ct_key, ct_cnt, ct_rep, ct_root, ct_slot, ct_wr, ct_off, ct_prog;  // per generation
     DevBuf sz, hh, hit, cov, sh, anc, pslot, ctx_idx, flag, rank;     // per position [E*cap]
-    DevBuf p1_tok, p1_val, p1_len, p1_slot, p1_pos, p1_need, p1_ins, p1_plen, p1_order, p1_pf;
-    DevBuf p2_tok, p2_val, p2_len, p2_ins, p2_plen, p2_order, p2_pf, e_src;
-    DevBuf cs_size, cs_need, cs_frames;                               // compile scratch
+    DevBuf p1_slot, p1_pos, p1_need, p1_ins, p1_plen, p1_order, p1_pf;
+    DevBuf p2_ins, p2_plen, p2_order, p2_pf, e_src;
     DevBuf mse1, cnt1, mse2, cnt2, ctr, stats, blk;
 };
 

@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <climits>
 
+#include "sr_compile_wave.h"
 #include "sr_ctx.h"
 #include "sr_ga_launch.h"
 #include "sr_kernels.h"
@@ -81,16 +82,10 @@ struct DmArgs {
     int32_t *ht_slot;
     uint64_t *ct_key;
     int32_t *ct_cnt, *ct_rep, *ct_root, *ct_slot, *ct_wr, *ct_off, *ct_prog;
-    int32_t *p1_tok;
-    float *p1_val;
-    int32_t *p1_len, *p1_slot, *p1_pos, *p1_need;
-    int32_t *p2_tok;
-    float *p2_val;
-    int32_t *p2_len, *e_src;
+    int32_t *p1_slot, *p1_pos, *p1_need;
+    int32_t *e_src;
     Ins *p1_ins, *p2_ins;
     int32_t *p1_plen, *p2_plen, *p1_pf, *p2_pf;
-    int32_t *cs_size, *cs_need;
-    CompileFrame *cs_frames;
     float *mse1, *mse2, *fit;
     long long *cnt1, *cnt2;
     int32_t *ctr;
@@ -182,14 +177,10 @@ __global__ void k_dm_clear_pos(DmArgs a)
         a.flag[p] = 0;
     }
     if (p < a.L1) {
-        a.p1_len[p] = 0;
         a.p1_plen[p] = 0;
         a.p1_need[p] = 0;
     }
-    if (p < a.E) {
-        a.p2_len[p] = 0;
-        a.p2_plen[p] = 0;
-    }
+    if (p < a.E) a.p2_plen[p] = 0;
 }
 
 __global__ void k_dm_clear_ct(DmArgs a)
@@ -572,183 +563,183 @@ __global__ void k_dm_pslot(DmArgs a)
     a.pslot[p] = a.ct_slot[a.cidx[p]];
 }
 
-// ---- 6. skeleton programs -------------------------------------------------------
-SR_DEV void put(int32_t *t, float *v, int &k, int tok, float val)
+// ---- 6. skeleton programs, emitted and compiled ------------------------------
+// One wave per program (blocks < L1: launch-1 roots, then one per expression
+// for launch 2).  The source span is staged in LDS; the tokens the program
+// keeps follow from a scan: a position inside an OP_REF region (a hit, or
+// for launch 2 also a shared subtree with a slot) is dropped, a region's
+// first position becomes one OP_REF, a nested key this root writes is
+// preceded by an OP_STORE, and a launch-1 program starts with the OP_STORE
+// of its own column.  The skeleton is then compiled by the wave
+// (sr_compile_wave.h), its first OP_REF operands marked for prefetch.
+__global__ __launch_bounds__(64) void k_dm_program(DmArgs a, int nv)
 {
-    t[k] = tok;
-    v[k] = val;
-    ++k;
-}
-
-// Launch 1: one program per root with a slot, from its first outermost
-// occurrence: the root's column stored, nested keys stored where this root
-// is their writer, old columns read.  One wave per program: the occurrence
-// is staged in LDS, lane 0 emits, the wave writes the program out.
-__global__ __launch_bounds__(64) void k_dm_emit1(DmArgs a)
-{
-    extern __shared__ unsigned char lds[];
-    int32_t *lhit = (int32_t *)lds, *lst = lhit + a.cap, *lsz = lst + a.cap, *ltk = lsz + a.cap;
-    float *lvl = (float *)(ltk + a.cap);
-    int32_t *otk = (int32_t *)(lvl + a.cap);
-    float *ovl = (float *)(otk + a.PC);
-    __shared__ int on;
-    const int r = blockIdx.x, lane = threadIdx.x;
-    if (r >= a.ctr[C_L1]) return;
-    const int64_t P = a.p1_pos[r];
-    const int n = a.sz[P];
-    const int ri = a.cidx[P];
-    for (int k = lane; k < n; k += 64) {
-        const int64_t x = P + k;
-        lhit[k] = a.hit[x];
-        int st = -1;
-        if (k > 0 && a.sh[x] && a.pslot[x] >= 0) {
-            const int i = a.cidx[x];
-            if (a.ct_root[i] == kNone && a.ct_wr[i] == ri && a.ct_off[i] == k) st = a.pslot[x];
-        }
-        lst[k] = st;
-        lsz[k] = a.sz[x];
-        ltk[k] = a.tok[x];
-        lvl[k] = a.val[x];
-    }
-    __syncthreads();
-    if (lane == 0) {
-        int k = 0;
-        put(otk, ovl, k, OP_STORE, (float)a.p1_slot[r]);
-        for (int x = 0; x < n; ++x) {
-            if (x > 0) {
-                if (lhit[x] >= 0) {
-                    put(otk, ovl, k, OP_REF, (float)lhit[x]);
-                    x += lsz[x] - 1;
-                    continue;
-                }
-                if (lst[x] >= 0) put(otk, ovl, k, OP_STORE, (float)lst[x]);
-            }
-            put(otk, ovl, k, ltk[x], lvl[x]);
-        }
-        on = k;
-        a.p1_len[r] = k;
-    }
-    __syncthreads();
-    for (int k = lane; k < on; k += 64) {
-        a.p1_tok[(int64_t)r * a.PC + k] = otk[k];
-        a.p1_val[(int64_t)r * a.PC + k] = ovl[k];
-    }
-}
-
-// Launch 2: one program per expression that needs one (one wave each).
-__global__ __launch_bounds__(64) void k_dm_emit2(DmArgs a)
-{
-    extern __shared__ unsigned char lds[];
-    int32_t *lref = (int32_t *)lds, *lsz = lref + a.cap, *ltk = lsz + a.cap;
-    float *lvl = (float *)(ltk + a.cap);
-    int32_t *otk = (int32_t *)(lvl + a.cap);
-    float *ovl = (float *)(otk + a.cap);
-    __shared__ int on;
-    const int e = blockIdx.x, lane = threadIdx.x;
-    const int64_t b = (int64_t)e * a.cap;
-    const int L = min(a.len[e], a.cap);
-    const int root = a.hit[b];
-    if (root >= 0 && a.s_has[root]) {
-        if (lane == 0) a.e_src[e] = -2 - root;
-        return;
-    }
-    if (a.sh[b] && a.pslot[b] >= 0 && a.ct_prog[a.cidx[b]] >= 0) {
-        if (lane == 0) {
-            const int r = a.ct_prog[a.cidx[b]];
-            a.e_src[e] = kFromLaunch1 | r;
-            a.p1_need[r] = 1;  // this root's MSE is an expression's fitness
-        }
-        return;
-    }
-    for (int k = lane; k < L; k += 64) {
-        const int h = a.hit[b + k], ps = a.pslot[b + k];
-        lref[k] = h >= 0 ? h : ps;
-        lsz[k] = a.sz[b + k];
-        ltk[k] = a.tok[b + k];
-        lvl[k] = a.val[b + k];
-    }
-    __syncthreads();
-    if (lane == 0) {
-        int k = 0;
-        for (int x = 0; x < L; ++x) {
-            if (lref[x] >= 0) {
-                put(otk, ovl, k, OP_REF, (float)lref[x]);
-                x += lsz[x] - 1;
-                continue;
-            }
-            put(otk, ovl, k, ltk[x], lvl[x]);
-        }
-        on = k;
-        a.p2_len[e] = k;
-        a.e_src[e] = e;
-    }
-    __syncthreads();
-    for (int k = lane; k < on; k += 64) {
-        a.p2_tok[(int64_t)e * a.PC + k] = otk[k];
-        a.p2_val[(int64_t)e * a.PC + k] = ovl[k];
-    }
-}
-
-// Bytecode of both launches' programs (one wave per program, compile scratch
-// in LDS), the first OP_REF operands marked for loading a tile ahead, and
-// token statistics.
-__global__ __launch_bounds__(64) void k_dm_compile(DmArgs a, int nv)
-{
-    extern __shared__ unsigned char lds[];
-    CompileFrame *lfr = (CompileFrame *)lds;
-    Ins *lout = (Ins *)(lfr + a.PC + 1);
-    int32_t *lt = (int32_t *)(lout + a.PC);
-    float *lv = (float *)(lt + a.PC);
-    int32_t *lsize = (int32_t *)(lv + a.PC);
-    int32_t *lneed = lsize + a.PC;
-    __shared__ int on;
+    extern __shared__ int32_t lds32[];
+    const int PC = a.PC, cap = a.cap;
+    int32_t *ot = lds32;
+    float *ov = (float *)(ot + PC);
+    Ins *lout = (Ins *)(ov + PC);
+    int32_t *scr = (int32_t *)(lout + PC);
+    const WaveCompileScratch cs = wave_compile_scratch(scr, PC);
+    // emission scratch, dead before the compile reuses the area
+    int32_t *xref = scr, *xst = xref + cap, *xsz = xst + cap, *xtok = xsz + cap;
+    float *xval = (float *)(xtok + cap);
+    int32_t *cnt = (int32_t *)(xval + cap), *pa = cnt + cap + 1, *pb = pa + kWaveLanes;
+    __shared__ int total;
+    DeviceWave w;
     const int g = blockIdx.x, lane = threadIdx.x;
     const bool first = g < a.L1;
     const int idx = first ? g : g - a.L1;
-    const int L = first ? a.p1_len[idx] : a.p2_len[idx];
-    if (L <= 0) return;
-    const int64_t o = (int64_t)idx * a.PC;
-    const int32_t *t = (first ? a.p1_tok : a.p2_tok) + o;
-    const float *v = (first ? a.p1_val : a.p2_val) + o;
-    for (int k = lane; k < L; k += 64) {
-        lt[k] = t[k];
-        lv[k] = v[k];
+    int64_t P;
+    int ns;
+    if (first) {
+        if (g >= a.ctr[C_L1]) return;
+        P = a.p1_pos[g];
+        ns = a.sz[P];
+    } else {
+        const int e = idx;
+        P = (int64_t)e * cap;
+        const int root = a.hit[P];
+        if (root >= 0 && a.s_has[root]) {
+            if (lane == 0) a.e_src[e] = -2 - root;
+            return;
+        }
+        if (a.sh[P] && a.pslot[P] >= 0 && a.ct_prog[a.cidx[P]] >= 0) {
+            if (lane == 0) {
+                const int r = a.ct_prog[a.cidx[P]];
+                a.e_src[e] = kFromLaunch1 | r;
+                a.p1_need[r] = 1;  // this root's MSE is an expression's fitness
+            }
+            return;
+        }
+        if (lane == 0) a.e_src[e] = e;
+        ns = min(a.len[e], cap);
+        if (ns <= 0) return;
     }
-    __syncthreads();
-    if (lane == 0) {
-        int32_t *pf = (first ? a.p1_pf : a.p2_pf) + (int64_t)idx * kRefPrefetch;
-        int sn = 0;
-        int n = raw_stack_depth(lt, L, true) >= 0 ? compile_expr(lt, lv, L, nv, lsize, lneed, lfr, lout, &sn) : -1;
-        if (n < 0 || sn > kStackSlots) {
+    const int ri = first ? a.cidx[P] : 0;
+    w.each([&](int l) {
+        for (int x = l; x < ns; x += kWaveLanes) {
+            const int64_t q = P + x;
+            int ref = -1, st = -1;
+            if (!first) {
+                const int h = a.hit[q];
+                ref = h >= 0 ? h : a.pslot[q];
+            } else if (x > 0) {
+                ref = a.hit[q];
+                if (a.sh[q] && a.pslot[q] >= 0) {
+                    const int i = a.cidx[q];
+                    if (a.ct_root[i] == kNone && a.ct_wr[i] == ri && a.ct_off[i] == x) st = a.pslot[q];
+                }
+            }
+            xref[x] = ref;
+            xst[x] = st;
+            xsz[x] = a.sz[q];
+            xtok[x] = a.tok[q];
+            xval[x] = a.val[q];
+            cnt[x] = 0;
+        }
+        if (l == 0) cnt[ns] = 0;
+    });
+    // OP_REF regions: +1 after the region's first position, -1 at its end
+    w.each([&](int l) {
+        for (int x = l; x < ns; x += kWaveLanes)
+            if (xref[x] >= 0) {
+                atomicAdd(&cnt[x + 1], 1);
+                atomicSub(&cnt[min(x + xsz[x], ns)], 1);
+            }
+    });
+    w.each([&](int l) {
+        int o = 0;
+        for (int x = wave_block_lo(ns, l); x < wave_block_hi(ns, l); ++x) o += cnt[x];
+        pa[l] = o;
+    });
+    // tokens kept per position (0: inside a region, 2: OP_STORE + token)
+    w.each([&](int l) {
+        int open = 0, c = 0;
+        for (int k = 0; k < l; ++k) open += pa[k];
+        for (int x = wave_block_lo(ns, l); x < wave_block_hi(ns, l); ++x) {
+            open += cnt[x];
+            const int e = open > 0 ? 0 : (xref[x] >= 0 ? 1 : 1 + (xst[x] >= 0));
+            cnt[x] = e;
+            c += e;
+        }
+        pb[l] = c;
+    });
+    w.each([&](int l) {
+        int o = first ? 1 : 0;
+        for (int k = 0; k < l; ++k) o += pb[k];
+        for (int x = wave_block_lo(ns, l); x < wave_block_hi(ns, l); ++x) {
+            const int e = cnt[x];
+            if (e == 0) continue;
+            if (xref[x] >= 0) {
+                ot[o] = OP_REF;
+                ov[o] = (float)xref[x];
+                ++o;
+                continue;
+            }
+            if (e == 2) {
+                ot[o] = OP_STORE;
+                ov[o] = (float)xst[x];
+                ++o;
+            }
+            ot[o] = xtok[x];
+            ov[o] = xval[x];
+            ++o;
+        }
+        if (l == kWaveLanes - 1) total = o;
+        if (first && l == 0) {
+            ot[0] = OP_STORE;
+            ov[0] = (float)a.p1_slot[g];
+        }
+    });
+    const int m = total;
+    int sn = 0;
+    int n = compile_wave(w, ot, ov, m, nv, cs, lout, &sn);
+    if (n < 0 || sn > kStackSlots) {
+        if (lane == 0) {
             atomicOr(&a.ctr[C_STATUS], 1);
             lout[0].code = 0 | (SRC_CONST << 8);
             lout[0].arg = f32_bits(__builtin_nanf(""));
-            n = 1;
         }
-        int j = 0;
-        for (int i = 0; i < n && j < kRefPrefetch; ++i) {
-            if (bc_src(lout[i].code) != SRC_REF) continue;
-            lout[i].code |= (j + 1) << BC_PF_SHIFT;
-            pf[j++] = lout[i].arg;
+        n = 1;
+        __syncthreads();
+    }
+    // the first kRefPrefetch OP_REF operands, in program order
+    int32_t *pf = (first ? a.p1_pf : a.p2_pf) + (int64_t)idx * kRefPrefetch;
+    int found = 0;
+    for (int c0 = 0; c0 < n && found < kRefPrefetch; c0 += kWaveLanes) {
+        const int i = c0 + lane;
+        const bool isref = i < n && bc_src(lout[i].code) == SRC_REF;
+        const uint64_t mask = __ballot(isref);
+        const int rank = found + __popcll(mask & ((1ull << lane) - 1));
+        if (isref && rank < kRefPrefetch) {
+            lout[i].code |= (rank + 1) << BC_PF_SHIFT;
+            pf[rank] = lout[i].arg;
         }
-        for (; j < kRefPrefetch; ++j) pf[j] = -1;
+        found += __popcll(mask);
+    }
+    if (lane >= found && lane < kRefPrefetch) pf[lane] = -1;
+    __syncthreads();
+    Ins *out = (first ? a.p1_ins : a.p2_ins) + (int64_t)idx * PC;
+    for (int k = lane; k < n; k += kWaveLanes) out[k] = lout[k];
+    long long refs = 0, stores = 0, ops = 0;
+    for (int k = lane; k < m; k += kWaveLanes) {
+        const int t = ot[k];
+        refs += t == OP_REF;
+        stores += t == OP_STORE;
+        ops += t > 0;
+    }
+    refs = wave_sum(refs);
+    stores = wave_sum(stores);
+    ops = wave_sum(ops);
+    if (lane == 0) {
         (first ? a.p1_plen : a.p2_plen)[idx] = n;
-        on = n;
-        long long refs = 0, stores = 0, ops = 0;
-        for (int i = 0; i < L; ++i) {
-            refs += lt[i] == OP_REF;
-            stores += lt[i] == OP_STORE;
-            ops += lt[i] > 0;
-        }
-        atomicAdd((unsigned long long *)&a.stats[S_EXEC], (unsigned long long)L);
+        atomicAdd((unsigned long long *)&a.stats[S_EXEC], (unsigned long long)m);
         atomicAdd((unsigned long long *)&a.stats[S_REFS], (unsigned long long)refs);
         atomicAdd((unsigned long long *)&a.stats[S_STORES], (unsigned long long)stores);
         atomicAdd((unsigned long long *)&a.stats[S_OPS], (unsigned long long)ops);
         if (!first) atomicAdd(&a.ctr[C_L2], 1);
     }
-    __syncthreads();
-    Ins *out = (first ? a.p1_ins : a.p2_ins) + o;
-    for (int k = lane; k < on; k += 64) out[k] = lout[k];
 }
 
 // ---- fitness of every expression; MSEs of launch-1 roots remembered ---------------
@@ -855,15 +846,9 @@ DmArgs dm_args(Ctx &c, int b)
     a.ct_wr = d.ct_wr.as<int32_t>();
     a.ct_off = d.ct_off.as<int32_t>();
     a.ct_prog = d.ct_prog.as<int32_t>();
-    a.p1_tok = d.p1_tok.as<int32_t>();
-    a.p1_val = d.p1_val.as<float>();
-    a.p1_len = d.p1_len.as<int32_t>();
     a.p1_slot = d.p1_slot.as<int32_t>();
     a.p1_pos = d.p1_pos.as<int32_t>();
     a.p1_need = d.p1_need.as<int32_t>();
-    a.p2_tok = d.p2_tok.as<int32_t>();
-    a.p2_val = d.p2_val.as<float>();
-    a.p2_len = d.p2_len.as<int32_t>();
     a.e_src = d.e_src.as<int32_t>();
     a.p1_ins = d.p1_ins.as<Ins>();
     a.p2_ins = d.p2_ins.as<Ins>();
@@ -871,9 +856,6 @@ DmArgs dm_args(Ctx &c, int b)
     a.p2_plen = d.p2_plen.as<int32_t>();
     a.p1_pf = d.p1_pf.as<int32_t>();
     a.p2_pf = d.p2_pf.as<int32_t>();
-    a.cs_size = d.cs_size.as<int32_t>();
-    a.cs_need = d.cs_need.as<int32_t>();
-    a.cs_frames = d.cs_frames.as<CompileFrame>();
     a.mse1 = d.mse1.as<float>();
     a.mse2 = d.mse2.as<float>();
     a.cnt1 = d.cnt1.as<long long>();
@@ -946,22 +928,15 @@ int dmemo_setup(Ctx &c, int capacity, int min_size)
     for (DevBuf *bb : {&d.sz, &d.hit, &d.cov, &d.sh, &d.anc, &d.pslot, &d.ctx_idx, &d.flag, &d.rank})
         need(*bb, (size_t)M * 4);
     const int64_t P1 = (int64_t)d.L1 * PC, P2 = (int64_t)E * PC;
-    need(d.p1_tok, P1 * 4);
-    need(d.p1_val, P1 * 4);
     need(d.p1_ins, P1 * sizeof(Ins));
-    need(d.p2_tok, P2 * 4);
-    need(d.p2_val, P2 * 4);
     need(d.p2_ins, P2 * sizeof(Ins));
-    for (DevBuf *bb : {&d.p1_len, &d.p1_slot, &d.p1_pos, &d.p1_need, &d.p1_plen, &d.p1_order, &d.mse1})
+    for (DevBuf *bb : {&d.p1_slot, &d.p1_pos, &d.p1_need, &d.p1_plen, &d.p1_order, &d.mse1})
         need(*bb, (size_t)d.L1 * 4);
     need(d.cnt1, (size_t)d.L1 * 8);
     need(d.p1_pf, (size_t)d.L1 * kRefPrefetch * 4);
-    for (DevBuf *bb : {&d.p2_len, &d.p2_plen, &d.p2_order, &d.e_src, &d.mse2}) need(*bb, (size_t)E * 4);
+    for (DevBuf *bb : {&d.p2_plen, &d.p2_order, &d.e_src, &d.mse2}) need(*bb, (size_t)E * 4);
     need(d.cnt2, (size_t)E * 8);
     need(d.p2_pf, (size_t)E * kRefPrefetch * 4);
-    need(d.cs_size, (size_t)(d.L1 + E) * PC * 4);
-    need(d.cs_need, (size_t)(d.L1 + E) * PC * 4);
-    need(d.cs_frames, (size_t)(d.L1 + E) * (PC + 1) * sizeof(CompileFrame));
     need(d.ctr, C_COUNT * 4);
     need(d.blk, (size_t)(std::max<int64_t>(M, C) / 256 + 2) * 4);
     need(d.stats, S_COUNT * 8);
@@ -1010,7 +985,6 @@ int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *
         if ((he = hipGetLastError()) != hipSuccess) return c.hip_fail(he, "device cache scan");
         return SR_OK;
     };
-    const size_t cap4 = (size_t)d.cap * 4, pc4 = (size_t)d.PC * 4;
     hipLaunchKernelGGL(k_dm_begin, dim3(1), dim3(64), 0, s, a);
     DM_LAUNCH(k_dm_clear_pos, big);
     DM_LAUNCH(k_dm_clear_ct, d.H2);
@@ -1033,10 +1007,8 @@ int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *
     DM_LAUNCH(k_dm_ht_clear, d.H);
     DM_LAUNCH(k_dm_ht_insert, d.C);
     DM_LAUNCH(k_dm_pslot, M);
-    DM_WAVES(k_dm_emit1, d.L1, 5 * cap4 + 2 * pc4);
-    DM_WAVES(k_dm_emit2, d.E, 6 * cap4);
-    DM_WAVES(k_dm_compile, (int64_t)d.L1 + d.E,
-             (size_t)(d.PC + 1) * sizeof(CompileFrame) + (size_t)d.PC * sizeof(Ins) + 4 * pc4, c.nv);
+    DM_WAVES(k_dm_program, (int64_t)d.L1 + d.E,
+             (size_t)d.PC * (8 + sizeof(Ins)) + 4 * (size_t)wave_compile_scratch_ints(d.PC), c.nv);
 #undef DM_WAVES
 #undef DM_LAUNCH
     if ((he = launch_order_by_len(d.p1_plen.as<int32_t>(), d.L1, d.p1_order.as<int32_t>(), s)) != hipSuccess ||

@@ -13,6 +13,7 @@
 #include <numeric>
 
 #include "sr_adjoint_prog.h"
+#include "sr_compile_wave.h"
 #include "sr_ctx.h"
 #include "sr_kernels.h"
 #include "sr_launch.h"
@@ -612,6 +613,33 @@ int sr_compile_expr(const int32_t *tokens, const float *values, int32_t len, int
     int sn = 0;
     int n = compile_expr(tokens, values, len, nv, size.data(), need.data(), frames.data(), ins.data(),
                          &sn);
+    if (n < 0) return SR_ERR_BAD_EXPR;
+    for (int i = 0; i < n; ++i) {
+        out_code[i] = ins[i].code;
+        out_arg[i] = ins[i].arg;
+    }
+    *stack_need = sn;
+    return n;
+}
+
+int sr_compile_skeleton(const int32_t *tokens, const float *values, int32_t len, int32_t nv, int32_t algorithm,
+                        int32_t *out_code, int32_t *out_arg, int32_t *stack_need)
+{
+    if (!tokens || !values || !out_code || !out_arg || !stack_need || (algorithm != 0 && algorithm != 1))
+        return SR_ERR_INVALID_ARG;
+    if (len <= 0 || len > MAX_EXPR_LEN) return SR_ERR_BAD_EXPR;
+    std::vector<Ins> ins(len);
+    int sn = 0, n = -1;
+    if (algorithm == 0) {
+        if (raw_stack_depth(tokens, len, true) < 0) return SR_ERR_BAD_EXPR;
+        std::vector<int> size(len), need(len);
+        std::vector<CompileFrame> frames(len + 1);
+        n = compile_expr(tokens, values, len, nv, size.data(), need.data(), frames.data(), ins.data(), &sn);
+    } else {
+        std::vector<int32_t> scratch(wave_compile_scratch_ints(len));
+        HostWave w;
+        n = compile_wave(w, tokens, values, len, nv, wave_compile_scratch(scratch.data(), len), ins.data(), &sn);
+    }
     if (n < 0) return SR_ERR_BAD_EXPR;
     for (int i = 0; i < n; ++i) {
         out_code[i] = ins[i].code;
