@@ -137,7 +137,7 @@ struct DMemoState {
     bool planned = false;  // a generation was planned and launched since setup
     int C = 0;        // slots (value columns)
     int H = 0;        // lookup table entries (power of two >= 2C)
-    int H2 = 0;       // per-generation count table entries (power of two >= 2 * E * cap)
+    int H2 = 0;       // per-generation count table entries (power of two >= E * cap)
     int E = 0, cap = 0, PC = 0, L1 = 0;  // population shape, program stride, launch-1 capacity
     int64_t N = 0;
     uint64_t data_version = 0;
@@ -146,7 +146,7 @@ struct DMemoState {
     DevBuf s_key, s_len, s_tok, s_val, s_seen, s_live, s_mse, s_cnt, s_has, s_free;  // per slot
     DevBuf ht_key, ht_slot;                                           // key -> slot
     DevBuf ct_key, ct_cnt, ct_rep, ct_root, ct_slot, ct_wr, ct_off, ct_prog;  // per generation
-    DevBuf sz, hh, hit, cov, sh, anc, pslot, ctx_idx, flag, rank;     // per position [E*cap]
+    DevBuf sz, hh, hit, sh, anc, pslot, ctx_idx, flag, rank;          // per position [E*cap]
     DevBuf p1_slot, p1_pos, p1_need, p1_ins, p1_plen, p1_order, p1_pf;
     DevBuf p2_ins, p2_plen, p2_order, p2_pf, e_src;
     DevBuf mse1, cnt1, mse2, cnt2, ctr, stats, blk;

@@ -7,7 +7,7 @@
 // host work and is captured as one hipGraph per generation.
 //
 // Per generation (E expressions of stride cap, positions p = e * cap + q):
-//   1. sizes and 64-bit structural hashes of every subtree;
+//   1. sizes and 64-bit keys (span hashes) of every subtree;
 //   2. every subtree of >= min_size tokens is looked up among the slots; a
 //      hit (key and token body equal) will be read from its column;
 //   3. positions outside hits are "executed"; executed subtrees without a
@@ -69,7 +69,7 @@ struct DmArgs {
     int E, cap, PC, L1, C, H, H2, ms;
     int32_t *sz;
     uint64_t *hh;
-    int32_t *hit, *cov, *sh, *anc, *pslot, *cidx, *flag, *rank;
+    int32_t *hit, *sh, *anc, *pslot, *cidx, *flag, *rank;
     uint64_t *s_key;
     int32_t *s_len, *s_tok;
     float *s_val;
@@ -146,30 +146,17 @@ SR_DEV bool same_body(const DmArgs &a, int64_t p, int64_t r, int n)
     return same_tokens(a.tok + p, a.val + p, a.tok + r, a.val + r, n);
 }
 
-SR_DEV bool executed(const DmArgs &a, int64_t p)
+// ---- per generation: reset (counters, per-position and per-key state) -----------
+__global__ void k_dm_clear(DmArgs a)
 {
-    const int e = (int)(p / a.cap), q = (int)(p - (int64_t)e * a.cap);
-    if (q >= a.len[e] || a.cov[p]) return false;
-    const int r = a.hit[(int64_t)e * a.cap];
-    return !(r >= 0 && a.s_has[r]);  // a settled expression runs nothing
-}
-
-// ---- per generation: reset -------------------------------------------------
-__global__ void k_dm_begin(DmArgs a)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) {
         a.ctr[C_GEN] += 1;
         for (int k = 1; k < C_COUNT; ++k) a.ctr[k] = 0;
         a.stats[S_GENS] += 1;
     }
-}
-
-__global__ void k_dm_clear_pos(DmArgs a)
-{
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p < (int64_t)a.E * a.cap) {
         a.hit[p] = -3;
-        a.cov[p] = 0;
         a.sh[p] = 0;
         a.anc[p] = kNone;
         a.pslot[p] = -1;
@@ -181,20 +168,28 @@ __global__ void k_dm_clear_pos(DmArgs a)
         a.p1_need[p] = 0;
     }
     if (p < a.E) a.p2_plen[p] = 0;
+    if (p < a.H2) {
+        a.ct_key[p] = 0;
+        a.ct_cnt[p] = 0;
+        a.ct_rep[p] = kNone;
+        a.ct_root[p] = kNone;
+        a.ct_slot[p] = -1;
+        a.ct_wr[p] = -1;
+        a.ct_off[p] = 0;
+        a.ct_prog[p] = -1;
+    }
 }
 
-__global__ void k_dm_clear_ct(DmArgs a)
+// Exclusive prefix of a per-lane value over the wave's lanes, through LDS
+// (part: kWaveLanes ints; every lane must call it).
+SR_DEV int lane_prefix_sum(int32_t *part, int v, int lane)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.H2) return;
-    a.ct_key[i] = 0;
-    a.ct_cnt[i] = 0;
-    a.ct_rep[i] = kNone;
-    a.ct_root[i] = kNone;
-    a.ct_slot[i] = -1;
-    a.ct_wr[i] = -1;
-    a.ct_off[i] = 0;
-    a.ct_prog[i] = -1;
+    part[lane] = v;
+    __syncthreads();
+    int x = 0;
+    for (int k = 0; k < lane; ++k) x += part[k];
+    __syncthreads();
+    return x;
 }
 
 // ---- 1. sizes and subtree keys (one wave per expression) ---------------------
@@ -327,97 +322,150 @@ __global__ void k_dm_lookup(DmArgs a)
     }
 }
 
-// Interior of every hit is read from the column, not executed.
-__global__ void k_dm_cover(DmArgs a)
+// ---- 3. executed subtrees without a column, counted by key (one wave per
+// expression) -------------------------------------------------------------------
+// A position inside a hit's span is read from the column, not executed: a
+// difference array over the hits' spans (+1 after the first position, -1 at
+// the end), scanned.  An expression whose whole tree has a column and a
+// remembered MSE runs nothing.
+__global__ __launch_bounds__(64) void k_dm_count(DmArgs a)
 {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (int64_t)a.E * a.cap || a.hit[p] < 0) return;
-    for (int k = 1; k < a.sz[p]; ++k) a.cov[p + k] = 1;
-}
-
-// ---- 3. executed subtrees without a column, counted by key -------------------
-__global__ void k_dm_count(DmArgs a)
-{
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (int64_t)a.E * a.cap || a.hit[p] != -1 || !executed(a, p)) return;
-    const uint64_t h = a.hh[p];
-    for (uint32_t i = (uint32_t)mix64(h) & (a.H2 - 1);; i = (i + 1) & (a.H2 - 1)) {
-        uint64_t k = a.ct_key[i];
-        if (k == 0) {
-            const uint64_t prev = atomicCAS((unsigned long long *)&a.ct_key[i], 0ull, (unsigned long long)h);
-            k = prev == 0 ? h : prev;
+    extern __shared__ int32_t lds32[];
+    int32_t *cnt = lds32;  // [cap + 1]
+    __shared__ int32_t part[kWaveLanes];
+    const int e = blockIdx.x, lane = threadIdx.x;
+    const int64_t b = (int64_t)e * a.cap;
+    const int L = max(0, min(a.len[e], a.cap));
+    const int r0 = a.hit[b];
+    if (L == 0 || (r0 >= 0 && a.s_has[r0])) return;
+    for (int q = lane; q <= L; q += kWaveLanes) cnt[q] = 0;
+    __syncthreads();
+    for (int q = lane; q < L; q += kWaveLanes)
+        if (a.hit[b + q] >= 0) {
+            atomicAdd(&cnt[q + 1], 1);
+            atomicSub(&cnt[min(q + a.sz[b + q], L)], 1);
         }
-        if (k != h) continue;
-        atomicAdd(&a.ct_cnt[i], 1);
-        atomicMin(&a.ct_rep[i], (int32_t)p);
-        a.cidx[p] = (int32_t)i;
-        return;
+    __syncthreads();
+    const int lo = wave_block_lo(L, lane), hi = wave_block_hi(L, lane);
+    int t = 0;
+    for (int q = lo; q < hi; ++q) t += cnt[q];
+    int open = lane_prefix_sum(part, t, lane);
+    for (int q = lo; q < hi; ++q) {
+        open += cnt[q];
+        const int64_t p = b + q;
+        if (open > 0 || a.hit[p] != -1) continue;
+        const uint64_t h = a.hh[p];
+        for (uint32_t i = (uint32_t)mix64(h) & (a.H2 - 1);; i = (i + 1) & (a.H2 - 1)) {
+            uint64_t k = a.ct_key[i];
+            if (k == 0) {
+                const uint64_t prev = atomicCAS((unsigned long long *)&a.ct_key[i], 0ull, (unsigned long long)h);
+                k = prev == 0 ? h : prev;
+            }
+            if (k != h) continue;
+            atomicAdd(&a.ct_cnt[i], 1);
+            atomicMin(&a.ct_rep[i], (int32_t)p);
+            a.cidx[p] = (int32_t)i;
+            break;
+        }
     }
 }
 
+// ---- shared occurrences and their outermost shared ancestor (one wave per
+// expression) -------------------------------------------------------------------
 // An occurrence is "shared" when its key was executed at least twice and its
 // body equals the key's first occurrence (a hash collision stays inline).
-__global__ void k_dm_verify(DmArgs a)
+// Shared spans nest or are disjoint, so the outermost shared occurrence
+// holding a shared position is the last top-level shared start at or before
+// it (top-level: inside no other shared span, found by scanning a difference
+// array of the spans); a top-level occurrence makes its key a root.
+__global__ __launch_bounds__(64) void k_dm_share(DmArgs a)
 {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (int64_t)a.E * a.cap) return;
-    const int i = a.cidx[p];
-    if (i < 0 || a.ct_cnt[i] < 2) return;
-    const int64_t r = a.ct_rep[i];
-    a.sh[p] = (p == r || (a.sz[p] == a.sz[r] && same_body(a, p, r, a.sz[p]))) ? 1 : 0;
-}
-
-// Outermost shared occurrence holding each shared position: the smallest
-// position among its shared ancestors (ancestors precede descendants in
-// prefix order), or itself; an outermost occurrence makes its key a root.
-__global__ void k_dm_outer(DmArgs a)
-{
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (int64_t)a.E * a.cap || !a.sh[p]) return;
-    const int n = a.sz[p];
-    for (int k0 = 1; k0 < n; k0 += kBodyChunk) {
-        const int last = n - 1;
-        int32_t f[kBodyChunk];
-#pragma unroll
-        for (int k = 0; k < kBodyChunk; ++k) f[k] = a.sh[p + min(k0 + k, last)];
-#pragma unroll
-        for (int k = 0; k < kBodyChunk; ++k)
-            if (k0 + k < n && f[k]) atomicMin(&a.anc[p + k0 + k], (int32_t)p);
+    extern __shared__ int32_t lds32[];
+    int32_t *code = lds32;          // [cap]: 0 not shared, 1 shared top-level, 2 shared inside
+    int32_t *cnt = code + a.cap;    // [cap + 1]
+    __shared__ int32_t part[kWaveLanes];
+    const int e = blockIdx.x, lane = threadIdx.x;
+    const int64_t b = (int64_t)e * a.cap;
+    const int L = max(0, min(a.len[e], a.cap));
+    if (L == 0) return;
+    for (int q = lane; q < L; q += kWaveLanes) {
+        const int64_t p = b + q;
+        const int i = a.cidx[p];
+        int s = 0;
+        if (i >= 0 && a.ct_cnt[i] >= 2) {
+            const int64_t r = a.ct_rep[i];
+            s = (p == r || (a.sz[p] == a.sz[r] && same_body(a, p, r, a.sz[p]))) ? 1 : 0;
+        }
+        code[q] = s;
+        cnt[q] = 0;
+        if (s) a.sh[p] = 1;
+    }
+    if (lane == 0) cnt[L] = 0;
+    __syncthreads();
+    for (int q = lane; q < L; q += kWaveLanes)
+        if (code[q]) {
+            atomicAdd(&cnt[q + 1], 1);
+            atomicSub(&cnt[min(q + a.sz[b + q], L)], 1);
+        }
+    __syncthreads();
+    const int lo = wave_block_lo(L, lane), hi = wave_block_hi(L, lane);
+    int t = 0;
+    for (int q = lo; q < hi; ++q) t += cnt[q];
+    int open = lane_prefix_sum(part, t, lane);
+    int last = -1;
+    for (int q = lo; q < hi; ++q) {
+        open += cnt[q];
+        if (code[q]) {
+            code[q] = open > 0 ? 2 : 1;
+            if (open == 0) last = q;
+        }
+    }
+    // last top-level start before this lane's block (a running max)
+    part[lane] = last;
+    __syncthreads();
+    int prev = -1;
+    for (int k = 0; k < lane; ++k) prev = max(prev, part[k]);
+    for (int q = lo; q < hi; ++q) {
+        const int c = code[q];
+        if (c == 0) continue;
+        const int64_t p = b + q;
+        if (c == 1) {
+            prev = q;
+            a.anc[p] = (int32_t)p;
+            atomicMin(&a.ct_root[a.cidx[p]], (int32_t)p);
+        } else {
+            a.anc[p] = (int32_t)(b + prev);
+        }
     }
 }
 
-__global__ void k_dm_anc(DmArgs a)
-{
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (int64_t)a.E * a.cap || !a.sh[p]) return;
-    if (a.anc[p] == kNone) {
-        a.anc[p] = (int32_t)p;
-        atomicMin(&a.ct_root[a.cidx[p]], (int32_t)p);
-    }
-}
-
-// A shared key with no outermost occurrence is written inside the program of
-// the root holding its first occurrence, at that occurrence's offset.
-__global__ void k_dm_writer(DmArgs a)
+// ---- 4. keys, eviction ---------------------------------------------------------------
+// Per key (i < H2): a root key flags its outermost first occurrence; a shared
+// key with no outermost occurrence is written inside the program of the root
+// holding its first occurrence, at that occurrence's offset.  Per slot
+// (i < C): slots whose subtree the population no longer holds are evicted.
+// The slot lookup table (i < H) is cleared for its rebuild.
+__global__ void k_dm_keys(DmArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.H2 || a.ct_cnt[i] < 2 || a.ct_root[i] != kNone) return;
-    const int32_t r = a.ct_rep[i];
-    const int32_t A = a.anc[r];
-    if (A == kNone) return;
-    a.ct_wr[i] = a.cidx[A];
-    a.ct_off[i] = r - A;
-}
-
-// ---- 4. eviction and the free list ------------------------------------------------
-__global__ void k_dm_evict(DmArgs a)
-{
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.C) return;
-    if (a.s_live[s] && a.s_seen[s] != (uint32_t)a.ctr[C_GEN]) {
-        a.s_live[s] = 0;
-        a.s_has[s] = 0;
+    if (i < a.H2 && a.ct_cnt[i] >= 2) {
+        const int32_t root = a.ct_root[i];
+        if (root != kNone) {
+            a.flag[root] = 1;
+        } else {
+            const int32_t r = a.ct_rep[i];
+            const int32_t A = a.anc[r];
+            if (A != kNone) {
+                a.ct_wr[i] = a.cidx[A];
+                a.ct_off[i] = r - A;
+            }
+        }
     }
+    if (i < a.C && a.s_live[i] && a.s_seen[i] != (uint32_t)a.ctr[C_GEN]) {
+        a.s_live[i] = 0;
+        a.s_has[i] = 0;
+    }
+    if (i < a.H) a.ht_key[i] = 0;
 }
 
 // Ordered compaction in three passes (256 items per block): count the
@@ -487,15 +535,12 @@ __global__ __launch_bounds__(256) void k_dm_scan_write(DmArgs a, int kind, int v
 }
 
 // ---- 5. slots for roots, then for nested shared keys ------------------------------
-__global__ void k_dm_flag(DmArgs a, int roots)
+// A nested key gets a slot only when its writer root got one.
+__global__ void k_dm_nested(DmArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.H2 || a.ct_cnt[i] < 2) return;
-    if (roots) {
-        if (a.ct_root[i] != kNone) a.flag[a.ct_root[i]] = 1;
-    } else if (a.ct_root[i] == kNone && a.ct_wr[i] >= 0 && a.ct_slot[a.ct_wr[i]] >= 0) {
-        a.flag[a.ct_rep[i]] = 2;
-    }
+    if (a.ct_root[i] == kNone && a.ct_wr[i] >= 0 && a.ct_slot[a.ct_wr[i]] >= 0) a.flag[a.ct_rep[i]] = 2;
 }
 
 SR_DEV void claim_slot(const DmArgs &a, int i, int slot, int64_t body)
@@ -537,30 +582,21 @@ __global__ void k_dm_alloc(DmArgs a, int roots)
     }
 }
 
-__global__ void k_dm_ht_clear(DmArgs a)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < a.H) a.ht_key[i] = 0;
-}
-
-__global__ void k_dm_ht_insert(DmArgs a)
-{
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.C || !a.s_live[s]) return;
-    const uint64_t h = a.s_key[s];
-    for (uint32_t i = (uint32_t)mix64(h) & (a.H - 1);; i = (i + 1) & (a.H - 1)) {
-        if (atomicCAS((unsigned long long *)&a.ht_key[i], 0ull, (unsigned long long)h) == 0ull) {
-            a.ht_slot[i] = s;
-            return;
-        }
-    }
-}
-
-__global__ void k_dm_pslot(DmArgs a)
+// Slot lookup table rebuilt from the live slots (i < C); every shared
+// position learns its key's slot (i < E * cap).
+__global__ void k_dm_index(DmArgs a)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (int64_t)a.E * a.cap || !a.sh[p]) return;
-    a.pslot[p] = a.ct_slot[a.cidx[p]];
+    if (p < a.C && a.s_live[p]) {
+        const uint64_t h = a.s_key[p];
+        for (uint32_t i = (uint32_t)mix64(h) & (a.H - 1);; i = (i + 1) & (a.H - 1)) {
+            if (atomicCAS((unsigned long long *)&a.ht_key[i], 0ull, (unsigned long long)h) == 0ull) {
+                a.ht_slot[i] = (int32_t)p;
+                break;
+            }
+        }
+    }
+    if (p < (int64_t)a.E * a.cap && a.sh[p]) a.pslot[p] = a.ct_slot[a.cidx[p]];
 }
 
 // ---- 6. skeleton programs, emitted and compiled ------------------------------
@@ -777,6 +813,7 @@ __global__ void k_dm_gather(DmArgs a, long long *cnt_out)
     }
 }
 
+// Empty slots and an empty slot lookup table (grid over max(C, H)).
 __global__ void k_dm_init_slots(DmArgs a)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -784,6 +821,7 @@ __global__ void k_dm_init_slots(DmArgs a)
         for (int k = 0; k < C_COUNT; ++k) a.ctr[k] = 0;
         for (int k = 0; k < S_COUNT; ++k) a.stats[k] = 0;
     }
+    if (s < a.H) a.ht_key[s] = 0;
     if (s >= a.C) return;
     a.s_live[s] = 0;
     a.s_has[s] = 0;
@@ -819,7 +857,6 @@ DmArgs dm_args(Ctx &c, int b)
     a.sz = d.sz.as<int32_t>();
     a.hh = d.hh.as<uint64_t>();
     a.hit = d.hit.as<int32_t>();
-    a.cov = d.cov.as<int32_t>();
     a.sh = d.sh.as<int32_t>();
     a.anc = d.anc.as<int32_t>();
     a.pslot = d.pslot.as<int32_t>();
@@ -897,7 +934,7 @@ int dmemo_setup(Ctx &c, int capacity, int min_size)
     d.L1 = 2 * E;
     d.C = C;
     d.H = pow2_at_least(2 * (int64_t)C);
-    d.H2 = pow2_at_least(2 * M);
+    d.H2 = pow2_at_least(M);
     d.N = c.N;
     d.data_version = c.data_version;
     d.min_size = min_size;
@@ -925,7 +962,7 @@ int dmemo_setup(Ctx &c, int capacity, int min_size)
         need(*bb, (size_t)d.H2 * 4);
     need(d.ct_key, (size_t)d.H2 * 8);
     need(d.hh, (size_t)M * 8);
-    for (DevBuf *bb : {&d.sz, &d.hit, &d.cov, &d.sh, &d.anc, &d.pslot, &d.ctx_idx, &d.flag, &d.rank})
+    for (DevBuf *bb : {&d.sz, &d.hit, &d.sh, &d.anc, &d.pslot, &d.ctx_idx, &d.flag, &d.rank})
         need(*bb, (size_t)M * 4);
     const int64_t P1 = (int64_t)d.L1 * PC, P2 = (int64_t)E * PC;
     need(d.p1_ins, P1 * sizeof(Ins));
@@ -945,8 +982,7 @@ int dmemo_setup(Ctx &c, int capacity, int min_size)
     int rc = plan_batched(c, MODE_MSE_REF, std::max(d.L1, E), lp);
     if (rc) return rc;
     DmArgs a = dm_args(c, 0);
-    hipLaunchKernelGGL(k_dm_init_slots, dim3(grid_of(C)), dim3(kThreads), 0, c.stream, a);
-    hipLaunchKernelGGL(k_dm_ht_clear, dim3(grid_of(d.H)), dim3(kThreads), 0, c.stream, a);
+    hipLaunchKernelGGL(k_dm_init_slots, dim3(grid_of(std::max(C, d.H))), dim3(kThreads), 0, c.stream, a);
     if ((he = hipGetLastError()) == hipSuccess) he = hipStreamSynchronize(c.stream);
     if (he != hipSuccess) return c.hip_fail(he, "device cache init");
     d.ready = true;
@@ -985,28 +1021,20 @@ int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *
         if ((he = hipGetLastError()) != hipSuccess) return c.hip_fail(he, "device cache scan");
         return SR_OK;
     };
-    hipLaunchKernelGGL(k_dm_begin, dim3(1), dim3(64), 0, s, a);
-    DM_LAUNCH(k_dm_clear_pos, big);
-    DM_LAUNCH(k_dm_clear_ct, d.H2);
+    const int64_t all_keys = std::max<int64_t>(d.H2, std::max(d.C, d.H));
+    DM_LAUNCH(k_dm_clear, std::max<int64_t>(big, d.H2));
     DM_WAVES(k_dm_hash, d.E, 12 * ((size_t)d.cap + 1));
     DM_LAUNCH(k_dm_lookup, M);
-    DM_LAUNCH(k_dm_cover, M);
-    DM_LAUNCH(k_dm_count, M);
-    DM_LAUNCH(k_dm_verify, M);
-    DM_LAUNCH(k_dm_outer, M);
-    DM_LAUNCH(k_dm_anc, M);
-    DM_LAUNCH(k_dm_writer, d.H2);
-    DM_LAUNCH(k_dm_evict, d.C);
+    DM_WAVES(k_dm_count, d.E, 4 * ((size_t)d.cap + 1));
+    DM_WAVES(k_dm_share, d.E, 4 * (2 * (size_t)d.cap + 1));
+    DM_LAUNCH(k_dm_keys, all_keys);
     if (int r = scan(1, 0, C_FREE)) return r;
-    DM_LAUNCH(k_dm_flag, d.H2, 1);
     if (int r = scan(0, 1, C_ROOTS)) return r;
     DM_LAUNCH(k_dm_alloc, d.H2, 1);
-    DM_LAUNCH(k_dm_flag, d.H2, 0);
+    DM_LAUNCH(k_dm_nested, d.H2);
     if (int r = scan(0, 2, C_NESTED)) return r;
     DM_LAUNCH(k_dm_alloc, d.H2, 0);
-    DM_LAUNCH(k_dm_ht_clear, d.H);
-    DM_LAUNCH(k_dm_ht_insert, d.C);
-    DM_LAUNCH(k_dm_pslot, M);
+    DM_LAUNCH(k_dm_index, std::max<int64_t>(d.C, M));
     DM_WAVES(k_dm_program, (int64_t)d.L1 + d.E,
              (size_t)d.PC * (8 + sizeof(Ins)) + 4 * (size_t)wave_compile_scratch_ints(d.PC), c.nv);
 #undef DM_WAVES

@@ -86,10 +86,14 @@ struct Rng {
         c3 = domain;
         word = 0;
     }
+    // word w of the stream is output w & 3 of Philox block w >> 2
     SR_HD uint32_t next()
     {
-        if ((word & 3u) == 0u) philox4x32_10(word >> 2, c1, c2, c3, k0, k1, buf);
-        return buf[word++ & 3u];
+        const uint32_t j = word & 3u;
+        if (j == 0u) philox4x32_10(word >> 2, c1, c2, c3, k0, k1, buf);
+        ++word;
+        // selects, not a dynamic index: keeps buf in registers on the device
+        return j == 0u ? buf[0] : (j == 1u ? buf[1] : (j == 2u ? buf[2] : buf[3]));
     }
     // [lo, hi): lo + (hi - lo) * u with u = top 24 bits / 2^24; each operation
     // rounds once (the build disables contraction).
@@ -130,22 +134,38 @@ SR_HD int ga_subtree_size(const int32_t *tok, int len, int start)
     return pos - start;
 }
 
+// Pending-child stack of the grow generator (entries: depth, or -1 for the
+// top-level call).
+constexpr int GA_GEN_STACK = 2 * 64 + 4;
+
+struct GaArrayStack {
+    int16_t a[GA_GEN_STACK];
+    int sp = 0;
+    SR_HD void push(int v) { a[sp++] = (int16_t)v; }
+    SR_HD int pop() { return a[--sp]; }
+};
+struct GaParamOps {
+    const GaParams &p;
+    SR_HD int unary(int i) const { return p.unary_ops[i]; }
+    SR_HD int binary(int i) const { return p.binary_ops[i]; }
+};
+
 // Grow generator (utils.cpp:995-1108) writing at most cap tokens into
 // tok/val.  Returns the length, or -1 if cap would be exceeded.  With store
 // false it only draws (the lanes of a wave that run the same generator and
-// leave the writes to one lane).
-SR_HD int ga_generate(const GaParams &p, int budget, Rng &r, int32_t *tok, float *val, int cap, bool store = true)
+// leave the writes to one lane).  Stack and Ops hold the pending children and
+// the operator tables (GaArrayStack / GaParamOps, or their per-wave forms).
+template <class R, class Stack, class Ops>
+SR_HD __attribute__((always_inline)) int ga_generate_with(const GaParams &p, int budget, R &r, int32_t *tok, float *val, int cap, bool store,
+                           Stack &stack, const Ops &ops)
 {
     if (budget <= 0) return 0;
     int left = p.max_tokens < budget ? p.max_tokens : budget;
     const int nvars = p.num_vars > 1 ? p.num_vars : 1;
     int n = 0;
-    // Pending child slots: depth, or -1 for the top-level call.
-    int16_t stack[2 * 64 + 4];
-    int sp = 0;
-    stack[sp++] = -1;
-    while (sp > 0) {
-        int d = stack[--sp];
+    stack.push(-1);
+    while (stack.sp > 0) {
+        int d = stack.pop();
         const bool top = d < 0;
         if (top) d = 0;
         if (left <= 0) {
@@ -179,10 +199,10 @@ SR_HD int ga_generate(const GaParams &p, int budget, Rng &r, int32_t *tok, float
         const bool unary = r.uniform(0.0f, 1.0f) < thresh;
         int op, arity;
         if (unary) {
-            op = p.unary_ops[r.randint(0, p.n_unary - 1)];
+            op = ops.unary(r.randint(0, p.n_unary - 1));
             arity = 1;
         } else {
-            op = p.binary_ops[r.randint(0, p.n_binary - 1)];
+            op = ops.binary(r.randint(0, p.n_binary - 1));
             arity = 2;
         }
         if (n >= cap) return -1;
@@ -192,8 +212,8 @@ SR_HD int ga_generate(const GaParams &p, int budget, Rng &r, int32_t *tok, float
         }
         ++n;
         --left;
-        if (sp + arity > (int)(sizeof(stack) / sizeof(stack[0]))) return -1;
-        for (int k = 0; k < arity; ++k) stack[sp++] = (int16_t)(d + 1);
+        if (stack.sp + arity > GA_GEN_STACK) return -1;
+        for (int k = 0; k < arity; ++k) stack.push(d + 1);
     }
     if (n == 0) {
         if (store) {
@@ -203,6 +223,12 @@ SR_HD int ga_generate(const GaParams &p, int budget, Rng &r, int32_t *tok, float
         n = 1;
     }
     return n;
+}
+
+SR_HD int ga_generate(const GaParams &p, int budget, Rng &r, int32_t *tok, float *val, int cap, bool store = true)
+{
+    GaArrayStack stack;
+    return ga_generate_with(p, budget, r, tok, val, cap, store, stack, GaParamOps{p});
 }
 
 SR_HD int ga_copy(const int32_t *st, const float *sv, int len, int32_t *dt, float *dv)
@@ -314,17 +340,92 @@ __device__ inline void ga_wave_copy(const int32_t *st, const float *sv, int len,
     }
 }
 
-__device__ inline void ga_wave_stage(const int32_t *src, int len, int32_t *lds, int lane)
+// A parent's subtree extents: the wave computes S[k] = sum over tokens before
+// k of (1 - ga_arity) into LDS; the subtree at `start` ends at the first
+// j > start with S[j] = S[start] + 1 (the end ga_subtree_size's count
+// reaches), and since S rises by at most 1 per token the search may skip
+// target - S[j] - 1 positions.
+__device__ inline void ga_wave_stage(const int32_t *src, int len, int32_t *S, int lane)
 {
-    for (int k = lane; k < len; k += 64) lds[k] = src[k];
+    len = len > 0 ? len : 0;
+    const int per = (len + 63) / 64;
+    const int lo = min(len, lane * per), hi = min(len, lo + per);
+    int d = 0;
+    for (int k = lo; k < hi; ++k) d += 1 - ga_arity(src[k]);
+    int inc = d;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += u;
+    }
+    int x = inc - d;
+    for (int k = lo; k < hi; ++k) {
+        S[k] = x;
+        x += 1 - ga_arity(src[k]);
+    }
+    if (lane == 63) S[len] = x;
     __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's LDS writes done
     __builtin_amdgcn_wave_barrier();
 }
 
+__device__ inline int ga_subtree_size_s(const int32_t *S, int len, int start)
+{
+    if (start < 0 || start >= len) return 0;
+    const int target = S[start] + 1;
+    int j = start + 1;
+    while (j < len) {
+        const int v = S[j];
+        if (v >= target) break;
+        j += target - v;
+    }
+    return min(j, len) - start;
+}
+
+// ga_tournament with the candidates drawn first (the same draws in the same
+// order) and their fitness loaded 8 at a time, so a tournament waits for a
+// few memory round trips instead of two per candidate.
+template <class R>
+__device__ inline int ga_tournament_wave(R &r, const float *fit, const int32_t *order, int pop, int t)
+{
+    constexpr int B = 8;
+    int best = r.randint(0, pop - 1);
+    float fb = fit[order[best]];
+    for (int k0 = 0; k0 < t - 1; k0 += B) {
+        const int m = min(B, t - 1 - k0);
+        int cand[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) cand[k] = k < m ? r.randint(0, pop - 1) : cand[0];
+        int who[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) who[k] = order[cand[k]];
+        float f[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) f[k] = fit[who[k]];
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            if (k < m && f[k] < fb) {
+                best = cand[k];
+                fb = f[k];
+            }
+    }
+    return best;
+}
+
+// LDS ints per wave of k_ga_next: two prefix rows and the generator stack.
+SR_HD int ga_wave_lds_ints(int cap) { return 2 * (cap + 1) + (GA_GEN_STACK + 1) / 2; }
+
+// The generator's stack in LDS shared by the wave's lanes (every lane pushes
+// the same values).
+struct GaLdsStack {
+    int16_t *a;
+    int sp = 0;
+    __device__ void push(int v) { a[sp++] = (int16_t)v; }
+    __device__ int pop() { return a[--sp]; }
+};
+
 __device__ inline int ga_child_wave(const GaParams &p, const SelectParams &s, int pop, int cap,
                                     const int32_t *tok, const float *val, const int32_t *len, const float *fit,
                                     const int32_t *order, uint64_t seed, uint32_t gen, int c, int32_t *ct,
-                                    float *cv, int32_t *l1, int32_t *l2, int lane)
+                                    float *cv, int32_t *l1, int32_t *l2, int16_t *gstack, int lane)
 {
     if (c < s.elites) {
         const int e = order[c];
@@ -333,15 +434,18 @@ __device__ inline int ga_child_wave(const GaParams &p, const SelectParams &s, in
     }
     Rng r;
     r.init(seed, (uint32_t)c, gen, GA_DOMAIN_NEXT);
-    const int b1 = order[ga_tournament(r, fit, order, pop, s.tournament)];
+    const int b1 = order[ga_tournament_wave(r, fit, order, pop, s.tournament)];
     const int32_t *pt = tok + (size_t)b1 * cap;
     const float *pv = val + (size_t)b1 * cap;
     const int plen = len[b1];
     ga_wave_stage(pt, plen, l1, lane);
     if (r.uniform(0.0f, 1.0f) < s.p_mutation) {  // ga_mutate
-        if (plen <= 0) return ga_generate(p, p.max_tokens, r, ct, cv, cap, lane == 0);
+        if (plen <= 0) {
+            GaLdsStack st{gstack};
+            return ga_generate_with(p, p.max_tokens, r, ct, cv, cap, lane == 0, st, GaParamOps{p});
+        }
         const int idx = r.randint(0, plen - 1);
-        int old = ga_subtree_size(l1, plen, idx);
+        int old = ga_subtree_size_s(l1, plen, idx);
         if (old <= 0) old = 1;
         const int base = plen - old;
         const int max_new = p.max_tokens - base;
@@ -352,12 +456,13 @@ __device__ inline int ga_child_wave(const GaParams &p, const SelectParams &s, in
         ga_wave_copy(pt, pv, idx, ct, cv, lane);
         const int tail = plen - (idx + old);
         const int room = cap - idx - tail;
-        const int nl = ga_generate(p, max_new, r, ct + idx, cv + idx, room, lane == 0);
+        GaLdsStack st{gstack};
+        const int nl = ga_generate_with(p, max_new, r, ct + idx, cv + idx, room, lane == 0, st, GaParamOps{p});
         if (nl < 0) return -1;
         ga_wave_copy(pt + idx + old, pv + idx + old, tail, ct + idx + nl, cv + idx + nl, lane);
         return idx + nl + tail;
     }
-    const int b2 = order[ga_tournament(r, fit, order, pop, s.tournament)];
+    const int b2 = order[ga_tournament_wave(r, fit, order, pop, s.tournament)];
     const int32_t *rt = tok + (size_t)b2 * cap;
     const float *rv = val + (size_t)b2 * cap;
     const int rlen = len[b2];
@@ -368,12 +473,12 @@ __device__ inline int ga_child_wave(const GaParams &p, const SelectParams &s, in
         return plen;
     }
     const int li = r.randint(0, plen - 1);
-    int lsub = ga_subtree_size(l1, plen, li);
+    int lsub = ga_subtree_size_s(l1, plen, li);
     if (lsub <= 0) lsub = 1;
     const int attempts = rlen < 32 ? rlen : 32;
     for (int a = 0; a < attempts; ++a) {
         const int ri = r.randint(0, rlen - 1);
-        const int rsub = ga_subtree_size(l2, rlen, ri);
+        const int rsub = ga_subtree_size_s(l2, rlen, ri);
         if (rsub <= 0) continue;
         const int total = plen - lsub + rsub;
         if (total > p.max_tokens) continue;

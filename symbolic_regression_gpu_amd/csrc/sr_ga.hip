@@ -43,8 +43,9 @@ __global__ __launch_bounds__(256) void k_ga_rank(const float *fit, int n, int32_
     if (lane == 0) order[rank] = i;
 }
 
-// One wave per child of generation *gen (parents staged in LDS: two rows of
-// cap tokens per wave, dynamic shared memory).
+// One wave per child of generation *gen (per wave in dynamic LDS: the
+// parents' subtree-extent prefixes, two rows of cap + 1, and the generator's
+// stack).
 __global__ __launch_bounds__(256) void k_ga_next(GaParams p, SelectParams s, int pop, int cap, const int32_t *tok,
                                                  const float *val, const int32_t *len, const float *fit,
                                                  const int32_t *order, uint64_t seed, const uint32_t *gen,
@@ -54,9 +55,10 @@ __global__ __launch_bounds__(256) void k_ga_next(GaParams p, SelectParams s, int
     const int w = threadIdx.x / 64, lane = threadIdx.x & 63;
     const int c = (int)(blockIdx.x * (blockDim.x / 64)) + w;
     if (c >= pop) return;
-    int32_t *l1 = ga_lds + (size_t)w * 2 * cap, *l2 = l1 + cap;
+    int32_t *l1 = ga_lds + (size_t)w * ga_wave_lds_ints(cap), *l2 = l1 + cap + 1;
+    int16_t *gstack = (int16_t *)(l2 + cap + 1);
     int n = ga_child_wave(p, s, pop, cap, tok, val, len, fit, order, seed, *gen, c, otok + (size_t)c * cap,
-                          oval + (size_t)c * cap, l1, l2, lane);
+                          oval + (size_t)c * cap, l1, l2, gstack, lane);
     if (lane == 0) {
         if (n < 0) {
             atomicOr((unsigned *)status, 2u);
@@ -188,7 +190,7 @@ hipError_t launch_ga_next(const GaParams &p, const SelectParams &sp, int pop, in
                           const int32_t *order, uint64_t seed, const uint32_t *gen, int32_t *otok,
                           float *oval, int32_t *olen, int32_t *status, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ga_next, dim3(blocks(pop, 4)), dim3(256), (size_t)4 * 2 * cap * sizeof(int32_t), s, p,
+    hipLaunchKernelGGL(k_ga_next, dim3(blocks(pop, 4)), dim3(256), (size_t)4 * ga_wave_lds_ints(cap) * sizeof(int32_t), s, p,
                        sp, pop, cap, tok, val, len, fit, order, seed, gen, otok, oval, olen, status);
     return hipGetLastError();
 }
