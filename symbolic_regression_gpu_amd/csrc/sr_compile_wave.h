@@ -34,23 +34,28 @@ namespace sr {
 
 constexpr int kWaveLanes = 64;
 
-// LDS (or host) scratch for programs of up to `cap` tokens.
+// LDS (or host) scratch for programs of up to `cap` tokens.  Per-node values
+// are 16-bit (|S|, sizes, counts, starts and indexes are <= MAX_EXPR_LEN), so
+// more programs fit in LDS at once.
 struct WaveCompileScratch {
-    int32_t *S;                        // [cap + 1]
-    int32_t *size, *need;              // [cap]
-    int32_t *W;                        // [cap + 1]
-    int32_t *acc0, *acc1, *up0, *up1;  // [cap]
     int32_t *part;                     // [kWaveLanes]
     int32_t *flag;                     // [4]
+    int16_t *S;                        // [cap + 1]
+    int16_t *size, *need;              // [cap]
+    int16_t *W;                        // [cap + 1]
+    int16_t *acc0, *acc1, *up0, *up1;  // [cap]
 };
+static_assert(MAX_EXPR_LEN < 32768, "16-bit compile scratch");
 
 // Scratch ints needed for programs of up to cap tokens.
-SR_HD int wave_compile_scratch_ints(int cap) { return 8 * cap + 2 + kWaveLanes + 4; }
+SR_HD int wave_compile_scratch_ints(int cap) { return kWaveLanes + 4 + (8 * cap + 2 + 1) / 2; }
 
 SR_HD WaveCompileScratch wave_compile_scratch(int32_t *base, int cap)
 {
     WaveCompileScratch s;
-    s.S = base;
+    s.part = base;
+    s.flag = s.part + kWaveLanes;
+    s.S = (int16_t *)(s.flag + 4);
     s.size = s.S + cap + 1;
     s.need = s.size + cap;
     s.W = s.need + cap;
@@ -58,8 +63,6 @@ SR_HD WaveCompileScratch wave_compile_scratch(int32_t *base, int cap)
     s.acc1 = s.acc0 + cap;
     s.up0 = s.acc1 + cap;
     s.up1 = s.up0 + cap;
-    s.part = s.up1 + cap;
-    s.flag = s.part + kWaveLanes;
     return s;
 }
 
@@ -108,7 +111,8 @@ SR_HD int compile_wave(Wave &w, const int32_t *tok, const float *val, int n, int
                        Ins *out, int *stack_need)
 {
     if (n <= 0 || n > MAX_EXPR_LEN) return -1;
-    int32_t *S = s.S, *size = s.size, *need = s.need, *W = s.W, *part = s.part, *flag = s.flag;
+    int16_t *S = s.S, *size = s.size, *need = s.need, *W = s.W;
+    int32_t *part = s.part, *flag = s.flag;
     // 1. validity and S
     w.each([&](int lane) {
         if (lane < 4) flag[lane] = 0;
@@ -247,7 +251,7 @@ SR_HD int compile_wave(Wave &w, const int32_t *tok, const float *val, int n, int
     });
     if (flag[0]) return -1;
     // 5. subtree starts by pointer jumping
-    int32_t *acc = s.acc0, *up = s.up0, *acc2 = s.acc1, *up2 = s.up1;
+    int16_t *acc = s.acc0, *up = s.up0, *acc2 = s.acc1, *up2 = s.up1;
     for (int span = 1; span < n; span <<= 1) {
         w.each([&](int lane) {
             for (int i = lane; i < n; i += kWaveLanes) {
@@ -256,7 +260,7 @@ SR_HD int compile_wave(Wave &w, const int32_t *tok, const float *val, int n, int
                 up2[i] = u >= 0 ? up[u] : -1;
             }
         });
-        int32_t *t0 = acc, *t1 = up;
+        int16_t *t0 = acc, *t1 = up;
         acc = acc2;
         up = up2;
         acc2 = t0;

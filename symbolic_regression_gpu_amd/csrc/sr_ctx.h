@@ -149,7 +149,7 @@ struct DMemoState {
     DevBuf sz, hh, hit, sh, anc, pslot, ctx_idx, flag, rank;          // per position [E*cap]
     DevBuf p1_slot, p1_pos, p1_need, p1_ins, p1_plen, p1_order, p1_pf;
     DevBuf p2_ins, p2_plen, p2_order, p2_pf, e_src;
-    DevBuf mse1, cnt1, mse2, cnt2, ctr, stats, blk;
+    DevBuf mse1, cnt1, ctr, stats, blk;
 };
 
 // Device-resident GA population (double-buffered, fixed stride cap) and the
@@ -298,6 +298,8 @@ struct EvalBufs {
     long long *cnt;
     const int *fin_count = nullptr;  // finalize only programs below this device count
     float *spill = nullptr;          // vector interpreter stack spill (plan kind KERNEL_VEC)
+    bool counters_zeroed = false;    // the caller zeroed the queue counters already
+    bool finalize = true;            // MSE modes: reduce the partials (else the caller does)
 };
 // Enqueue counter reset + batched kernel (+ finalize for MODE_MSE).  a must
 // carry the program / order / output pointers; the rest is filled in here.
@@ -329,7 +331,10 @@ int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *
 int dmemo_stats(Ctx &c, long long out[9]);
 // The two interpreter launches of the last planned generation again (same
 // columns, same partial sums); ev: optional {start1, end1, start2, end2}.
-int dmemo_launch(Ctx &c, hipStream_t s, hipEvent_t *ev = nullptr);
+// Within dmemo_enqueue (planned = true) the queue counters are already zero
+// and launch 2's partials are reduced by the fitness gather (*P_out: the
+// partials per program).
+int dmemo_launch(Ctx &c, hipStream_t s, hipEvent_t *ev = nullptr, bool planned = false, int *P_out = nullptr);
 
 }  // namespace sr
 

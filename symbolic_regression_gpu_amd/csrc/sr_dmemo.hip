@@ -43,6 +43,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int32_t kNone = INT_MAX;
 constexpr int32_t kFromLaunch1 = 0x40000000;  // e_src: launch-1 program r is kFromLaunch1 | r
+constexpr int kQueueCounters = 64;             // queue counters per launch (c.d_counters holds two sets)
 
 // counters (int32): generation, free slots, launch-1 programs, nested slots
 // given, launch-2 programs, roots found, nested found, status
@@ -86,8 +87,8 @@ struct DmArgs {
     int32_t *e_src;
     Ins *p1_ins, *p2_ins;
     int32_t *p1_plen, *p2_plen, *p1_pf, *p2_pf;
-    float *mse1, *mse2, *fit;
-    long long *cnt1, *cnt2;
+    float *mse1, *fit;
+    long long *cnt1;
     int32_t *ctr;
     long long *stats;
 };
@@ -779,10 +780,14 @@ __global__ __launch_bounds__(64) void k_dm_program(DmArgs a, int nv)
 }
 
 // ---- fitness of every expression; MSEs of launch-1 roots remembered ---------------
-__global__ void k_dm_gather(DmArgs a, long long *cnt_out)
+// One wave per item.  A launch-2 program's MSE is reduced here from its
+// partials, in k_finalize's order (lane-strided sums, the same shuffle tree),
+// so it equals the batched evaluation's bit for bit.
+__global__ __launch_bounds__(256) void k_dm_gather(DmArgs a, long long *cnt_out, const double *psum,
+                                                   const long long *pcnt, int P)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < a.L1 && e < a.ctr[C_L1] && a.p1_need[e]) {
+    const int e = (int)((blockIdx.x * 256u + threadIdx.x) / 64), lane = threadIdx.x & 63;
+    if (lane == 0 && e < a.L1 && e < a.ctr[C_L1] && a.p1_need[e]) {
         // only roots that were whole expressions had their MSE computed
         const int s = a.p1_slot[e];
         a.s_mse[s] = a.mse1[e];
@@ -800,9 +805,18 @@ __global__ void k_dm_gather(DmArgs a, long long *cnt_out)
         m = a.mse1[src & ~kFromLaunch1];
         c = a.cnt1[src & ~kFromLaunch1];
     } else {
-        m = a.mse2[e];
-        c = a.cnt2[e];
+        double sum = 0.0;
+        long long n = 0;
+        for (int k = lane; k < P; k += 64) {
+            sum += psum[(int64_t)e * P + k];
+            n += pcnt[(int64_t)e * P + k];
+        }
+        sum = wave_sum(sum);
+        n = wave_sum(n);
+        m = n > 0 ? (float)(sum / (double)n) : qnan();
+        c = n;
     }
+    if (lane != 0) return;
     a.fit[e] = m;
     if (cnt_out) cnt_out[e] = c;
     atomicAdd((unsigned long long *)&a.stats[S_LOGICAL], (unsigned long long)a.len[e]);
@@ -894,9 +908,7 @@ DmArgs dm_args(Ctx &c, int b)
     a.p1_pf = d.p1_pf.as<int32_t>();
     a.p2_pf = d.p2_pf.as<int32_t>();
     a.mse1 = d.mse1.as<float>();
-    a.mse2 = d.mse2.as<float>();
     a.cnt1 = d.cnt1.as<long long>();
-    a.cnt2 = d.cnt2.as<long long>();
     a.fit = c.d_mse.as<float>();
     a.ctr = d.ctr.as<int32_t>();
     a.stats = d.stats.as<long long>();
@@ -971,8 +983,7 @@ int dmemo_setup(Ctx &c, int capacity, int min_size)
         need(*bb, (size_t)d.L1 * 4);
     need(d.cnt1, (size_t)d.L1 * 8);
     need(d.p1_pf, (size_t)d.L1 * kRefPrefetch * 4);
-    for (DevBuf *bb : {&d.p2_plen, &d.p2_order, &d.e_src, &d.mse2}) need(*bb, (size_t)E * 4);
-    need(d.cnt2, (size_t)E * 8);
+    for (DevBuf *bb : {&d.p2_plen, &d.p2_order, &d.e_src}) need(*bb, (size_t)E * 4);
     need(d.p2_pf, (size_t)E * kRefPrefetch * 4);
     need(d.ctr, C_COUNT * 4);
     need(d.blk, (size_t)(std::max<int64_t>(M, C) / 256 + 2) * 4);
@@ -1035,16 +1046,23 @@ int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *
     if (int r = scan(0, 2, C_NESTED)) return r;
     DM_LAUNCH(k_dm_alloc, d.H2, 0);
     DM_LAUNCH(k_dm_index, std::max<int64_t>(d.C, M));
-    DM_WAVES(k_dm_program, (int64_t)d.L1 + d.E,
-             (size_t)d.PC * (8 + sizeof(Ins)) + 4 * (size_t)wave_compile_scratch_ints(d.PC), c.nv);
+    // LDS: skeleton tokens and values, bytecode, then the larger of the
+    // compile scratch and the emission scratch that it reuses
+    const int scratch = std::max(wave_compile_scratch_ints(d.PC), 6 * d.cap + 1 + 2 * kWaveLanes);
+    DM_WAVES(k_dm_program, (int64_t)d.L1 + d.E, (size_t)d.PC * (8 + sizeof(Ins)) + 4 * (size_t)scratch, c.nv);
 #undef DM_WAVES
 #undef DM_LAUNCH
-    if ((he = launch_order_by_len(d.p1_plen.as<int32_t>(), d.L1, d.p1_order.as<int32_t>(), s)) != hipSuccess ||
-        (he = launch_order_by_len(d.p2_plen.as<int32_t>(), d.E, d.p2_order.as<int32_t>(), s)) != hipSuccess)
+    // both launches' program orders; their queue counters zeroed
+    if ((he = launch_order_by_len2(d.p1_plen.as<int32_t>(), d.L1, d.p1_order.as<int32_t>(),
+                                   d.p2_plen.as<int32_t>(), d.E, d.p2_order.as<int32_t>(),
+                                   c.d_counters.as<uint32_t>(), 2 * kQueueCounters, s)) != hipSuccess)
         return c.hip_fail(he, "device cache order");
-    int rc = dmemo_launch(c, s, ev);
+    int P = 0;
+    int rc = dmemo_launch(c, s, ev, true, &P);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_dm_gather, dim3(grid_of(std::max(d.L1, d.E))), dim3(kThreads), 0, s, a, cnt_out);
+    const int waves = std::max(d.L1, d.E);
+    hipLaunchKernelGGL(k_dm_gather, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, a, cnt_out,
+                       (const double *)c.d_psum.as<double>(), (const long long *)c.d_pcnt.as<long long>(), P);
     if ((he = hipGetLastError()) != hipSuccess) return c.hip_fail(he, "k_dm_gather");
     d.planned = true;
     return SR_OK;
@@ -1054,13 +1072,15 @@ int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *
 // again recomputes the same columns and partial sums (columns hold exactly
 // what the interpreter computes, with every kernel variant), which the
 // skeleton-kernel tuner relies on.
-int dmemo_launch(Ctx &c, hipStream_t s, hipEvent_t *ev)
+int dmemo_launch(Ctx &c, hipStream_t s, hipEvent_t *ev, bool planned, int *P_out)
 {
     DMemoState &d = c.dm;
     if (!d.ready) return c.fail(SR_ERR_INVALID_ARG, "device cache not set up");
     LaunchPlan lp;
     int rc = plan_launch(c, MODE_MSE_REF, lp);
     if (rc) return rc;
+    if (lp.nq > kQueueCounters) return c.fail(SR_ERR_UNSUPPORTED, "device cache: too many queues");
+    if (P_out) *P_out = lp.P;
     float *cols = d.cols.as<float>();
     // launch 1: the roots (count on the device); launch 2: the expressions,
     // numbered by expression, so every expression's MSE slot is finalized
@@ -1077,11 +1097,14 @@ int dmemo_launch(Ctx &c, hipStream_t s, hipEvent_t *ev)
         ba.out = cols;
         ba.n_exprs_dev = d.ctr.as<int32_t>() + (one ? C_L1 : C_L2);
         ba.need_mse = one ? d.p1_need.as<int32_t>() : nullptr;
-        EvalBufs bufs{c.d_psum.as<double>(), c.d_pcnt.as<long long>(), c.d_counters.as<uint32_t>(),
-                      one ? d.mse1.as<float>() : d.mse2.as<float>(),
-                      one ? d.cnt1.as<long long>() : d.cnt2.as<long long>()};
+        // each launch its own queue counters
+        EvalBufs bufs{c.d_psum.as<double>(), c.d_pcnt.as<long long>(),
+                      c.d_counters.as<uint32_t>() + (one ? 0 : kQueueCounters), d.mse1.as<float>(),
+                      d.cnt1.as<long long>()};
         bufs.fin_count = one ? ba.n_exprs_dev : nullptr;
         bufs.spill = c.d_spill.as<float>();
+        bufs.counters_zeroed = planned;
+        bufs.finalize = one;  // launch 2's partials: reduced by k_dm_gather (or unused by the tuner)
         rc = enqueue_batched(c, MODE_MSE_REF, lp, ba, one ? d.L1 : d.E, s, bufs, ev ? ev[2 * launch] : nullptr,
                              ev ? ev[2 * launch + 1] : nullptr);
         if (rc) return rc;

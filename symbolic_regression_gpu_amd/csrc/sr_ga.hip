@@ -53,7 +53,9 @@ __global__ __launch_bounds__(256) void k_ga_next(GaParams p, SelectParams s, int
 {
     extern __shared__ int32_t ga_lds[];
     const int w = threadIdx.x / 64, lane = threadIdx.x & 63;
-    const int c = (int)(blockIdx.x * (blockDim.x / 64)) + w;
+    // the child index is wave-uniform; saying so keeps the whole decision
+    // path (Philox, tournaments, the generator) on the scalar unit
+    const int c = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / 64)) + w);
     if (c >= pop) return;
     int32_t *l1 = ga_lds + (size_t)w * ga_wave_lds_ints(cap), *l2 = l1 + cap + 1;
     int16_t *gstack = (int16_t *)(l2 + cap + 1);
@@ -105,8 +107,9 @@ __global__ void k_compile_pop(const int32_t *tok, const float *val, const int32_
     atomicMax(max_need, sn);
 }
 
-// Longest program first (single workgroup counting sort, lengths < 4096).
-__global__ __launch_bounds__(1024) void k_order_by_len(const int32_t *plen, int n, int32_t *order)
+// Longest program first (one workgroup of 1024 threads: counting sort,
+// lengths < 4096).
+__device__ void order_by_len_block(const int32_t *plen, int n, int32_t *order)
 {
     constexpr int B = 4096;
     __shared__ int hist[B];
@@ -149,14 +152,36 @@ __global__ __launch_bounds__(1024) void k_order_by_len(const int32_t *plen, int 
     }
 }
 
-// hist[(*gen - gen0) * pop + i] = src[i] for 4-byte words (generations
-// outside [gen0, gen0 + max_gens) are dropped).
-__global__ void k_record(const uint32_t *src, int pop, uint32_t *hist, const uint32_t *gen,
-                         uint32_t gen0, int max_gens)
+__global__ __launch_bounds__(1024) void k_order_by_len(const int32_t *plen, int n, int32_t *order)
+{
+    order_by_len_block(plen, n, order);
+}
+
+// Two orders in one launch (a workgroup each); the first workgroup also
+// zeroes n_zero words (the persistent launches' queue counters).
+__global__ __launch_bounds__(1024) void k_order_by_len2(const int32_t *plen0, int n0, int32_t *order0,
+                                                        const int32_t *plen1, int n1, int32_t *order1,
+                                                        uint32_t *zero, int n_zero)
+{
+    if (blockIdx.x == 0) {
+        for (int k = threadIdx.x; k < n_zero; k += blockDim.x) zero[k] = 0u;
+        order_by_len_block(plen0, n0, order0);
+    } else {
+        order_by_len_block(plen1, n1, order1);
+    }
+}
+
+// hist[(*gen - gen0) * pop + i] = src[i] for 4-byte words, for two
+// (src, hist) pairs (the second optional; generations outside
+// [gen0, gen0 + max_gens) are dropped).
+__global__ void k_record(const uint32_t *src0, uint32_t *hist0, const uint32_t *src1, uint32_t *hist1, int pop,
+                         const uint32_t *gen, uint32_t gen0, int max_gens)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t g = *gen - gen0;
-    if (i < pop && g < (uint32_t)max_gens) hist[(size_t)g * pop + i] = src[i];
+    if (i >= pop || g >= (uint32_t)max_gens) return;
+    hist0[(size_t)g * pop + i] = src0[i];
+    if (src1) hist1[(size_t)g * pop + i] = src1[i];
 }
 
 __global__ void k_u32_add(uint32_t *p, uint32_t v)
@@ -211,11 +236,19 @@ hipError_t launch_order_by_len(const int32_t *plen, int n, int32_t *order, hipSt
     return hipGetLastError();
 }
 
-hipError_t launch_record(const void *src, int pop, void *hist, const uint32_t *gen, uint32_t gen0,
-                         int max_gens, hipStream_t s)
+hipError_t launch_order_by_len2(const int32_t *plen0, int n0, int32_t *order0, const int32_t *plen1, int n1,
+                                int32_t *order1, uint32_t *zero, int n_zero, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_record, dim3(blocks(pop, 256)), dim3(256), 0, s, (const uint32_t *)src, pop,
-                       (uint32_t *)hist, gen, gen0, max_gens);
+    hipLaunchKernelGGL(k_order_by_len2, dim3(2), dim3(1024), 0, s, plen0, n0, order0, plen1, n1, order1, zero,
+                       n_zero);
+    return hipGetLastError();
+}
+
+hipError_t launch_record(const void *src0, void *hist0, const void *src1, void *hist1, int pop,
+                         const uint32_t *gen, uint32_t gen0, int max_gens, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_record, dim3(blocks(pop, 256)), dim3(256), 0, s, (const uint32_t *)src0,
+                       (uint32_t *)hist0, (const uint32_t *)src1, (uint32_t *)hist1, pop, gen, gen0, max_gens);
     return hipGetLastError();
 }
 

@@ -436,9 +436,8 @@ int sr_ga_run(sr_ctx *x, const sr_select_params *spp, int mode, int32_t gens, ui
         int r = device_cache ? dmemo_enqueue(c, b, s, nullptr, time_launches ? kev : nullptr)
                              : ga_enqueue_eval(c, b, lp, s);
         if (r) return r;
-        hipError_t e = launch_record(c.d_mse.p, g.pop, g.hist.p, g.gen.as<uint32_t>(), gen0, gens, s);
-        if (e == hipSuccess)
-            e = launch_record(g.len[b].p, g.pop, g.hist_len.p, g.gen.as<uint32_t>(), gen0, gens, s);
+        hipError_t e = launch_record(c.d_mse.p, g.hist.p, g.len[b].p, g.hist_len.p, g.pop, g.gen.as<uint32_t>(),
+                                     gen0, gens, s);
         if (e != hipSuccess) return c.hip_fail(e, "record");
         r = ga_enqueue_next(c, b, sp, seed, c.d_mse.as<float>(), s);
         if (r) return r;

@@ -153,7 +153,8 @@ int plan_batched(Ctx &c, int mode, int E, LaunchPlan &lp)
     hipError_t he;
     int rc = plan_launch(c, mode, lp);
     if (rc) return rc;
-    if ((he = c.d_counters.ensure(64 * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
+    // queue counters: two sets of 64 (the device cache's two launches use one each)
+    if ((he = c.d_counters.ensure(2 * 64 * 4)) != hipSuccess) return c.hip_fail(he, "hipMalloc");
     if (lp.kind == KERNEL_VEC &&
         (he = c.d_spill.ensure((size_t)lp.grid * (kBlock / kWave) * kVecSpillBytesPerWave)) != hipSuccess)
         return c.hip_fail(he, "hipMalloc");
@@ -191,13 +192,14 @@ int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, 
     a.n_queues = lp.nq;
     a.chunks_per_queue = lp.cpq;
     a.P = lp.P;
-    if ((he = launch_zero_u32(a.counters, lp.nq, s)) != hipSuccess) return c.hip_fail(he, "zero counters");
+    if (!b.counters_zeroed && (he = launch_zero_u32(a.counters, lp.nq, s)) != hipSuccess)
+        return c.hip_fail(he, "zero counters");
     if (k0) hipEventRecord(k0, s);
     a.spill = b.spill;
     he = lp.kind == KERNEL_VEC ? launch_vec(mode, a, lp.grid, s) : launch_batched(mode, lp.dpt, lp.xc, a, lp.grid, s);
     if (he != hipSuccess) return c.hip_fail(he, "batched launch");
     if (k1) hipEventRecord(k1, s);
-    if (is_mse_mode(mode)) {
+    if (is_mse_mode(mode) && b.finalize) {
         if ((he = launch_finalize(a.psum, a.pcnt, lp.P, E, b.mse, b.cnt, s, b.fin_count)) != hipSuccess)
             return c.hip_fail(he, "finalize launch");
     }
