@@ -263,7 +263,8 @@ int sr_compile_expr(const int32_t *tokens, const float *values, int32_t len, int
  * OP_STORE's value = destination column) with algorithm 0 = the serial
  * compiler of sr_compile_expr, 1 = the wave-parallel compiler the
  * device-planned cache runs on the GPU, here executed on the host lane by
- * lane.  Both give the same bytecode.  Returns the instruction count or a
+ * lane.  Both give the same bytecode, the first four OP_REF operands carrying
+ * 1 + their order in code bits 16..18.  Returns the instruction count or a
  * negative sr_status.  A test and inspection hook; no counterpart in the
  * reference. */
 int sr_compile_skeleton(const int32_t *tokens, const float *values, int32_t len, int32_t nv, int32_t algorithm,

@@ -40,6 +40,7 @@ constexpr int kWaveLanes = 64;
 struct WaveCompileScratch {
     int32_t *part;                     // [kWaveLanes]
     int32_t *flag;                     // [4]
+    uint32_t *bits;                    // [(cap + 31) / 32] OP_REF operands by position
     int16_t *S;                        // [cap + 1]
     int16_t *size, *need;              // [cap]
     int16_t *W;                        // [cap + 1]
@@ -48,14 +49,15 @@ struct WaveCompileScratch {
 static_assert(MAX_EXPR_LEN < 32768, "16-bit compile scratch");
 
 // Scratch ints needed for programs of up to cap tokens.
-SR_HD int wave_compile_scratch_ints(int cap) { return kWaveLanes + 4 + (8 * cap + 2 + 1) / 2; }
+SR_HD int wave_compile_scratch_ints(int cap) { return kWaveLanes + 4 + (cap + 31) / 32 + (8 * cap + 2 + 1) / 2; }
 
 SR_HD WaveCompileScratch wave_compile_scratch(int32_t *base, int cap)
 {
     WaveCompileScratch s;
     s.part = base;
     s.flag = s.part + kWaveLanes;
-    s.S = (int16_t *)(s.flag + 4);
+    s.bits = (uint32_t *)(s.flag + 4);
+    s.S = (int16_t *)(s.bits + (cap + 31) / 32);
     s.size = s.S + cap + 1;
     s.need = s.size + cap;
     s.W = s.need + cap;
@@ -102,13 +104,35 @@ SR_HD int wave_block_hi(int n, int lane)
 
 SR_HD int tok_delta(int t) { return is_leaf(t) ? 1 : (is_binary(t) ? -1 : 0); }
 
-// Compile tok/val[0..n) (OP_REF and OP_STORE accepted); out: [n].  Returns
-// the instruction count (uniform), or -1 for a malformed stream, and sets
-// *stack_need.  Same result as raw_stack_depth(tok, n, true) >= 0 followed
-// by compile_expr.
+// OR into a scratch word shared by the lanes.
+SR_HD void wave_or(uint32_t *p, uint32_t v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicOr(p, v);
+#else
+    *p |= v;
+#endif
+}
+
+// Number of set bits below position pos, stopping once it reaches cap_count.
+SR_HD int bits_below(const uint32_t *bits, int pos, int cap_count)
+{
+    int r = 0;
+    for (int w = 0; w < (pos >> 5) && r < cap_count; ++w) r += __builtin_popcount(bits[w]);
+    if (pos & 31) r += __builtin_popcount(bits[pos >> 5] & ((1u << (pos & 31)) - 1u));
+    return r;
+}
+
+// Compile tok/val[0..n) (OP_REF and OP_STORE accepted); out: [n] (written
+// only when the stream is valid).  Returns the instruction count (uniform),
+// or -1 for a malformed stream, and sets *stack_need.  Same result as
+// raw_stack_depth(tok, n, true) >= 0 followed by compile_expr.  With pf
+// (kRefPrefetch entries), the first kRefPrefetch OP_REF operands in program
+// order carry j + 1 in bits BC_PF_SHIFT.. and pf[j] = their slot (-1 past
+// the last one), as the skeleton launches expect.
 template <class Wave>
 SR_HD int compile_wave(Wave &w, const int32_t *tok, const float *val, int n, int nv, WaveCompileScratch s,
-                       Ins *out, int *stack_need)
+                       Ins *out, int *stack_need, int32_t *pf = nullptr)
 {
     if (n <= 0 || n > MAX_EXPR_LEN) return -1;
     int16_t *S = s.S, *size = s.size, *need = s.need, *W = s.W;
@@ -186,7 +210,10 @@ SR_HD int compile_wave(Wave &w, const int32_t *tok, const float *val, int n, int
         int src;
         int32_t arg;
         leaf_src(tok[0], val[0], nv, src, arg);
-        out[0] = {0 | (src << 8) | (CL_LOAD << BC_CLS_SHIFT), arg};
+        const bool ref = src == SRC_REF && pf;
+        out[0] = {0 | (src << 8) | (CL_LOAD << BC_CLS_SHIFT) | (ref ? 1 << BC_PF_SHIFT : 0), arg};
+        if (pf)
+            for (int j = 0; j < kRefPrefetch; ++j) pf[j] = j == 0 && ref ? arg : -1;
         return 1;
     }
     // 4. instruction counts (prefix W); pointer-jumping state of every node
@@ -205,6 +232,7 @@ SR_HD int compile_wave(Wave &w, const int32_t *tok, const float *val, int n, int
             s.acc0[k] = 0;
         }
         part[lane] = x;
+        for (int k = lane; k < (n + 31) / 32; k += kWaveLanes) s.bits[k] = 0u;
     });
     w.each([&](int lane) {
         int x = 0;
@@ -266,53 +294,92 @@ SR_HD int compile_wave(Wave &w, const int32_t *tok, const float *val, int n, int
         acc2 = t0;
         up2 = t1;
     }
-    // 6. instructions, then the push bits
-    w.each([&](int lane) {
-        for (int i = lane; i < n; i += kWaveLanes) {
-            const int t = tok[i];
-            if (is_leaf(t)) continue;
-            const int start = acc[i], at = start + count(i) - 1;
-            const int a = i + 1;
-            int src;
-            int32_t arg;
-            if (!is_binary(t)) {
-                if (is_leaf(tok[a])) {
-                    leaf_src(tok[a], val[a], nv, src, arg);
-                    out[at] = {t | (src << 8) | (CL_UN << BC_CLS_SHIFT), arg};
-                } else if (t == OP_STORE) {
-                    out[at] = {STORE_CODE, (int32_t)val[i]};
-                } else {
-                    out[at] = {t | (SRC_NONE << 8) | (CL_UN << BC_CLS_SHIFT), 0};
-                }
-                continue;
-            }
-            const int b = a + size[a];
-            const bool la = is_leaf(tok[a]), lb = is_leaf(tok[b]);
-            const int bin = CL_BIN << BC_CLS_SHIFT;
-            if (la && lb) {
-                leaf_src(tok[a], val[a], nv, src, arg);
-                out[start] = {0 | (src << 8) | (CL_LOAD << BC_CLS_SHIFT), arg};
-                leaf_src(tok[b], val[b], nv, src, arg);
-                out[at] = {t | (src << 8) | bin, arg};
-            } else if (la) {
-                leaf_src(tok[a], val[a], nv, src, arg);
-                out[at] = {t | (src << 8) | (is_commutative(t) ? 0 : BC_SWAP) | bin, arg};
-            } else if (lb) {
-                leaf_src(tok[b], val[b], nv, src, arg);
-                out[at] = {t | (src << 8) | bin, arg};
+    // 6. the instructions of operator node i (1, or 2 for a binary operator
+    // of two leaves: its LOAD at start), as (position, code, arg).  The
+    // instruction a subtree starts with pushes the accumulator unless the
+    // subtree starts the program: start > 0 means it is evaluated after a
+    // sibling whose value waits on the stack.
+    auto node_ins = [&](int i, int *pos, int32_t *code, int32_t *arg) -> int {
+        const int t = tok[i];
+        const int start = acc[i], at = start + count(i) - 1;
+        const int push = start > 0 ? BC_PUSH : 0;
+        const int a = i + 1;
+        int src;
+        int32_t g;
+        pos[0] = at;
+        if (!is_binary(t)) {
+            if (is_leaf(tok[a])) {
+                leaf_src(tok[a], val[a], nv, src, g);
+                code[0] = t | (src << 8) | push | (CL_UN << BC_CLS_SHIFT);
+                arg[0] = g;
+            } else if (t == OP_STORE) {
+                code[0] = STORE_CODE;
+                arg[0] = (int32_t)val[i];
             } else {
-                const bool a_first = need[a] > need[b];
-                out[at] = {t | (SRC_POP << 8) | ((a_first && !is_commutative(t)) ? BC_SWAP : 0) | bin, 0};
+                code[0] = t | (SRC_NONE << 8) | (CL_UN << BC_CLS_SHIFT);
+                arg[0] = 0;
             }
+            return 1;
         }
-    });
+        const int b = a + size[a];
+        const bool la = is_leaf(tok[a]), lb = is_leaf(tok[b]);
+        const int bin = CL_BIN << BC_CLS_SHIFT;
+        if (la && lb) {
+            leaf_src(tok[a], val[a], nv, src, g);
+            pos[0] = start;
+            code[0] = 0 | (src << 8) | push | (CL_LOAD << BC_CLS_SHIFT);
+            arg[0] = g;
+            leaf_src(tok[b], val[b], nv, src, g);
+            pos[1] = at;
+            code[1] = t | (src << 8) | bin;
+            arg[1] = g;
+            return 2;
+        }
+        if (la) {
+            leaf_src(tok[a], val[a], nv, src, g);
+            code[0] = t | (src << 8) | (is_commutative(t) ? 0 : BC_SWAP) | bin;
+            arg[0] = g;
+        } else if (lb) {
+            leaf_src(tok[b], val[b], nv, src, g);
+            code[0] = t | (src << 8) | bin;
+            arg[0] = g;
+        } else {
+            const bool a_first = need[a] > need[b];
+            code[0] = t | (SRC_POP << 8) | ((a_first && !is_commutative(t)) ? BC_SWAP : 0) | bin;
+            arg[0] = 0;
+        }
+        return 1;
+    };
+    if (pf) {  // positions of the OP_REF operands
+        w.each([&](int lane) {
+            for (int i = lane; i < n; i += kWaveLanes) {
+                if (is_leaf(tok[i])) continue;
+                int pos[2];
+                int32_t code[2], arg[2];
+                const int k = node_ins(i, pos, code, arg);
+                for (int j = 0; j < k; ++j)
+                    if (bc_src(code[j]) == SRC_REF) wave_or(&s.bits[pos[j] >> 5], 1u << (pos[j] & 31));
+            }
+        });
+    }
     w.each([&](int lane) {
         for (int i = lane; i < n; i += kWaveLanes) {
-            if (!is_binary(tok[i])) continue;
-            int f, g;
-            children(i, f, g);
-            if (g >= 0) out[acc[g]].code |= BC_PUSH;
+            if (is_leaf(tok[i])) continue;
+            int pos[2];
+            int32_t code[2], arg[2];
+            const int k = node_ins(i, pos, code, arg);
+            for (int j = 0; j < k; ++j) {
+                if (pf && bc_src(code[j]) == SRC_REF) {
+                    const int r = bits_below(s.bits, pos[j], kRefPrefetch);
+                    if (r < kRefPrefetch) {
+                        code[j] |= (r + 1) << BC_PF_SHIFT;
+                        pf[r] = arg[j];
+                    }
+                }
+                out[pos[j]] = {code[j], arg[j]};
+            }
         }
+        if (pf && lane < kRefPrefetch && bits_below(s.bits, W[n], kRefPrefetch) <= lane) pf[lane] = -1;
     });
     return W[n];
 }

@@ -50,6 +50,15 @@ constexpr int kQueueCounters = 64;             // queue counters per launch (c.d
 enum { C_GEN = 0, C_FREE, C_L1, C_NEST, C_L2, C_ROOTS, C_NESTED, C_STATUS, C_COUNT };
 // statistics (int64), accumulated over generations
 enum { S_LOGICAL = 0, S_EXEC, S_REFS, S_STORES, S_OPS, S_L1, S_L2, S_LIVE, S_GENS, S_COUNT };
+// Counters every program or expression adds to (S_LOGICAL .. S_OPS) are
+// spread over kStatSpread copies after the S_COUNT totals (stats[S_COUNT +
+// copy * S_COUNT + k]), so thousands of blocks do not queue on one address;
+// the host sums the copies.
+constexpr int kStatSpread = 64;
+SR_DEV void stat_add(long long *stats, int who, int k, long long v)
+{
+    atomicAdd((unsigned long long *)&stats[S_COUNT + (who & (kStatSpread - 1)) * S_COUNT + k], (unsigned long long)v);
+}
 
 SR_DEV uint64_t mix64(uint64_t x)
 {
@@ -315,7 +324,9 @@ __global__ void k_dm_lookup(DmArgs a)
         const bool same = a.s_len[s] == n && same_tokens(a.s_tok + sb, a.s_val + sb, a.tok + p, a.val + p, n);
         if (same) {
             a.hit[p] = s;
-            a.s_seen[s] = (uint32_t)a.ctr[C_GEN];
+            // popular slots are hit thousands of times: store only when it changes
+            const uint32_t g = (uint32_t)a.ctr[C_GEN];
+            if (a.s_seen[s] != g) a.s_seen[s] = g;
         } else {
             a.hit[p] = -2;  // the key belongs to another body: never cached
         }
@@ -363,8 +374,10 @@ __global__ __launch_bounds__(64) void k_dm_count(DmArgs a)
                 k = prev == 0 ? h : prev;
             }
             if (k != h) continue;
-            atomicAdd(&a.ct_cnt[i], 1);
-            atomicMin(&a.ct_rep[i], (int32_t)p);
+            // only "at least twice" is ever asked of the count, and the
+            // minimum only moves down: popular keys skip most atomics
+            if (a.ct_cnt[i] < 2) atomicAdd(&a.ct_cnt[i], 1);
+            if ((int32_t)p < a.ct_rep[i]) atomicMin(&a.ct_rep[i], (int32_t)p);
             a.cidx[p] = (int32_t)i;
             break;
         }
@@ -433,7 +446,8 @@ __global__ __launch_bounds__(64) void k_dm_share(DmArgs a)
         if (c == 1) {
             prev = q;
             a.anc[p] = (int32_t)p;
-            atomicMin(&a.ct_root[a.cidx[p]], (int32_t)p);
+            const int i = a.cidx[p];
+            if ((int32_t)p < a.ct_root[i]) atomicMin(&a.ct_root[i], (int32_t)p);
         } else {
             a.anc[p] = (int32_t)(b + prev);
         }
@@ -608,15 +622,15 @@ __global__ void k_dm_index(DmArgs a)
 // first position becomes one OP_REF, a nested key this root writes is
 // preceded by an OP_STORE, and a launch-1 program starts with the OP_STORE
 // of its own column.  The skeleton is then compiled by the wave
-// (sr_compile_wave.h), its first OP_REF operands marked for prefetch.
+// (sr_compile_wave.h) straight into the program buffer, its first OP_REF
+// operands marked for prefetch.
 __global__ __launch_bounds__(64) void k_dm_program(DmArgs a, int nv)
 {
     extern __shared__ int32_t lds32[];
     const int PC = a.PC, cap = a.cap;
     int32_t *ot = lds32;
     float *ov = (float *)(ot + PC);
-    Ins *lout = (Ins *)(ov + PC);
-    int32_t *scr = (int32_t *)(lout + PC);
+    int32_t *scr = (int32_t *)(ov + PC);
     const WaveCompileScratch cs = wave_compile_scratch(scr, PC);
     // emission scratch, dead before the compile reuses the area
     int32_t *xref = scr, *xst = xref + cap, *xsz = xst + cap, *xtok = xsz + cap;
@@ -731,34 +745,19 @@ __global__ __launch_bounds__(64) void k_dm_program(DmArgs a, int nv)
     });
     const int m = total;
     int sn = 0;
-    int n = compile_wave(w, ot, ov, m, nv, cs, lout, &sn);
+    // bytecode straight to the program buffer, OP_REF prefetch list beside it
+    Ins *out = (first ? a.p1_ins : a.p2_ins) + (int64_t)idx * PC;
+    int32_t *pf = (first ? a.p1_pf : a.p2_pf) + (int64_t)idx * kRefPrefetch;
+    int n = compile_wave(w, ot, ov, m, nv, cs, out, &sn, pf);
     if (n < 0 || sn > kStackSlots) {
         if (lane == 0) {
             atomicOr(&a.ctr[C_STATUS], 1);
-            lout[0].code = 0 | (SRC_CONST << 8);
-            lout[0].arg = f32_bits(__builtin_nanf(""));
+            out[0].code = 0 | (SRC_CONST << 8);
+            out[0].arg = f32_bits(__builtin_nanf(""));
         }
+        if (lane < kRefPrefetch) pf[lane] = -1;
         n = 1;
-        __syncthreads();
     }
-    // the first kRefPrefetch OP_REF operands, in program order
-    int32_t *pf = (first ? a.p1_pf : a.p2_pf) + (int64_t)idx * kRefPrefetch;
-    int found = 0;
-    for (int c0 = 0; c0 < n && found < kRefPrefetch; c0 += kWaveLanes) {
-        const int i = c0 + lane;
-        const bool isref = i < n && bc_src(lout[i].code) == SRC_REF;
-        const uint64_t mask = __ballot(isref);
-        const int rank = found + __popcll(mask & ((1ull << lane) - 1));
-        if (isref && rank < kRefPrefetch) {
-            lout[i].code |= (rank + 1) << BC_PF_SHIFT;
-            pf[rank] = lout[i].arg;
-        }
-        found += __popcll(mask);
-    }
-    if (lane >= found && lane < kRefPrefetch) pf[lane] = -1;
-    __syncthreads();
-    Ins *out = (first ? a.p1_ins : a.p2_ins) + (int64_t)idx * PC;
-    for (int k = lane; k < n; k += kWaveLanes) out[k] = lout[k];
     long long refs = 0, stores = 0, ops = 0;
     for (int k = lane; k < m; k += kWaveLanes) {
         const int t = ot[k];
@@ -771,11 +770,10 @@ __global__ __launch_bounds__(64) void k_dm_program(DmArgs a, int nv)
     ops = wave_sum(ops);
     if (lane == 0) {
         (first ? a.p1_plen : a.p2_plen)[idx] = n;
-        atomicAdd((unsigned long long *)&a.stats[S_EXEC], (unsigned long long)m);
-        atomicAdd((unsigned long long *)&a.stats[S_REFS], (unsigned long long)refs);
-        atomicAdd((unsigned long long *)&a.stats[S_STORES], (unsigned long long)stores);
-        atomicAdd((unsigned long long *)&a.stats[S_OPS], (unsigned long long)ops);
-        if (!first) atomicAdd(&a.ctr[C_L2], 1);
+        stat_add(a.stats, g, S_EXEC, m);
+        stat_add(a.stats, g, S_REFS, refs);
+        stat_add(a.stats, g, S_STORES, stores);
+        stat_add(a.stats, g, S_OPS, ops);
     }
 }
 
@@ -819,7 +817,7 @@ __global__ __launch_bounds__(256) void k_dm_gather(DmArgs a, long long *cnt_out,
     if (lane != 0) return;
     a.fit[e] = m;
     if (cnt_out) cnt_out[e] = c;
-    atomicAdd((unsigned long long *)&a.stats[S_LOGICAL], (unsigned long long)a.len[e]);
+    stat_add(a.stats, e, S_LOGICAL, a.len[e]);
     if (e == 0) {
         a.stats[S_L1] += a.ctr[C_L1];
         a.stats[S_L2] += a.ctr[C_L2];
@@ -833,7 +831,7 @@ __global__ void k_dm_init_slots(DmArgs a)
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s == 0) {
         for (int k = 0; k < C_COUNT; ++k) a.ctr[k] = 0;
-        for (int k = 0; k < S_COUNT; ++k) a.stats[k] = 0;
+        for (int k = 0; k < S_COUNT * (1 + kStatSpread); ++k) a.stats[k] = 0;
     }
     if (s < a.H) a.ht_key[s] = 0;
     if (s >= a.C) return;
@@ -987,7 +985,7 @@ int dmemo_setup(Ctx &c, int capacity, int min_size)
     need(d.p2_pf, (size_t)E * kRefPrefetch * 4);
     need(d.ctr, C_COUNT * 4);
     need(d.blk, (size_t)(std::max<int64_t>(M, C) / 256 + 2) * 4);
-    need(d.stats, S_COUNT * 8);
+    need(d.stats, (size_t)S_COUNT * (1 + kStatSpread) * 8);
     if (he != hipSuccess) return c.hip_fail(he, "hipMalloc device cache");
     LaunchPlan lp;  // MSE partials and fitness buffers for max(launch-1 capacity, E) programs
     int rc = plan_batched(c, MODE_MSE_REF, std::max(d.L1, E), lp);
@@ -1046,16 +1044,18 @@ int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *
     if (int r = scan(0, 2, C_NESTED)) return r;
     DM_LAUNCH(k_dm_alloc, d.H2, 0);
     DM_LAUNCH(k_dm_index, std::max<int64_t>(d.C, M));
-    // LDS: skeleton tokens and values, bytecode, then the larger of the
-    // compile scratch and the emission scratch that it reuses
+    // LDS: skeleton tokens and values, then the larger of the compile scratch
+    // and the emission scratch that it reuses
     const int scratch = std::max(wave_compile_scratch_ints(d.PC), 6 * d.cap + 1 + 2 * kWaveLanes);
-    DM_WAVES(k_dm_program, (int64_t)d.L1 + d.E, (size_t)d.PC * (8 + sizeof(Ins)) + 4 * (size_t)scratch, c.nv);
+    DM_WAVES(k_dm_program, (int64_t)d.L1 + d.E, (size_t)d.PC * 8 + 4 * (size_t)scratch, c.nv);
 #undef DM_WAVES
 #undef DM_LAUNCH
     // both launches' program orders; their queue counters zeroed
+    // (and the launch-2 program count: expressions with a program)
     if ((he = launch_order_by_len2(d.p1_plen.as<int32_t>(), d.L1, d.p1_order.as<int32_t>(),
                                    d.p2_plen.as<int32_t>(), d.E, d.p2_order.as<int32_t>(),
-                                   c.d_counters.as<uint32_t>(), 2 * kQueueCounters, s)) != hipSuccess)
+                                   c.d_counters.as<uint32_t>(), 2 * kQueueCounters, d.ctr.as<int32_t>() + C_L2,
+                                   s)) != hipSuccess)
         return c.hip_fail(he, "device cache order");
     int P = 0;
     int rc = dmemo_launch(c, s, ev, true, &P);
@@ -1120,8 +1120,13 @@ int dmemo_stats(Ctx &c, long long out[9])
         for (int k = 0; k < S_COUNT; ++k) out[k] = 0;
         return SR_OK;
     }
-    hipError_t he = hipMemcpy(out, c.dm.stats.p, S_COUNT * 8, hipMemcpyDeviceToHost);
+    long long all[S_COUNT * (1 + kStatSpread)];
+    hipError_t he = hipMemcpy(all, c.dm.stats.p, sizeof(all), hipMemcpyDeviceToHost);
     if (he != hipSuccess) return c.hip_fail(he, "device cache statistics");
+    for (int k = 0; k < S_COUNT; ++k) {
+        out[k] = all[k];
+        for (int j = 0; j < kStatSpread; ++j) out[k] += all[S_COUNT + j * S_COUNT + k];
+    }
     int32_t st = 0;
     he = hipMemcpy(&st, c.dm.ctr.as<int32_t>() + C_STATUS, 4, hipMemcpyDeviceToHost);
     if (he != hipSuccess) return c.hip_fail(he, "device cache status");

@@ -109,7 +109,7 @@ __global__ void k_compile_pop(const int32_t *tok, const float *val, const int32_
 
 // Longest program first (one workgroup of 1024 threads: counting sort,
 // lengths < 4096).
-__device__ void order_by_len_block(const int32_t *plen, int n, int32_t *order)
+__device__ void order_by_len_block(const int32_t *plen, int n, int32_t *order, int32_t *nonzero = nullptr)
 {
     constexpr int B = 4096;
     __shared__ int hist[B];
@@ -121,6 +121,7 @@ __device__ void order_by_len_block(const int32_t *plen, int n, int32_t *order)
         atomicAdd(&hist[B - 1 - l], 1);
     }
     __syncthreads();
+    if (nonzero && threadIdx.x == 0) *nonzero = n - hist[B - 1];  // bin B - 1: length 0
     // exclusive scan of the histogram: 4 bins per thread, then the threads'
     // sums across the wave (shuffles) and across the 16 waves (LDS)
     __shared__ int wsum[1024 / 64];
@@ -158,16 +159,17 @@ __global__ __launch_bounds__(1024) void k_order_by_len(const int32_t *plen, int 
 }
 
 // Two orders in one launch (a workgroup each); the first workgroup also
-// zeroes n_zero words (the persistent launches' queue counters).
+// zeroes n_zero words (the persistent launches' queue counters), the second
+// writes how many of its lengths are nonzero.
 __global__ __launch_bounds__(1024) void k_order_by_len2(const int32_t *plen0, int n0, int32_t *order0,
                                                         const int32_t *plen1, int n1, int32_t *order1,
-                                                        uint32_t *zero, int n_zero)
+                                                        uint32_t *zero, int n_zero, int32_t *nonzero1)
 {
     if (blockIdx.x == 0) {
         for (int k = threadIdx.x; k < n_zero; k += blockDim.x) zero[k] = 0u;
         order_by_len_block(plen0, n0, order0);
     } else {
-        order_by_len_block(plen1, n1, order1);
+        order_by_len_block(plen1, n1, order1, nonzero1);
     }
 }
 
@@ -237,10 +239,10 @@ hipError_t launch_order_by_len(const int32_t *plen, int n, int32_t *order, hipSt
 }
 
 hipError_t launch_order_by_len2(const int32_t *plen0, int n0, int32_t *order0, const int32_t *plen1, int n1,
-                                int32_t *order1, uint32_t *zero, int n_zero, hipStream_t s)
+                                int32_t *order1, uint32_t *zero, int n_zero, int32_t *nonzero1, hipStream_t s)
 {
     hipLaunchKernelGGL(k_order_by_len2, dim3(2), dim3(1024), 0, s, plen0, n0, order0, plen1, n1, order1, zero,
-                       n_zero);
+                       n_zero, nonzero1);
     return hipGetLastError();
 }
 

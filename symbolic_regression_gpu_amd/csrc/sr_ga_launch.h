@@ -20,9 +20,10 @@ hipError_t launch_compile_pop(const int32_t *tok, const float *val, const int32_
                               int32_t *need_scratch, CompileFrame *frames, int32_t *max_need,
                               int32_t *status, hipStream_t s);
 hipError_t launch_order_by_len(const int32_t *plen, int n, int32_t *order, hipStream_t s);
-// Both orders in one launch; also zeroes zero[0, n_zero).
+// Both orders in one launch; also zeroes zero[0, n_zero) and sets *nonzero1
+// to the number of nonzero plen1 entries.
 hipError_t launch_order_by_len2(const int32_t *plen0, int n0, int32_t *order0, const int32_t *plen1, int n1,
-                                int32_t *order1, uint32_t *zero, int n_zero, hipStream_t s);
+                                int32_t *order1, uint32_t *zero, int n_zero, int32_t *nonzero1, hipStream_t s);
 // hist0/hist1 row (*gen - gen0) = src0/src1 (4-byte words; src1 may be null).
 hipError_t launch_record(const void *src0, void *hist0, const void *src1, void *hist1, int pop,
                          const uint32_t *gen, uint32_t gen0, int max_gens, hipStream_t s);

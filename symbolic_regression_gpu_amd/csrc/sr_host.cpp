@@ -632,15 +632,20 @@ int sr_compile_skeleton(const int32_t *tokens, const float *values, int32_t len,
     if (len <= 0 || len > MAX_EXPR_LEN) return SR_ERR_BAD_EXPR;
     std::vector<Ins> ins(len);
     int sn = 0, n = -1;
+    int32_t pf[kRefPrefetch];
     if (algorithm == 0) {
         if (raw_stack_depth(tokens, len, true) < 0) return SR_ERR_BAD_EXPR;
         std::vector<int> size(len), need(len);
         std::vector<CompileFrame> frames(len + 1);
         n = compile_expr(tokens, values, len, nv, size.data(), need.data(), frames.data(), ins.data(), &sn);
+        // the first kRefPrefetch OP_REF operands carry their order + 1
+        for (int i = 0, j = 0; n > 0 && i < n && j < kRefPrefetch; ++i)
+            if (bc_src(ins[i].code) == SRC_REF) ins[i].code |= ++j << BC_PF_SHIFT;
     } else {
         std::vector<int32_t> scratch(wave_compile_scratch_ints(len));
         HostWave w;
-        n = compile_wave(w, tokens, values, len, nv, wave_compile_scratch(scratch.data(), len), ins.data(), &sn);
+        n = compile_wave(w, tokens, values, len, nv, wave_compile_scratch(scratch.data(), len), ins.data(), &sn,
+                         pf);
     }
     if (n < 0) return SR_ERR_BAD_EXPR;
     for (int i = 0; i < n; ++i) {

@@ -87,8 +87,13 @@ def test_wave_compiler_equals_serial_on_skeletons(seed):
         tok, val = pop.expr(e)
         st, sv = _skeleton(tok, val, rng)
         code, arg, _ = _both(st, sv, 3)
-        refs += int(np.sum(((code >> 8) & 7) == 3))
+        is_ref = ((code >> 8) & 7) == 3
+        refs += int(np.sum(is_ref))
         assert np.sum(code == 31 | (2 << 13)) == np.sum(st == OP_STORE)  # STORE_CODE per OP_STORE
+        # prefetch marks: 1..4 on the first four OP_REF operands in program order
+        marks = (code >> 16) & 7
+        assert list(marks[is_ref][:4]) == list(range(1, min(4, int(np.sum(is_ref))) + 1))
+        assert not np.any(marks[~is_ref]) and not np.any(marks[is_ref][4:])
     assert refs > 0
 
 
