@@ -143,7 +143,7 @@ struct DMemoState {
     uint64_t data_version = 0;
     int min_size = 3;
     DevBuf cols;                                                      // [C][N] fp32
-    DevBuf s_key, s_len, s_tok, s_val, s_seen, s_live, s_mse, s_cnt, s_has, s_free;  // per slot
+    DevBuf s_key, s_len, s_tok, s_val, s_src, s_seen, s_live, s_mse, s_cnt, s_has, s_free;  // per slot
     DevBuf ht_key, ht_slot;                                           // key -> slot
     DevBuf ct_key, ct_cnt, ct_rep, ct_root, ct_slot, ct_wr, ct_off, ct_prog;  // per generation
     DevBuf sz, hh, hit, sh, anc, pslot, ctx_idx, flag, rank;          // per position [E*cap]

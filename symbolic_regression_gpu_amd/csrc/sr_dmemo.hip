@@ -43,6 +43,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int32_t kNone = INT_MAX;
 constexpr int32_t kFromLaunch1 = 0x40000000;  // e_src: launch-1 program r is kFromLaunch1 | r
+constexpr int32_t kHitPending = -6;            // hit: kHitPending - slot awaits its body compare
 constexpr int kQueueCounters = 64;             // queue counters per launch (c.d_counters holds two sets)
 
 // counters (int32): generation, free slots, launch-1 programs, nested slots
@@ -81,7 +82,7 @@ struct DmArgs {
     uint64_t *hh;
     int32_t *hit, *sh, *anc, *pslot, *cidx, *flag, *rank;
     uint64_t *s_key;
-    int32_t *s_len, *s_tok;
+    int32_t *s_len, *s_tok, *s_src;
     float *s_val;
     uint32_t *s_seen;
     int32_t *s_live;
@@ -102,7 +103,7 @@ struct DmArgs {
     long long *stats;
 };
 
-// Token bodies compared / copied 8 tokens at a time: the loads of a chunk are
+// Token bodies compared 8 tokens at a time: the loads of a chunk are
 // independent (indexes clamped to the body), so a long body costs one memory
 // latency per chunk instead of one per token.
 constexpr int kBodyChunk = 8;
@@ -129,31 +130,62 @@ SR_DEV bool same_tokens(const int32_t *__restrict__ t1, const float *__restrict_
     return same;
 }
 
-SR_DEV void copy_tokens(int32_t *__restrict__ td, float *__restrict__ vd, const int32_t *__restrict__ ts,
-                        const float *__restrict__ vs, int n)
-{
-    for (int j0 = 0; j0 < n; j0 += kBodyChunk) {
-        const int m = min(kBodyChunk, n - j0);
-        int32_t x[kBodyChunk];
-        float u[kBodyChunk];
-#pragma unroll
-        for (int k = 0; k < kBodyChunk; ++k) {
-            const int j = j0 + min(k, m - 1);
-            x[k] = ts[j];
-            u[k] = vs[j];
-        }
-#pragma unroll
-        for (int k = 0; k < kBodyChunk; ++k)
-            if (k < m) {
-                td[j0 + k] = x[k];
-                vd[j0 + k] = u[k];
-            }
-    }
-}
-
 SR_DEV bool same_body(const DmArgs &a, int64_t p, int64_t r, int n)
 {
     return same_tokens(a.tok + p, a.val + p, a.tok + r, a.val + r, n);
+}
+
+// Exclusive prefix of a per-lane value over the wave's lanes, through LDS
+// (part: kWaveLanes ints; every lane must call it).
+SR_DEV int lane_prefix_sum(int32_t *part, int v, int lane)
+{
+    part[lane] = v;
+    __syncthreads();
+    int x = 0;
+    for (int k = 0; k < lane; ++k) x += part[k];
+    __syncthreads();
+    return x;
+}
+
+// Wave-cooperative body compares: candidate k compares n[k] tokens of the
+// population at position p[k] with a second body (body2(r[k]) gives its
+// tokens and values); mism[k] = 1 where they differ.  Every body is cut into
+// chunks of kCoopChunk tokens and the chunks of all candidates are spread
+// over the lanes, so a long body costs no more round trips than a short one.
+// LDS: p, r, n, off, mism [nc]; part [kWaveLanes].  Every lane must call it.
+constexpr int kCoopChunk = 16;
+
+template <class Body2>
+SR_DEV void wave_compare(const DmArgs &a, int nc, const int32_t *cp, const int32_t *cr, const int32_t *cn,
+                         int32_t *off, int32_t *mism, int32_t *part, int lane, Body2 body2)
+{
+    const int lo = wave_block_lo(nc, lane), hi = wave_block_hi(nc, lane);
+    int x = 0;
+    for (int k = lo; k < hi; ++k) x += (cn[k] + kCoopChunk - 1) / kCoopChunk;
+    int o = lane_prefix_sum(part, x, lane);
+    for (int k = lo; k < hi; ++k) {
+        off[k] = o;
+        mism[k] = 0;
+        o += (cn[k] + kCoopChunk - 1) / kCoopChunk;
+    }
+    const int total = __shfl(o, kWaveLanes - 1, kWaveLanes);
+    __syncthreads();
+    for (int c = lane; c < total; c += kWaveLanes) {
+        int l = 0, h = nc - 1;  // the last candidate whose chunks start at or before c
+        while (l < h) {
+            const int mid = (l + h + 1) >> 1;
+            if (off[mid] <= c) l = mid;
+            else h = mid - 1;
+        }
+        const int j0 = (c - off[l]) * kCoopChunk;
+        const int32_t *t2;
+        const float *v2;
+        body2(cr[l], t2, v2);
+        const int64_t p = cp[l];
+        if (!same_tokens(a.tok + p + j0, a.val + p + j0, t2 + j0, v2 + j0, min(kCoopChunk, cn[l] - j0)))
+            mism[l] = 1;
+    }
+    __syncthreads();
 }
 
 // ---- per generation: reset (counters, per-position and per-key state) -----------
@@ -188,18 +220,6 @@ __global__ void k_dm_clear(DmArgs a)
         a.ct_off[p] = 0;
         a.ct_prog[p] = -1;
     }
-}
-
-// Exclusive prefix of a per-lane value over the wave's lanes, through LDS
-// (part: kWaveLanes ints; every lane must call it).
-SR_DEV int lane_prefix_sum(int32_t *part, int v, int lane)
-{
-    part[lane] = v;
-    __syncthreads();
-    int x = 0;
-    for (int k = 0; k < lane; ++k) x += part[k];
-    __syncthreads();
-    return x;
 }
 
 // ---- 1. sizes and subtree keys (one wave per expression) ---------------------
@@ -303,7 +323,9 @@ __global__ __launch_bounds__(64) void k_dm_hash(DmArgs a)
     }
 }
 
-// ---- 2. lookup of every subtree of >= min_size tokens ------------------------
+// ---- 2. lookup of every subtree of >= min_size tokens -------------------------
+// hit[p]: slot (>= 0, body verified), -1 no key, -2 another body's key, -3 not
+// looked up, kHitPending - slot: key and length match, body not yet compared. ------------------------
 __global__ void k_dm_lookup(DmArgs a)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -320,16 +342,9 @@ __global__ void k_dm_lookup(DmArgs a)
         }
         if (k != h) continue;
         const int s = a.ht_slot[i];
-        const int64_t sb = (int64_t)s * a.cap;
-        const bool same = a.s_len[s] == n && same_tokens(a.s_tok + sb, a.s_val + sb, a.tok + p, a.val + p, n);
-        if (same) {
-            a.hit[p] = s;
-            // popular slots are hit thousands of times: store only when it changes
-            const uint32_t g = (uint32_t)a.ctr[C_GEN];
-            if (a.s_seen[s] != g) a.s_seen[s] = g;
-        } else {
-            a.hit[p] = -2;  // the key belongs to another body: never cached
-        }
+        // equal lengths: the token bodies are compared by k_dm_count's wave;
+        // else the key belongs to another body: never cached
+        a.hit[p] = a.s_len[s] == n ? kHitPending - s : -2;
         return;
     }
 }
@@ -343,17 +358,51 @@ __global__ void k_dm_lookup(DmArgs a)
 __global__ __launch_bounds__(64) void k_dm_count(DmArgs a)
 {
     extern __shared__ int32_t lds32[];
+    const int cap = a.cap;
     int32_t *cnt = lds32;  // [cap + 1]
+    int32_t *lhit = cnt + cap + 1, *cp = lhit + cap, *cr = cp + cap, *cn = cr + cap, *coff = cn + cap,
+            *mism = coff + cap;  // [cap] each
     __shared__ int32_t part[kWaveLanes];
+    __shared__ int32_t nc;
     const int e = blockIdx.x, lane = threadIdx.x;
     const int64_t b = (int64_t)e * a.cap;
     const int L = max(0, min(a.len[e], a.cap));
-    const int r0 = a.hit[b];
-    if (L == 0 || (r0 >= 0 && a.s_has[r0])) return;
+    if (L == 0) return;
+    // the lookup's pending hits: bodies compared by the wave
+    if (lane == 0) nc = 0;
+    __syncthreads();
+    for (int q = lane; q < L; q += kWaveLanes) {
+        const int h = a.hit[b + q];
+        lhit[q] = h;
+        if (h <= kHitPending) {
+            const int k = atomicAdd(&nc, 1);
+            cp[k] = (int32_t)(b + q);
+            cr[k] = kHitPending - h;
+            cn[k] = a.sz[b + q];
+        }
+    }
+    __syncthreads();
+    const int ncand = nc;
+    wave_compare(a, ncand, cp, cr, cn, coff, mism, part, lane, [&](int slot, const int32_t *&t, const float *&v) {
+        t = a.s_tok + (int64_t)slot * a.cap;
+        v = a.s_val + (int64_t)slot * a.cap;
+    });
+    const uint32_t gen = (uint32_t)a.ctr[C_GEN];
+    for (int k = lane; k < ncand; k += kWaveLanes) {
+        const int q = (int)(cp[k] - b), slot = cr[k];
+        const int h = mism[k] ? -2 : slot;
+        lhit[q] = h;
+        a.hit[cp[k]] = h;
+        // popular slots are hit thousands of times: store only when it changes
+        if (h >= 0 && a.s_seen[slot] != gen) a.s_seen[slot] = gen;
+    }
+    __syncthreads();
+    const int r0 = lhit[0];
+    if (r0 >= 0 && a.s_has[r0]) return;  // a settled expression runs nothing
     for (int q = lane; q <= L; q += kWaveLanes) cnt[q] = 0;
     __syncthreads();
     for (int q = lane; q < L; q += kWaveLanes)
-        if (a.hit[b + q] >= 0) {
+        if (lhit[q] >= 0) {
             atomicAdd(&cnt[q + 1], 1);
             atomicSub(&cnt[min(q + a.sz[b + q], L)], 1);
         }
@@ -365,7 +414,7 @@ __global__ __launch_bounds__(64) void k_dm_count(DmArgs a)
     for (int q = lo; q < hi; ++q) {
         open += cnt[q];
         const int64_t p = b + q;
-        if (open > 0 || a.hit[p] != -1) continue;
+        if (open > 0 || lhit[q] != -1) continue;
         const uint64_t h = a.hh[p];
         for (uint32_t i = (uint32_t)mix64(h) & (a.H2 - 1);; i = (i + 1) & (a.H2 - 1)) {
             uint64_t k = a.ct_key[i];
@@ -395,26 +444,49 @@ __global__ __launch_bounds__(64) void k_dm_count(DmArgs a)
 __global__ __launch_bounds__(64) void k_dm_share(DmArgs a)
 {
     extern __shared__ int32_t lds32[];
+    const int cap = a.cap;
     int32_t *code = lds32;          // [cap]: 0 not shared, 1 shared top-level, 2 shared inside
-    int32_t *cnt = code + a.cap;    // [cap + 1]
+    int32_t *cnt = code + cap;      // [cap + 1]
+    int32_t *cp = cnt + cap + 1, *cr = cp + cap, *cn = cr + cap, *coff = cn + cap, *mism = coff + cap;  // [cap]
     __shared__ int32_t part[kWaveLanes];
+    __shared__ int32_t nc;
     const int e = blockIdx.x, lane = threadIdx.x;
     const int64_t b = (int64_t)e * a.cap;
     const int L = max(0, min(a.len[e], a.cap));
     if (L == 0) return;
+    if (lane == 0) nc = 0;
+    __syncthreads();
     for (int q = lane; q < L; q += kWaveLanes) {
         const int64_t p = b + q;
         const int i = a.cidx[p];
         int s = 0;
         if (i >= 0 && a.ct_cnt[i] >= 2) {
             const int64_t r = a.ct_rep[i];
-            s = (p == r || (a.sz[p] == a.sz[r] && same_body(a, p, r, a.sz[p]))) ? 1 : 0;
+            if (p == r) {
+                s = 1;
+            } else if (a.sz[p] == a.sz[r]) {  // body compared by the wave below
+                const int k = atomicAdd(&nc, 1);
+                cp[k] = (int32_t)p;
+                cr[k] = (int32_t)r;
+                cn[k] = a.sz[p];
+            }
         }
         code[q] = s;
         cnt[q] = 0;
         if (s) a.sh[p] = 1;
     }
     if (lane == 0) cnt[L] = 0;
+    __syncthreads();
+    const int ncand = nc;
+    wave_compare(a, ncand, cp, cr, cn, coff, mism, part, lane, [&](int r, const int32_t *&t, const float *&v) {
+        t = a.tok + r;
+        v = a.val + r;
+    });
+    for (int k = lane; k < ncand; k += kWaveLanes)
+        if (!mism[k]) {
+            code[cp[k] - b] = 1;
+            a.sh[cp[k]] = 1;
+        }
     __syncthreads();
     for (int q = lane; q < L; q += kWaveLanes)
         if (code[q]) {
@@ -558,13 +630,13 @@ __global__ void k_dm_nested(DmArgs a)
     if (a.ct_root[i] == kNone && a.ct_wr[i] >= 0 && a.ct_slot[a.ct_wr[i]] >= 0) a.flag[a.ct_rep[i]] = 2;
 }
 
+// The slot's token body is copied by k_dm_index (one wave per claimed slot).
 SR_DEV void claim_slot(const DmArgs &a, int i, int slot, int64_t body)
 {
-    const int n = a.sz[body];
     a.ct_slot[i] = slot;
     a.s_key[slot] = a.ct_key[i];
-    a.s_len[slot] = n;
-    copy_tokens(a.s_tok + (int64_t)slot * a.cap, a.s_val + (int64_t)slot * a.cap, a.tok + body, a.val + body, n);
+    a.s_len[slot] = a.sz[body];
+    a.s_src[slot] = (int32_t)body;
     a.s_live[slot] = 1;
     a.s_has[slot] = 0;
     a.s_seen[slot] = (uint32_t)a.ctr[C_GEN];
@@ -598,10 +670,24 @@ __global__ void k_dm_alloc(DmArgs a, int roots)
 }
 
 // Slot lookup table rebuilt from the live slots (i < C); every shared
-// position learns its key's slot (i < E * cap).
+// position learns its key's slot (i < E * cap); the bodies of the slots
+// claimed this generation (s_free[0, L1 + NEST)) are copied, a wave per slot.
 __global__ void k_dm_index(DmArgs a)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    {
+        const int lane = threadIdx.x & 63;
+        const int claims = a.ctr[C_L1] + a.ctr[C_NEST];
+        const int waves = (int)(gridDim.x * (blockDim.x / 64));
+        for (int j = (int)(p / 64); j < claims; j += waves) {
+            const int slot = a.s_free[j];
+            const int64_t src = a.s_src[slot], dst = (int64_t)slot * a.cap;
+            for (int k = lane; k < a.s_len[slot]; k += 64) {
+                a.s_tok[dst + k] = a.tok[src + k];
+                a.s_val[dst + k] = a.val[src + k];
+            }
+        }
+    }
     if (p < a.C && a.s_live[p]) {
         const uint64_t h = a.s_key[p];
         for (uint32_t i = (uint32_t)mix64(h) & (a.H - 1);; i = (i + 1) & (a.H - 1)) {
@@ -878,6 +964,7 @@ DmArgs dm_args(Ctx &c, int b)
     a.s_key = d.s_key.as<uint64_t>();
     a.s_len = d.s_len.as<int32_t>();
     a.s_tok = d.s_tok.as<int32_t>();
+    a.s_src = d.s_src.as<int32_t>();
     a.s_val = d.s_val.as<float>();
     a.s_seen = d.s_seen.as<uint32_t>();
     a.s_live = d.s_live.as<int32_t>();
@@ -958,6 +1045,7 @@ int dmemo_setup(Ctx &c, int capacity, int min_size)
     need(d.cols, (size_t)C * (size_t)c.N * 4);
     need(d.s_key, (size_t)C * 8);
     need(d.s_len, (size_t)C * 4);
+    need(d.s_src, (size_t)C * 4);
     need(d.s_tok, (size_t)C * cap * 4);
     need(d.s_val, (size_t)C * cap * 4);
     need(d.s_seen, (size_t)C * 4);
@@ -1034,8 +1122,8 @@ int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *
     DM_LAUNCH(k_dm_clear, std::max<int64_t>(big, d.H2));
     DM_WAVES(k_dm_hash, d.E, 12 * ((size_t)d.cap + 1));
     DM_LAUNCH(k_dm_lookup, M);
-    DM_WAVES(k_dm_count, d.E, 4 * ((size_t)d.cap + 1));
-    DM_WAVES(k_dm_share, d.E, 4 * (2 * (size_t)d.cap + 1));
+    DM_WAVES(k_dm_count, d.E, 4 * (7 * (size_t)d.cap + 1));
+    DM_WAVES(k_dm_share, d.E, 4 * (7 * (size_t)d.cap + 1));
     DM_LAUNCH(k_dm_keys, all_keys);
     if (int r = scan(1, 0, C_FREE)) return r;
     if (int r = scan(0, 1, C_ROOTS)) return r;
