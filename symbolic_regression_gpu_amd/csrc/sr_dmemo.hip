@@ -351,58 +351,16 @@ __global__ void k_dm_lookup(DmArgs a)
 
 // ---- 3. executed subtrees without a column, counted by key (one wave per
 // expression) -------------------------------------------------------------------
-// A position inside a hit's span is read from the column, not executed: a
-// difference array over the hits' spans (+1 after the first position, -1 at
-// the end), scanned.  An expression whose whole tree has a column and a
-// remembered MSE runs nothing.
-__global__ __launch_bounds__(64) void k_dm_count(DmArgs a)
+// cnt[q] = how many spans [s + 1, s + sz[s]) of the marked positions s hold
+// q, for q < L: a difference array over the spans, scanned.  Every lane must
+// call it.
+template <class Mark>
+SR_DEV void wave_cover(const DmArgs &a, int64_t b, int L, int32_t *cnt, int32_t *part, int lane, Mark mark)
 {
-    extern __shared__ int32_t lds32[];
-    const int cap = a.cap;
-    int32_t *cnt = lds32;  // [cap + 1]
-    int32_t *lhit = cnt + cap + 1, *cp = lhit + cap, *cr = cp + cap, *cn = cr + cap, *coff = cn + cap,
-            *mism = coff + cap;  // [cap] each
-    __shared__ int32_t part[kWaveLanes];
-    __shared__ int32_t nc;
-    const int e = blockIdx.x, lane = threadIdx.x;
-    const int64_t b = (int64_t)e * a.cap;
-    const int L = max(0, min(a.len[e], a.cap));
-    if (L == 0) return;
-    // the lookup's pending hits: bodies compared by the wave
-    if (lane == 0) nc = 0;
-    __syncthreads();
-    for (int q = lane; q < L; q += kWaveLanes) {
-        const int h = a.hit[b + q];
-        lhit[q] = h;
-        if (h <= kHitPending) {
-            const int k = atomicAdd(&nc, 1);
-            cp[k] = (int32_t)(b + q);
-            cr[k] = kHitPending - h;
-            cn[k] = a.sz[b + q];
-        }
-    }
-    __syncthreads();
-    const int ncand = nc;
-    wave_compare(a, ncand, cp, cr, cn, coff, mism, part, lane, [&](int slot, const int32_t *&t, const float *&v) {
-        t = a.s_tok + (int64_t)slot * a.cap;
-        v = a.s_val + (int64_t)slot * a.cap;
-    });
-    const uint32_t gen = (uint32_t)a.ctr[C_GEN];
-    for (int k = lane; k < ncand; k += kWaveLanes) {
-        const int q = (int)(cp[k] - b), slot = cr[k];
-        const int h = mism[k] ? -2 : slot;
-        lhit[q] = h;
-        a.hit[cp[k]] = h;
-        // popular slots are hit thousands of times: store only when it changes
-        if (h >= 0 && a.s_seen[slot] != gen) a.s_seen[slot] = gen;
-    }
-    __syncthreads();
-    const int r0 = lhit[0];
-    if (r0 >= 0 && a.s_has[r0]) return;  // a settled expression runs nothing
     for (int q = lane; q <= L; q += kWaveLanes) cnt[q] = 0;
     __syncthreads();
     for (int q = lane; q < L; q += kWaveLanes)
-        if (lhit[q] >= 0) {
+        if (mark(q)) {
             atomicAdd(&cnt[q + 1], 1);
             atomicSub(&cnt[min(q + a.sz[b + q], L)], 1);
         }
@@ -413,8 +371,80 @@ __global__ __launch_bounds__(64) void k_dm_count(DmArgs a)
     int open = lane_prefix_sum(part, t, lane);
     for (int q = lo; q < hi; ++q) {
         open += cnt[q];
+        cnt[q] = open;
+    }
+    __syncthreads();
+}
+
+// First the lookup's pending hits: the outermost ones (inside no other
+// pending span) have their bodies compared by the wave; a hit inside a
+// verified one is never read as a column (its positions are read through
+// the outer one) and only keeps its slot alive, so it is taken as is —
+// unless some outermost compare failed, then all are compared.  Then a
+// position inside a hit's span is read from the column, not executed; an
+// expression whose whole tree has a column and a remembered MSE runs nothing;
+// the executed positions without a column are counted by key.
+__global__ __launch_bounds__(64) void k_dm_count(DmArgs a)
+{
+    extern __shared__ int32_t lds32[];
+    const int cap = a.cap;
+    int32_t *cnt = lds32;  // [cap + 1]
+    int32_t *lhit = cnt + cap + 1, *cp = lhit + cap, *cr = cp + cap, *cn = cr + cap, *coff = cn + cap,
+            *mism = coff + cap;  // [cap] each
+    __shared__ int32_t part[kWaveLanes];
+    __shared__ int32_t nc, failed;
+    const int e = blockIdx.x, lane = threadIdx.x;
+    const int64_t b = (int64_t)e * a.cap;
+    const int L = max(0, min(a.len[e], a.cap));
+    if (L == 0) return;
+    const uint32_t gen = (uint32_t)a.ctr[C_GEN];
+    for (int q = lane; q < L; q += kWaveLanes) lhit[q] = a.hit[b + q];
+    if (lane == 0) failed = 0;
+    __syncthreads();
+    auto pending = [&](int q) { return lhit[q] <= kHitPending; };
+    auto settle = [&](int q, int slot, bool same) {
+        const int h = same ? slot : -2;
+        lhit[q] = h;
+        a.hit[b + q] = h;
+        // popular slots are hit thousands of times: store only when it changes
+        if (same && a.s_seen[slot] != gen) a.s_seen[slot] = gen;
+    };
+    auto body2 = [&](int slot, const int32_t *&t, const float *&v) {
+        t = a.s_tok + (int64_t)slot * a.cap;
+        v = a.s_val + (int64_t)slot * a.cap;
+    };
+    // compares the pending positions that pass `pick`, settles them
+    auto compare = [&](auto pick) {
+        if (lane == 0) nc = 0;
+        __syncthreads();
+        for (int q = lane; q < L; q += kWaveLanes)
+            if (pending(q) && pick(q)) {
+                const int k = atomicAdd(&nc, 1);
+                cp[k] = (int32_t)(b + q);
+                cr[k] = kHitPending - lhit[q];
+                cn[k] = a.sz[b + q];
+            }
+        __syncthreads();
+        const int ncand = nc;
+        wave_compare(a, ncand, cp, cr, cn, coff, mism, part, lane, body2);
+        for (int k = lane; k < ncand; k += kWaveLanes) {
+            settle((int)(cp[k] - b), cr[k], !mism[k]);
+            if (mism[k]) failed = 1;
+        }
+        __syncthreads();
+    };
+    wave_cover(a, b, L, cnt, part, lane, pending);
+    compare([&](int q) { return cnt[q] == 0; });
+    if (failed) compare([&](int q) { return true; });
+    for (int q = lane; q < L; q += kWaveLanes)
+        if (pending(q)) settle(q, kHitPending - lhit[q], true);
+    __syncthreads();
+    const int r0 = lhit[0];
+    if (r0 >= 0 && a.s_has[r0]) return;  // a settled expression runs nothing
+    wave_cover(a, b, L, cnt, part, lane, [&](int q) { return lhit[q] >= 0; });
+    for (int q = lane; q < L; q += kWaveLanes) {
         const int64_t p = b + q;
-        if (open > 0 || lhit[q] != -1) continue;
+        if (cnt[q] > 0 || lhit[q] != -1) continue;
         const uint64_t h = a.hh[p];
         for (uint32_t i = (uint32_t)mix64(h) & (a.H2 - 1);; i = (i + 1) & (a.H2 - 1)) {
             uint64_t k = a.ct_key[i];
