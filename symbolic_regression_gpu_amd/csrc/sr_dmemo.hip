@@ -222,7 +222,23 @@ __global__ void k_dm_clear(DmArgs a)
     }
 }
 
-// ---- 1. sizes and subtree keys (one wave per expression) ---------------------
+// ---- lookup of a subtree among the slots ------------------------------------------
+// hit value: slot (>= 0, body verified), -1 no key, -2 another body's key, -3
+// not looked up, kHitPending - slot: key and length match, body compared
+// later (by k_dm_count's wave).
+SR_DEV int lookup(const DmArgs &a, uint64_t h, int n)
+{
+    for (uint32_t i = (uint32_t)mix64(h) & (a.H - 1);; i = (i + 1) & (a.H - 1)) {
+        const uint64_t k = a.ht_key[i];
+        if (k == 0) return -1;
+        if (k != h) continue;
+        const int s = a.ht_slot[i];
+        return a.s_len[s] == n ? kHitPending - s : -2;
+    }
+}
+
+// ---- 1. sizes, subtree keys and the lookup of every subtree of >= min_size
+// tokens (one wave per expression) ------------------------------------------------
 // A subtree is the token span [i, i + size) of the prefix stream.  With
 // S[k] = sum over tokens before k of (1 - arity), its end is the first
 // j > i with S[j] = S[i] + 1; S rises by at most 1 per token, so a search
@@ -317,35 +333,11 @@ __global__ __launch_bounds__(64) void k_dm_hash(DmArgs a)
         }
         j = min(j, L);
         const int n = j - i;
-        const uint64_t x = mix64((lu[j] - lu[i]) * kHashPowers.pw[i] + (uint64_t)n * 0xbf58476d1ce4e5b9ULL);
-        a.hh[b + i] = x ? x : 1;
+        uint64_t x = mix64((lu[j] - lu[i]) * kHashPowers.pw[i] + (uint64_t)n * 0xbf58476d1ce4e5b9ULL);
+        x = x ? x : 1;
+        a.hh[b + i] = x;
         a.sz[b + i] = n;
-    }
-}
-
-// ---- 2. lookup of every subtree of >= min_size tokens -------------------------
-// hit[p]: slot (>= 0, body verified), -1 no key, -2 another body's key, -3 not
-// looked up, kHitPending - slot: key and length match, body not yet compared. ------------------------
-__global__ void k_dm_lookup(DmArgs a)
-{
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (int64_t)a.E * a.cap) return;
-    const int e = (int)(p / a.cap), q = (int)(p - (int64_t)e * a.cap);
-    if (q >= a.len[e] || a.sz[p] < a.ms) return;
-    const uint64_t h = a.hh[p];
-    const int n = a.sz[p];
-    for (uint32_t i = (uint32_t)mix64(h) & (a.H - 1);; i = (i + 1) & (a.H - 1)) {
-        const uint64_t k = a.ht_key[i];
-        if (k == 0) {
-            a.hit[p] = -1;
-            return;
-        }
-        if (k != h) continue;
-        const int s = a.ht_slot[i];
-        // equal lengths: the token bodies are compared by k_dm_count's wave;
-        // else the key belongs to another body: never cached
-        a.hit[p] = a.s_len[s] == n ? kHitPending - s : -2;
-        return;
+        if (n >= a.ms) a.hit[b + i] = lookup(a, x, n);
     }
 }
 
@@ -585,9 +577,14 @@ __global__ void k_dm_keys(DmArgs a)
     if (i < a.H) a.ht_key[i] = 0;
 }
 
-// Ordered compaction in three passes (256 items per block): count the
-// selected items of each block, scan the block counts, then give each
-// selected item its rank.  Selection: kind 0 = flag[i] == v over positions
+SR_DEV int wave_sum_i32(int v)
+{
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// Ordered compaction in two passes (256 items per block): count the selected
+// items of each block, then give each selected item its rank.  Selection: kind 0 = flag[i] == v over positions
 // (rank[i] written), kind 1 = a free slot (s_free[rank] = i).
 SR_DEV bool scan_pick(const DmArgs &a, int kind, int v, int64_t i)
 {
@@ -607,44 +604,27 @@ __global__ __launch_bounds__(256) void k_dm_scan_count(DmArgs a, int kind, int v
     if (threadIdx.x == 0) blk[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
 }
 
-// Exclusive scan of nb block counts in place (one workgroup); the total goes
-// to ctr[ci].
-__global__ __launch_bounds__(1024) void k_dm_scan_blocks(DmArgs a, int32_t *blk, int nb, int ci)
+// Each block's offset is the sum of the counts of the blocks before it
+// (summed by the block itself); the last block writes the total to ctr[ci].
+__global__ __launch_bounds__(256) void k_dm_scan_write(DmArgs a, int kind, int v, const int32_t *blk, int ci)
 {
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
-    const int per = (nb + 1023) / 1024;
-    const int lo = min(nb, t * per), hi = min(nb, lo + per);
-    int n = 0;
-    for (int k = lo; k < hi; ++k) n += blk[k];
-    part[t] = n;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-        const int w = t >= d ? part[t - d] : 0;
-        __syncthreads();
-        part[t] += w;
-        __syncthreads();
-    }
-    int at = part[t] - n;
-    for (int k = lo; k < hi; ++k) {
-        const int c = blk[k];
-        blk[k] = at;
-        at += c;
-    }
-    if (t == 1023) a.ctr[ci] = part[1023];
-}
-
-__global__ __launch_bounds__(256) void k_dm_scan_write(DmArgs a, int kind, int v, const int32_t *blk)
-{
-    __shared__ int wc[4];
+    __shared__ int wc[4], ws[4];
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int w = threadIdx.x >> 6;
+    int before = 0;
+    for (int k = threadIdx.x; k < (int)blockIdx.x; k += 256) before += blk[k];
+    before = wave_sum_i32(before);
     const bool pick = i < scan_n(a, kind) && scan_pick(a, kind, v, i);
     const uint64_t m = __ballot(pick);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) wc[w] = __popcll(m);
+    if ((threadIdx.x & 63) == 0) {
+        wc[w] = __popcll(m);
+        ws[w] = before;
+    }
     __syncthreads();
+    const int base = ws[0] + ws[1] + ws[2] + ws[3];
+    if (threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) a.ctr[ci] = base + blk[blockIdx.x];
     if (!pick) return;
-    int r = blk[blockIdx.x];
+    int r = base;
     for (int k = 0; k < w; ++k) r += wc[k];
     r += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
     if (kind == 0) a.rank[i] = r;
@@ -1143,15 +1123,13 @@ int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *
         const int64_t n = kind == 0 ? M : d.C;
         const int nb = (int)((n + 255) / 256);
         hipLaunchKernelGGL(k_dm_scan_count, dim3(nb), dim3(256), 0, s, a, kind, v, blk);
-        hipLaunchKernelGGL(k_dm_scan_blocks, dim3(1), dim3(1024), 0, s, a, blk, nb, ci);
-        hipLaunchKernelGGL(k_dm_scan_write, dim3(nb), dim3(256), 0, s, a, kind, v, (const int32_t *)blk);
+        hipLaunchKernelGGL(k_dm_scan_write, dim3(nb), dim3(256), 0, s, a, kind, v, (const int32_t *)blk, ci);
         if ((he = hipGetLastError()) != hipSuccess) return c.hip_fail(he, "device cache scan");
         return SR_OK;
     };
     const int64_t all_keys = std::max<int64_t>(d.H2, std::max(d.C, d.H));
     DM_LAUNCH(k_dm_clear, std::max<int64_t>(big, d.H2));
     DM_WAVES(k_dm_hash, d.E, 12 * ((size_t)d.cap + 1));
-    DM_LAUNCH(k_dm_lookup, M);
     DM_WAVES(k_dm_count, d.E, 4 * (7 * (size_t)d.cap + 1));
     DM_WAVES(k_dm_share, d.E, 4 * (7 * (size_t)d.cap + 1));
     DM_LAUNCH(k_dm_keys, all_keys);

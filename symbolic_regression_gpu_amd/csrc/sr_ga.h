@@ -111,7 +111,83 @@ struct Rng {
         const uint32_t range = (uint32_t)(hi - lo) + 1u;
         return lo + (int)(((uint64_t)next() * range) >> 32);
     }
+    // uniform(0, 1) < p with t = chance_threshold(p), in integers (the same
+    // draw, the same outcome)
+    SR_HD bool chance(uint32_t t) { return (next() >> 8) < t; }
 };
+
+#if defined(__HIPCC__)
+// The same stream for a wave whose 64 lanes all consume it alike: lane k
+// holds Philox block base / 4 + k, so one parallel round of Philox gives the
+// next 256 words, and a word is a read of one lane.  Must be used with every
+// lane of the wave active, in uniform control flow.
+struct WaveRng {
+    uint32_t k0, k1, c1, c2, c3;
+    uint32_t word, base;
+    uint32_t b0, b1, b2, b3;
+
+    __device__ void init(uint64_t seed, uint32_t stream, uint32_t gen, uint32_t domain)
+    {
+        k0 = (uint32_t)seed;
+        k1 = (uint32_t)(seed >> 32);
+        c1 = stream;
+        c2 = gen;
+        c3 = domain;
+        word = 0;
+        fill();
+    }
+    __device__ void fill()
+    {
+        base = word & ~3u;
+        uint32_t o[4];
+        philox4x32_10((base >> 2) + (uint32_t)(threadIdx.x & 63), c1, c2, c3, k0, k1, o);
+        b0 = o[0];
+        b1 = o[1];
+        b2 = o[2];
+        b3 = o[3];
+    }
+    __device__ uint32_t next()
+    {
+        if (word - base >= 256u) fill();
+        const uint32_t idx = word - base;
+        ++word;
+        const int src = (int)(idx >> 2);
+        switch (idx & 3u) {
+        case 0: return (uint32_t)__builtin_amdgcn_readlane((int)b0, src);
+        case 1: return (uint32_t)__builtin_amdgcn_readlane((int)b1, src);
+        case 2: return (uint32_t)__builtin_amdgcn_readlane((int)b2, src);
+        default: return (uint32_t)__builtin_amdgcn_readlane((int)b3, src);
+        }
+    }
+    __device__ float uniform(float lo, float hi)
+    {
+        const float u = (float)(next() >> 8) * (1.0f / 16777216.0f);
+        const float span = hi - lo;
+        const float t = span * u;
+        return lo + t;
+    }
+    __device__ int randint(int lo, int hi)
+    {
+        if (hi <= lo) return lo;
+        const uint32_t range = (uint32_t)(hi - lo) + 1u;
+        return lo + (int)(((uint64_t)next() * range) >> 32);
+    }
+    __device__ bool chance(uint32_t t) { return (next() >> 8) < t; }
+};
+#endif
+
+// uniform(0, 1) is exactly m / 2^24 (m = next() >> 8; the product by 1 and
+// the sum with 0 are exact), so uniform(0, 1) < p  <=>  m < p * 2^24  <=>
+// m < ceil(p * 2^24) for p in (0, 1); never for p <= 0 or NaN, always for
+// p >= 1.
+SR_HD uint32_t chance_threshold(float p)
+{
+    if (!(p > 0.0f)) return 0u;
+    if (p >= 1.0f) return 1u << 24;
+    const float x = p * 16777216.0f;  // exact: a power of two
+    const uint32_t f = (uint32_t)x;
+    return (float)f == x ? f : f + 1u;
+}
 
 // opcodes.h:42-78 arity: unary 10..27 (and the clamped exp 29), everything
 // else binary; leaves have none.
@@ -162,6 +238,8 @@ SR_HD __attribute__((always_inline)) int ga_generate_with(const GaParams &p, int
     if (budget <= 0) return 0;
     int left = p.max_tokens < budget ? p.max_tokens : budget;
     const int nvars = p.num_vars > 1 ? p.num_vars : 1;
+    const uint32_t t_leaf = chance_threshold(p.prob_leaf), t_const = chance_threshold(p.prob_const_leaf),
+                   t_unary = chance_threshold(p.prob_unary);
     int n = 0;
     stack.push(-1);
     while (stack.sp > 0) {
@@ -180,11 +258,11 @@ SR_HD __attribute__((always_inline)) int ga_generate_with(const GaParams &p, int
             continue;
         }
         bool leaf = d >= p.max_depth || left == 1;
-        if (!leaf && r.uniform(0.0f, 1.0f) < p.prob_leaf) leaf = true;
+        if (!leaf && r.chance(t_leaf)) leaf = true;
         if (!leaf && !(left >= 2)) leaf = true;
         if (leaf) {
             if (n >= cap) return -1;
-            bool is_const = r.uniform(0.0f, 1.0f) < p.prob_const_leaf;
+            bool is_const = r.chance(t_const);
             const float lv = is_const ? r.uniform(p.const_min, p.const_max) : (float)r.randint(0, nvars - 1);
             if (store) {
                 tok[n] = is_const ? TOK_CONST : TOK_VAR;
@@ -194,9 +272,7 @@ SR_HD __attribute__((always_inline)) int ga_generate_with(const GaParams &p, int
             --left;
             continue;
         }
-        float thresh = p.prob_unary;
-        if (!(left >= 3)) thresh = 1.0f;
-        const bool unary = r.uniform(0.0f, 1.0f) < thresh;
+        const bool unary = r.chance(left >= 3 ? t_unary : 1u << 24);
         int op, arity;
         if (unary) {
             op = ops.unary(r.randint(0, p.n_unary - 1));
@@ -320,7 +396,7 @@ SR_HD int ga_child(const GaParams &p, const SelectParams &s, int pop, int cap, c
     Rng r;
     r.init(seed, (uint32_t)c, gen, GA_DOMAIN_NEXT);
     const int b1 = order[ga_tournament(r, fit, order, pop, s.tournament)];
-    if (r.uniform(0.0f, 1.0f) < s.p_mutation)
+    if (r.chance(chance_threshold(s.p_mutation)))
         return ga_mutate(p, tok + (size_t)b1 * cap, val + (size_t)b1 * cap, len[b1], r, ct, cv, cap);
     const int b2 = order[ga_tournament(r, fit, order, pop, s.tournament)];
     return ga_crossover(p, tok + (size_t)b1 * cap, val + (size_t)b1 * cap, len[b1],
@@ -419,7 +495,9 @@ struct GaLdsStack {
     int16_t *a;
     int sp = 0;
     __device__ void push(int v) { a[sp++] = (int16_t)v; }
-    __device__ int pop() { return a[--sp]; }
+    // the value is the same in every lane: saying so keeps the generator's
+    // decisions on the scalar unit
+    __device__ int pop() { return __builtin_amdgcn_readfirstlane(a[--sp]); }
 };
 
 __device__ inline int ga_child_wave(const GaParams &p, const SelectParams &s, int pop, int cap,
@@ -432,14 +510,14 @@ __device__ inline int ga_child_wave(const GaParams &p, const SelectParams &s, in
         ga_wave_copy(tok + (size_t)e * cap, val + (size_t)e * cap, len[e], ct, cv, lane);
         return len[e];
     }
-    Rng r;
+    WaveRng r;
     r.init(seed, (uint32_t)c, gen, GA_DOMAIN_NEXT);
     const int b1 = order[ga_tournament_wave(r, fit, order, pop, s.tournament)];
     const int32_t *pt = tok + (size_t)b1 * cap;
     const float *pv = val + (size_t)b1 * cap;
     const int plen = len[b1];
     ga_wave_stage(pt, plen, l1, lane);
-    if (r.uniform(0.0f, 1.0f) < s.p_mutation) {  // ga_mutate
+    if (r.chance(chance_threshold(s.p_mutation))) {  // ga_mutate
         if (plen <= 0) {
             GaLdsStack st{gstack};
             return ga_generate_with(p, p.max_tokens, r, ct, cv, cap, lane == 0, st, GaParamOps{p});

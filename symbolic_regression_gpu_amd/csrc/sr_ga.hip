@@ -52,7 +52,7 @@ __global__ __launch_bounds__(256) void k_ga_next(GaParams p, SelectParams s, int
                                                  int32_t *otok, float *oval, int32_t *olen, int32_t *status)
 {
     extern __shared__ int32_t ga_lds[];
-    const int w = threadIdx.x / 64, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x & 63;
     // the child index is wave-uniform; saying so keeps the whole decision
     // path (Philox, tournaments, the generator) on the scalar unit
     const int c = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x / 64)) + w);
