@@ -175,7 +175,8 @@ __global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
                 return nullptr;
             };
             float top[kVecD], o[kVecD], y[kVecD];
-            load(a.X + (int64_t)a.nv * N + t0, y);  // the target, needed after the program
+            // the target, needed after the program (only when this item's MSE is)
+            if (need) load(a.X + (int64_t)a.nv * N + t0, y);
             vec_each([&](int k) { top[k] = 0.0f; });
             int sp = 0;
             // operands two instructions ahead: instruction i reads buffer
