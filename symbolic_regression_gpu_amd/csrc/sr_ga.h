@@ -173,6 +173,16 @@ struct WaveRng {
         return lo + (int)(((uint64_t)next() * range) >> 32);
     }
     __device__ bool chance(uint32_t t) { return (next() >> 8) < t; }
+    // consume a word whose value is computed elsewhere (word_at)
+    __device__ void skip() { ++word; }
+    // word w of the stream, computed on its own (any lane, any w)
+    __device__ uint32_t word_at(uint32_t w) const
+    {
+        uint32_t o[4];
+        philox4x32_10(w >> 2, c1, c2, c3, k0, k1, o);
+        const uint32_t j = w & 3u;
+        return j == 0u ? o[0] : (j == 1u ? o[1] : (j == 2u ? o[2] : o[3]));
+    }
 };
 #endif
 
@@ -500,6 +510,89 @@ struct GaLdsStack {
     __device__ int pop() { return __builtin_amdgcn_readfirstlane(a[--sp]); }
 };
 
+// ga_generate for a wave, in two passes.  The draws that decide the shape
+// (leaf? constant? unary?) are made in order by the wave; a token's value
+// draw (constant, variable index, operator index) is only counted, its word
+// index kept in plan[k] with the token kind in bits 28..31.  The lanes then
+// compute every token's value from its word at once.  Same draws, same
+// words, same tokens as ga_generate (utils.cpp:995-1108).
+enum { GEN_FALLBACK = 0, GEN_CONST, GEN_VAR, GEN_UNARY, GEN_BINARY };
+
+__device__ inline int ga_generate_wave(const GaParams &p, int budget, WaveRng &r, int32_t *tok, float *val,
+                                       int cap, GaLdsStack &stack, int32_t *plan, int lane)
+{
+    if (budget <= 0) return 0;
+    int left = p.max_tokens < budget ? p.max_tokens : budget;
+    const int nvars = p.num_vars > 1 ? p.num_vars : 1;
+    const uint32_t t_leaf = chance_threshold(p.prob_leaf), t_const = chance_threshold(p.prob_const_leaf),
+                   t_unary = chance_threshold(p.prob_unary);
+    int n = 0;
+    stack.push(-1);
+    while (stack.sp > 0) {
+        int d = stack.pop();
+        const bool top = d < 0;
+        if (top) d = 0;
+        if (left <= 0) {
+            if (!top) {  // budget exhausted under a pending child: fallback leaf
+                if (n >= cap) return -1;
+                plan[n++] = GEN_FALLBACK << 28;
+            }
+            continue;
+        }
+        bool leaf = d >= p.max_depth || left == 1;
+        if (!leaf && r.chance(t_leaf)) leaf = true;
+        if (leaf) {
+            if (n >= cap) return -1;
+            const bool is_const = r.chance(t_const);
+            plan[n++] = ((is_const ? GEN_CONST : GEN_VAR) << 28) | (int32_t)r.word;
+            if (is_const || nvars > 1) r.skip();
+            --left;
+            continue;
+        }
+        const bool unary = r.chance(left >= 3 ? t_unary : 1u << 24);
+        const int32_t w = (int32_t)r.word;
+        if (unary ? p.n_unary > 1 : p.n_binary > 1) r.skip();
+        if (n >= cap) return -1;
+        plan[n++] = ((unary ? GEN_UNARY : GEN_BINARY) << 28) | w;
+        --left;
+        if (stack.sp + (unary ? 1 : 2) > GA_GEN_STACK) return -1;
+        stack.push(d + 1);
+        if (!unary) stack.push(d + 1);
+    }
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the plan is in LDS
+    __builtin_amdgcn_wave_barrier();
+    for (int k = lane; k < n; k += 64) {
+        const int32_t e = plan[k];
+        const int kind = (e >> 28) & 15;
+        const uint32_t m = kind == GEN_FALLBACK ? 0u : r.word_at((uint32_t)(e & 0x0fffffff));
+        int t = TOK_VAR;
+        float v = 0.0f;
+        if (kind == GEN_CONST) {  // uniform(const_min, const_max)
+            t = TOK_CONST;
+            const float u = (float)(m >> 8) * (1.0f / 16777216.0f);
+            const float span = p.const_max - p.const_min;
+            const float tt = span * u;
+            v = p.const_min + tt;
+        } else if (kind == GEN_VAR) {  // randint(0, nvars - 1)
+            v = nvars > 1 ? (float)(int)(((uint64_t)m * (uint32_t)nvars) >> 32) : 0.0f;
+        } else if (kind == GEN_UNARY) {
+            t = p.unary_ops[p.n_unary > 1 ? (int)(((uint64_t)m * (uint32_t)p.n_unary) >> 32) : 0];
+        } else if (kind == GEN_BINARY) {
+            t = p.binary_ops[p.n_binary > 1 ? (int)(((uint64_t)m * (uint32_t)p.n_binary) >> 32) : 0];
+        }
+        tok[k] = t;
+        val[k] = v;
+    }
+    if (n == 0) {
+        if (lane == 0) {
+            tok[0] = TOK_CONST;
+            val[0] = 0.0f;
+        }
+        n = 1;
+    }
+    return n;
+}
+
 __device__ inline int ga_child_wave(const GaParams &p, const SelectParams &s, int pop, int cap,
                                     const int32_t *tok, const float *val, const int32_t *len, const float *fit,
                                     const int32_t *order, uint64_t seed, uint32_t gen, int c, int32_t *ct,
@@ -520,7 +613,7 @@ __device__ inline int ga_child_wave(const GaParams &p, const SelectParams &s, in
     if (r.chance(chance_threshold(s.p_mutation))) {  // ga_mutate
         if (plen <= 0) {
             GaLdsStack st{gstack};
-            return ga_generate_with(p, p.max_tokens, r, ct, cv, cap, lane == 0, st, GaParamOps{p});
+            return ga_generate_wave(p, p.max_tokens, r, ct, cv, cap, st, l2, lane);
         }
         const int idx = r.randint(0, plen - 1);
         int old = ga_subtree_size_s(l1, plen, idx);
@@ -535,7 +628,7 @@ __device__ inline int ga_child_wave(const GaParams &p, const SelectParams &s, in
         const int tail = plen - (idx + old);
         const int room = cap - idx - tail;
         GaLdsStack st{gstack};
-        const int nl = ga_generate_with(p, max_new, r, ct + idx, cv + idx, room, lane == 0, st, GaParamOps{p});
+        const int nl = ga_generate_wave(p, max_new, r, ct + idx, cv + idx, room, st, l2, lane);
         if (nl < 0) return -1;
         ga_wave_copy(pt + idx + old, pv + idx + old, tail, ct + idx + nl, cv + idx + nl, lane);
         return idx + nl + tail;
