@@ -554,8 +554,9 @@ __global__ __launch_bounds__(64) void k_dm_share(DmArgs a)
 // holding its first occurrence, at that occurrence's offset.  Per slot
 // (i < C): slots whose subtree the population no longer holds are evicted.
 // The slot lookup table (i < H) is cleared for its rebuild.
-__global__ void k_dm_keys(DmArgs a)
+__global__ __launch_bounds__(256) void k_dm_keys(DmArgs a, int32_t *blk)
 {
+    __shared__ int wc[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < a.H2 && a.ct_cnt[i] >= 2) {
         const int32_t root = a.ct_root[i];
@@ -570,11 +571,21 @@ __global__ void k_dm_keys(DmArgs a)
             }
         }
     }
-    if (i < a.C && a.s_live[i] && a.s_seen[i] != (uint32_t)a.ctr[C_GEN]) {
-        a.s_live[i] = 0;
-        a.s_has[i] = 0;
+    bool free_slot = false;
+    if (i < a.C) {
+        free_slot = !a.s_live[i];
+        if (!free_slot && a.s_seen[i] != (uint32_t)a.ctr[C_GEN]) {
+            a.s_live[i] = 0;
+            a.s_has[i] = 0;
+            free_slot = true;
+        }
     }
     if (i < a.H) a.ht_key[i] = 0;
+    // the free-slot compaction's block counts (blocks of the slot range)
+    const uint64_t m = __ballot(free_slot);
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0 && (int64_t)blockIdx.x * 256 < a.C) blk[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
 }
 
 SR_DEV int wave_sum_i32(int v)
@@ -1119,10 +1130,11 @@ int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *
         if ((he = hipGetLastError()) != hipSuccess) return c.hip_fail(he, #K);                   \
     } while (0)
     int32_t *blk = d.blk.as<int32_t>();
+    // (the free-slot compaction's counts come from k_dm_keys)
     auto scan = [&](int kind, int v, int ci) -> int {
         const int64_t n = kind == 0 ? M : d.C;
         const int nb = (int)((n + 255) / 256);
-        hipLaunchKernelGGL(k_dm_scan_count, dim3(nb), dim3(256), 0, s, a, kind, v, blk);
+        if (kind == 0) hipLaunchKernelGGL(k_dm_scan_count, dim3(nb), dim3(256), 0, s, a, kind, v, blk);
         hipLaunchKernelGGL(k_dm_scan_write, dim3(nb), dim3(256), 0, s, a, kind, v, (const int32_t *)blk, ci);
         if ((he = hipGetLastError()) != hipSuccess) return c.hip_fail(he, "device cache scan");
         return SR_OK;
@@ -1132,7 +1144,7 @@ int dmemo_enqueue(Ctx &c, int b, hipStream_t s, long long *cnt_out, hipEvent_t *
     DM_WAVES(k_dm_hash, d.E, 12 * ((size_t)d.cap + 1));
     DM_WAVES(k_dm_count, d.E, 4 * (7 * (size_t)d.cap + 1));
     DM_WAVES(k_dm_share, d.E, 4 * (7 * (size_t)d.cap + 1));
-    DM_LAUNCH(k_dm_keys, all_keys);
+    DM_LAUNCH(k_dm_keys, all_keys, blk);
     if (int r = scan(1, 0, C_FREE)) return r;
     if (int r = scan(0, 1, C_ROOTS)) return r;
     DM_LAUNCH(k_dm_alloc, d.H2, 1);

@@ -30,7 +30,9 @@ __global__ void k_ga_init(GaParams p, int pop, int cap, uint64_t seed, int32_t *
 // order[rank] = i under fitness_before (ascending fitness, NaN last, ties by
 // index): one wave per individual, its lanes compare against n / 64 others
 // each and the counts are summed across the wave.
-__global__ __launch_bounds__(256) void k_ga_rank(const float *fit, int n, int32_t *order)
+// With a record (rec.hist set), row (*gen - gen0) of the fitness and length
+// histories is written too (rows outside [gen0, gen0 + max_gens) dropped).
+__global__ __launch_bounds__(256) void k_ga_rank(const float *fit, int n, int32_t *order, GaRecord rec)
 {
     const int i = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) / 64);
     const int lane = threadIdx.x & 63;
@@ -40,7 +42,15 @@ __global__ __launch_bounds__(256) void k_ga_rank(const float *fit, int n, int32_
     for (int j = lane; j < n; j += 64) rank += fitness_before(fit[j], j, fi, i) ? 1 : 0;
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) rank += __shfl_xor(rank, m, 64);
-    if (lane == 0) order[rank] = i;
+    if (lane != 0) return;
+    order[rank] = i;
+    if (rec.hist) {
+        const uint32_t g = *rec.gen - rec.gen0;
+        if (g < (uint32_t)rec.max_gens) {
+            rec.hist[(size_t)g * n + i] = __float_as_uint(fi);
+            rec.hist_len[(size_t)g * n + i] = (uint32_t)rec.len[i];
+        }
+    }
 }
 
 // One wave per child of generation *gen (per wave in dynamic LDS: the
@@ -173,19 +183,6 @@ __global__ __launch_bounds__(1024) void k_order_by_len2(const int32_t *plen0, in
     }
 }
 
-// hist[(*gen - gen0) * pop + i] = src[i] for 4-byte words, for two
-// (src, hist) pairs (the second optional; generations outside
-// [gen0, gen0 + max_gens) are dropped).
-__global__ void k_record(const uint32_t *src0, uint32_t *hist0, const uint32_t *src1, uint32_t *hist1, int pop,
-                         const uint32_t *gen, uint32_t gen0, int max_gens)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t g = *gen - gen0;
-    if (i >= pop || g >= (uint32_t)max_gens) return;
-    hist0[(size_t)g * pop + i] = src0[i];
-    if (src1) hist1[(size_t)g * pop + i] = src1[i];
-}
-
 __global__ void k_u32_add(uint32_t *p, uint32_t v)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p += v;
@@ -206,9 +203,9 @@ hipError_t launch_ga_init(const GaParams &p, int pop, int cap, uint64_t seed, in
     return hipGetLastError();
 }
 
-hipError_t launch_ga_rank(const float *fit, int n, int32_t *order, hipStream_t s)
+hipError_t launch_ga_rank(const float *fit, int n, int32_t *order, hipStream_t s, const GaRecord &rec)
 {
-    hipLaunchKernelGGL(k_ga_rank, dim3(blocks(n, 4)), dim3(256), 0, s, fit, n, order);
+    hipLaunchKernelGGL(k_ga_rank, dim3(blocks(n, 4)), dim3(256), 0, s, fit, n, order, rec);
     return hipGetLastError();
 }
 
@@ -243,14 +240,6 @@ hipError_t launch_order_by_len2(const int32_t *plen0, int n0, int32_t *order0, c
 {
     hipLaunchKernelGGL(k_order_by_len2, dim3(2), dim3(1024), 0, s, plen0, n0, order0, plen1, n1, order1, zero,
                        n_zero, nonzero1);
-    return hipGetLastError();
-}
-
-hipError_t launch_record(const void *src0, void *hist0, const void *src1, void *hist1, int pop,
-                         const uint32_t *gen, uint32_t gen0, int max_gens, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_record, dim3(blocks(pop, 256)), dim3(256), 0, s, (const uint32_t *)src0,
-                       (uint32_t *)hist0, (const uint32_t *)src1, (uint32_t *)hist1, pop, gen, gen0, max_gens);
     return hipGetLastError();
 }
 

@@ -3,7 +3,7 @@
 //
 // One generation in BATCHED mode is, all on the device:
 //   k_compile_pop -> k_order_by_len -> zero counters -> k_batched (MSE) ->
-//   k_finalize -> k_record (fitness, lengths) -> k_ga_rank -> k_ga_next -> gen+1
+//   k_finalize -> k_ga_rank (+ fitness and length history) -> k_ga_next -> gen+1
 // (cl_sr_system.cpp:168-333 runs the same sequence with host selection and
 // variation).  The two buffer parities are captured as two graphs and replayed
 // alternately.
@@ -104,10 +104,10 @@ static SelectParams to_select(const sr_select_params &sp, int pop)
 // Rank fitness fit (device [pop]) and write generation *gen's offspring of
 // buffer b into buffer 1-b.
 static int ga_enqueue_next(Ctx &c, int b, const SelectParams &sp, uint64_t seed, const float *fit,
-                           hipStream_t s)
+                           hipStream_t s, const GaRecord &rec = GaRecord{})
 {
     GaState &g = c.ga;
-    hipError_t he = launch_ga_rank(fit, g.pop, g.order_fit.as<int32_t>(), s);
+    hipError_t he = launch_ga_rank(fit, g.pop, g.order_fit.as<int32_t>(), s, rec);
     if (he == hipSuccess)
         he = launch_ga_next(g.p, sp, g.pop, g.cap, g.tok[b].as<int32_t>(), g.val[b].as<float>(),
                             g.len[b].as<int32_t>(), fit, g.order_fit.as<int32_t>(), seed,
@@ -436,12 +436,17 @@ int sr_ga_run(sr_ctx *x, const sr_select_params *spp, int mode, int32_t gens, ui
         int r = device_cache ? dmemo_enqueue(c, b, s, nullptr, time_launches ? kev : nullptr)
                              : ga_enqueue_eval(c, b, lp, s);
         if (r) return r;
-        hipError_t e = launch_record(c.d_mse.p, g.hist.p, g.len[b].p, g.hist_len.p, g.pop, g.gen.as<uint32_t>(),
-                                     gen0, gens, s);
-        if (e != hipSuccess) return c.hip_fail(e, "record");
-        r = ga_enqueue_next(c, b, sp, seed, c.d_mse.as<float>(), s);
+        // the ranking launch also writes this generation's history row
+        GaRecord rec;
+        rec.hist = g.hist.as<uint32_t>();
+        rec.hist_len = g.hist_len.as<uint32_t>();
+        rec.len = g.len[b].as<int32_t>();
+        rec.gen = g.gen.as<uint32_t>();
+        rec.gen0 = gen0;
+        rec.max_gens = gens;
+        r = ga_enqueue_next(c, b, sp, seed, c.d_mse.as<float>(), s, rec);
         if (r) return r;
-        e = launch_u32_add(g.gen.as<uint32_t>(), 1u, s);
+        hipError_t e = launch_u32_add(g.gen.as<uint32_t>(), 1u, s);
         if (e != hipSuccess) return c.hip_fail(e, "gen counter");
         return SR_OK;
     };

@@ -10,7 +10,15 @@ namespace sr {
 
 hipError_t launch_ga_init(const GaParams &p, int pop, int cap, uint64_t seed, int32_t *tok,
                           float *val, int32_t *len, int32_t *status, hipStream_t s);
-hipError_t launch_ga_rank(const float *fit, int n, int32_t *order, hipStream_t s);
+// Optional history row written by the ranking launch (hist null: none).
+struct GaRecord {
+    uint32_t *hist = nullptr, *hist_len = nullptr;  // [max_gens][n] fitness bits, lengths
+    const int32_t *len = nullptr;
+    const uint32_t *gen = nullptr;
+    uint32_t gen0 = 0;
+    int max_gens = 0;
+};
+hipError_t launch_ga_rank(const float *fit, int n, int32_t *order, hipStream_t s, const GaRecord &rec = GaRecord{});
 hipError_t launch_ga_next(const GaParams &p, const SelectParams &sp, int pop, int cap,
                           const int32_t *tok, const float *val, const int32_t *len, const float *fit,
                           const int32_t *order, uint64_t seed, const uint32_t *gen, int32_t *otok,
@@ -24,9 +32,6 @@ hipError_t launch_order_by_len(const int32_t *plen, int n, int32_t *order, hipSt
 // to the number of nonzero plen1 entries.
 hipError_t launch_order_by_len2(const int32_t *plen0, int n0, int32_t *order0, const int32_t *plen1, int n1,
                                 int32_t *order1, uint32_t *zero, int n_zero, int32_t *nonzero1, hipStream_t s);
-// hist0/hist1 row (*gen - gen0) = src0/src1 (4-byte words; src1 may be null).
-hipError_t launch_record(const void *src0, void *hist0, const void *src1, void *hist1, int pop,
-                         const uint32_t *gen, uint32_t gen0, int max_gens, hipStream_t s);
 hipError_t launch_u32_add(uint32_t *p, uint32_t v, hipStream_t s);
 hipError_t launch_u32_set(uint32_t *p, uint32_t v, hipStream_t s);
 hipError_t launch_evogp_to_prefix(int pop, int max_len, const float *value, const int16_t *type,
