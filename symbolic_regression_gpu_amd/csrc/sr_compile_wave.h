@@ -278,21 +278,29 @@ SR_HD int compile_wave(Wave &w, const int32_t *tok, const float *val, int n, int
         }
     });
     if (flag[0]) return -1;
-    // 5. subtree starts by pointer jumping
+    // 5. subtree starts by pointer jumping, until every node has reached the
+    // root (log2 of the height rounds)
     int16_t *acc = s.acc0, *up = s.up0, *acc2 = s.acc1, *up2 = s.up1;
-    for (int span = 1; span < n; span <<= 1) {
+    for (int round = 0, span = 1; span < n; span <<= 1, ++round) {
+        const int f = 1 + (round & 1);
         w.each([&](int lane) {
+            if (lane == 0) flag[3 - f] = 0;
+            bool more = false;
             for (int i = lane; i < n; i += kWaveLanes) {
                 const int u = up[i];
+                const int u2 = u >= 0 ? up[u] : -1;
                 acc2[i] = u >= 0 ? acc[i] + acc[u] : acc[i];
-                up2[i] = u >= 0 ? up[u] : -1;
+                up2[i] = u2;
+                more = more || u2 >= 0;
             }
+            if (more) flag[f] = 1;
         });
         int16_t *t0 = acc, *t1 = up;
         acc = acc2;
         up = up2;
         acc2 = t0;
         up2 = t1;
+        if (!flag[f]) break;
     }
     // 6. the instructions of operator node i (1, or 2 for a binary operator
     // of two leaves: its LOAD at start), as (position, code, arg).  The
