@@ -38,13 +38,13 @@ SR_DEV Ins fetch_ins(ConstIns *p, int i)
     return Ins{p[i].code, p[i].arg};
 }
 
-// A raw buffer resource over the memory at p (scalar registers): a load is
-// then one buffer instruction from a 32-bit per-lane byte offset.  Word 3 is
-// the gfx9-family raw-buffer format word; the record count is the maximum,
-// so no offset this kernel forms is clamped.
-SR_DEV __amdgpu_buffer_rsrc_t col_rsrc(const float *p)
+// A raw buffer resource over `bytes` bytes at p (scalar registers): a load is
+// then one buffer instruction from a 32-bit per-lane byte offset, and a load
+// at or past `bytes` returns 0 without touching memory (the range check of
+// raw buffers, stride 0).  Word 3 is the gfx9-family raw-buffer format word.
+SR_DEV __amdgpu_buffer_rsrc_t col_rsrc(const float *p, uint32_t bytes)
 {
-    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)0xffffffff, 0x00020000);
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
 }
 
 template <typename F>
@@ -54,22 +54,35 @@ SR_DEV void vec_each(F &&f)
     for (int k = 0; k < kVecD; ++k) f(k);
 }
 
+// The multi-instruction library ops on the datapoints in two halves, with a
+// scheduling barrier between them: their temporaries are live for half the
+// datapoints at a time (fewer registers at the kernel's peak).
+template <class F>
+SR_DEV void vec_halves(F &&f)
+{
+#pragma unroll
+    for (int k = 0; k < kVecD / 2; ++k) f(k);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = kVecD / 2; k < kVecD; ++k) f(k);
+}
+
 SR_DEV void vec_unary(int op, float (&t)[kVecD])
 {
     using namespace sr_ops;
     switch (op) {
-    case OP_SIN: vec_each([&](int k) { t[k] = f_sin(t[k]); }); break;
-    case OP_COS: vec_each([&](int k) { t[k] = f_cos(t[k]); }); break;
-    case OP_TAN: vec_each([&](int k) { t[k] = f_tan(t[k]); }); break;
-    case OP_SINH: vec_each([&](int k) { t[k] = f_sinh(t[k]); }); break;
-    case OP_COSH: vec_each([&](int k) { t[k] = f_cosh(t[k]); }); break;
-    case OP_TANH: vec_each([&](int k) { t[k] = f_tanh(t[k]); }); break;
-    case OP_EXP: vec_each([&](int k) { t[k] = f_exp(t[k]); }); break;
-    case OP_LOG: vec_each([&](int k) { t[k] = f_log(t[k]); }); break;
+    case OP_SIN: vec_halves([&](int k) { t[k] = f_sin(t[k]); }); break;
+    case OP_COS: vec_halves([&](int k) { t[k] = f_cos(t[k]); }); break;
+    case OP_TAN: vec_halves([&](int k) { t[k] = f_tan(t[k]); }); break;
+    case OP_SINH: vec_halves([&](int k) { t[k] = f_sinh(t[k]); }); break;
+    case OP_COSH: vec_halves([&](int k) { t[k] = f_cosh(t[k]); }); break;
+    case OP_TANH: vec_halves([&](int k) { t[k] = f_tanh(t[k]); }); break;
+    case OP_EXP: vec_halves([&](int k) { t[k] = f_exp(t[k]); }); break;
+    case OP_LOG: vec_halves([&](int k) { t[k] = f_log(t[k]); }); break;
     case OP_INV: vec_each([&](int k) { t[k] = f_inv(t[k]); }); break;
-    case OP_ASIN: vec_each([&](int k) { t[k] = f_asin(t[k]); }); break;
-    case OP_ACOS: vec_each([&](int k) { t[k] = f_acos(t[k]); }); break;
-    case OP_ATAN: vec_each([&](int k) { t[k] = f_atan(t[k]); }); break;
+    case OP_ASIN: vec_halves([&](int k) { t[k] = f_asin(t[k]); }); break;
+    case OP_ACOS: vec_halves([&](int k) { t[k] = f_acos(t[k]); }); break;
+    case OP_ATAN: vec_halves([&](int k) { t[k] = f_atan(t[k]); }); break;
     case OP_LOOSE_LOG: vec_each([&](int k) { t[k] = f_loose_log(t[k]); }); break;
     case OP_LOOSE_INV: vec_each([&](int k) { t[k] = f_loose_inv(t[k]); }); break;
     case OP_ABS: vec_each([&](int k) { t[k] = f_abs(t[k]); }); break;
@@ -111,7 +124,7 @@ SR_DEV void vec_binary_swapped(int op, float (&t)[kVecD], const float (&o)[kVecD
 }
 
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_vec(BatchArgs a)
 {
     __shared__ float s_stk[kBlock / kWave][kVecLdsSlots][kVecD][kWave];
     const int w = threadIdx.x / kWave;
@@ -150,15 +163,16 @@ __global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
             const int last = (int)((c1 - t0 < kVecTile ? c1 - t0 : kVecTile) - 1);
             const bool full = last == kVecTile - 1;
             // six coalesced loads per lane from a uniform base (datapoint
-            // t0 + 64 k + lane), clamped to the tile's last datapoint: the
-            // byte offsets are set once per tile, so a load is one
-            // instruction (scalar base + vector offset) on either kind of tile
-            uint32_t offk[kVecD];
-            vec_each([&](int k) { offk[k] = (uint32_t)min(k * kWave + lane, last) * 4u; });
+            // t0 + 64 k + lane): one per-lane byte offset and a scalar one per
+            // k; the resource's record count ends at the tile's last
+            // datapoint, so on a ragged tile the loads past it read 0 (those
+            // lanes are masked out of the MSE and the column stores)
+            const int lane4 = lane * 4;
+            const uint32_t tile_bytes = (uint32_t)(last + 1) * 4u;
             auto load = [&](const float *base, float(&dst)[kVecD]) {
-                const __amdgpu_buffer_rsrc_t r = col_rsrc(base);
+                const __amdgpu_buffer_rsrc_t r = col_rsrc(base, tile_bytes);
                 vec_each([&](int k) {
-                    dst[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)offk[k], 0, 0));
+                    dst[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, lane4, k * kWave * 4, 0));
                 });
             };
             // operand of an instruction: its variable / column, a constant,
@@ -174,9 +188,12 @@ __global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
                 if (has_ref(MODE) && src == SRC_REF) return ct + off;
                 return nullptr;
             };
-            float top[kVecD], o[kVecD], y[kVecD];
-            // the target, needed after the program (only when this item's MSE is)
-            if (need) load(a.X + (int64_t)a.nv * N + t0, y);
+            float top[kVecD], o[kVecD];
+            // the target, needed after the program (only when this item's MSE
+            // is): loaded into the operand buffer of instruction n, the first
+            // one no instruction uses, so it arrives during the program's last
+            // instructions and takes no registers of its own
+            const float *yt = a.X + (int64_t)a.nv * N + t0;
             vec_each([&](int k) { top[k] = 0.0f; });
             int sp = 0;
             // operands two instructions ahead: instruction i reads buffer
@@ -184,7 +201,10 @@ __global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
             float ba[kVecD], bb[kVecD];
             int code_a = 0, arg_a = 0, code_b = 0, arg_b = 0;
             auto fetch_to = [&](int i, int &c_, int &a_, float(&B)[kVecD]) {
-                if (i >= n) return;
+                if (i >= n) {
+                    if (i == n && need) load(yt, B);
+                    return;
+                }
                 const Ins x = fetch_ins(prog, i);
                 c_ = __builtin_amdgcn_readfirstlane(x.code);
                 a_ = __builtin_amdgcn_readfirstlane(x.arg);
@@ -238,9 +258,10 @@ __global__ __launch_bounds__(kBlock) void k_vec(BatchArgs a)
                 step(i, code_a, arg_a, ba);
                 if (i + 1 < n) step(i + 1, code_b, arg_b, bb);
             }
+            const bool y_in_b = n & 1;
             if (need) vec_each([&](int k) {
                 const bool ok = k * kWave + lane <= last && isfinite(top[k]);
-                const float diff = top[k] - y[k];
+                const float diff = top[k] - (y_in_b ? bb[k] : ba[k]);
                 const float sq = diff * diff;
                 sum += ok ? (double)sq : 0.0;
                 cnt += ok ? 1 : 0;
