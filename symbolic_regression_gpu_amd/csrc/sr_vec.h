@@ -124,7 +124,7 @@ SR_DEV void vec_binary_swapped(int op, float (&t)[kVecD], const float (&o)[kVecD
 }
 
 template <int MODE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_vec(BatchArgs a)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_vec(BatchArgs a)
 {
     __shared__ float s_stk[kBlock / kWave][kVecLdsSlots][kVecD][kWave];
     const int w = threadIdx.x / kWave;
