@@ -196,7 +196,11 @@ int enqueue_batched(Ctx &c, int mode, const LaunchPlan &lp, BatchArgs a, int E, 
         return c.hip_fail(he, "zero counters");
     if (k0) hipEventRecord(k0, s);
     a.spill = b.spill;
-    he = lp.kind == KERNEL_VEC ? launch_vec(mode, a, lp.grid, s) : launch_batched(mode, lp.dpt, lp.xc, a, lp.grid, s);
+    // no more blocks than the work items can keep busy (a wave takes one
+    // item; small launches would otherwise start thousands of idle waves)
+    const int64_t per_queue = ((int64_t)E * lp.cpq + kBlock / kWave - 1) / (kBlock / kWave);
+    const int grid = (int)std::min<int64_t>(lp.grid, (int64_t)lp.nq * std::max<int64_t>(per_queue, 1));
+    he = lp.kind == KERNEL_VEC ? launch_vec(mode, a, grid, s) : launch_batched(mode, lp.dpt, lp.xc, a, grid, s);
     if (he != hipSuccess) return c.hip_fail(he, "batched launch");
     if (k1) hipEventRecord(k1, s);
     if (is_mse_mode(mode) && b.finalize) {
