@@ -156,8 +156,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         const int64_t c0 = q0 + (int64_t)chunk * a.chunk;
         const int64_t c1 = c0 + a.chunk < q1 ? c0 + a.chunk : q1;
         const bool need = !has_ref(MODE) || !a.need_mse || __builtin_amdgcn_readfirstlane(a.need_mse[e]) != 0;
+        // per-lane running sum (fp64, datapoints in ascending order) and
+        // count of valid datapoints (at most chunk / 64 per lane)
         double sum = 0.0;
-        long long cnt = 0;
+        int cnt = 0;
 
         for (int64_t t0 = c0; t0 < c1; t0 += kVecTile) {
             const int last = (int)((c1 - t0 < kVecTile ? c1 - t0 : kVecTile) - 1);
@@ -262,19 +264,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             if (need) vec_each([&](int k) {
                 const bool ok = k * kWave + lane <= last && isfinite(top[k]);
                 const float diff = top[k] - (y_in_b ? bb[k] : ba[k]);
-                const float sq = diff * diff;
-                sum += ok ? (double)sq : 0.0;
+                // an invalid datapoint adds +0.0: the sum is never -0.0, so
+                // this leaves it unchanged
+                sum += (double)(ok ? diff * diff : 0.0f);
                 cnt += ok ? 1 : 0;
             });
         }
+        long long wcnt = 0;
         if (need) {
             sum = wave_sum(sum);
-            cnt = wave_sum(cnt);
+            wcnt = wave_sum((long long)cnt);
         }
         if (lane == 0) {
             const int slot = q * a.chunks_per_queue + chunk;
             a.psum[(int64_t)e * a.P + slot] = sum;
-            a.pcnt[(int64_t)e * a.P + slot] = cnt;
+            a.pcnt[(int64_t)e * a.P + slot] = wcnt;
         }
     }
 }
