@@ -183,12 +183,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             // unsigned 32 x 32-bit product: N < 2^32 for this kernel)
             const float *xt = a.X + t0;
             const float *ct = a.cache + t0;
-            auto operand_ptr = [&](int code, int arg) -> const float * {
-                const int src = bc_src(code);
-                const uint64_t off = (uint64_t)(uint32_t)arg * (uint32_t)N;
-                if (src == SRC_VAR) return xt + off;
-                if (has_ref(MODE) && src == SRC_REF) return ct + off;
-                return nullptr;
+            static_assert(SRC_REF == (SRC_VAR | 1), "variable and column sources differ in bit 0");
+            auto in_memory = [&](int code) {
+                return has_ref(MODE) ? (bc_src(code) | 1) == SRC_REF : bc_src(code) == SRC_VAR;
             };
             float top[kVecD], o[kVecD];
             // the target, needed after the program (only when this item's MSE
@@ -202,22 +199,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             // i % 2, which then receives the operand of instruction i + 2
             float ba[kVecD], bb[kVecD];
             int code_a = 0, arg_a = 0, code_b = 0, arg_b = 0;
+            // one test for "operand in memory" (variable or column), then a
+            // select of the base: fewer scalar branches per instruction
             auto fetch_to = [&](int i, int &c_, int &a_, float(&B)[kVecD]) {
-                if (i >= n) {
-                    if (i == n && need) load(yt, B);
-                    return;
+                if (i < n) {
+                    const Ins x = fetch_ins(prog, i);
+                    c_ = __builtin_amdgcn_readfirstlane(x.code);
+                    a_ = __builtin_amdgcn_readfirstlane(x.arg);
+                    if (in_memory(c_)) {
+                        const float *base = has_ref(MODE) && bc_src(c_) == SRC_REF ? ct : xt;
+                        load(base + (uint64_t)(uint32_t)a_ * (uint32_t)N, B);
+                    }
+                } else if (i == n && need) {
+                    load(yt, B);
                 }
-                const Ins x = fetch_ins(prog, i);
-                c_ = __builtin_amdgcn_readfirstlane(x.code);
-                a_ = __builtin_amdgcn_readfirstlane(x.arg);
-                if (const float *p = operand_ptr(c_, a_)) load(p, B);
             };
             fetch_to(0, code_a, arg_a, ba);
             fetch_to(1, code_b, arg_b, bb);
             auto step = [&](int i, int &code_r, int &arg_r, float(&B)[kVecD]) {
                 const int code = code_r, arg = arg_r;
                 const int src = bc_src(code);
-                if (src == SRC_VAR || (has_ref(MODE) && src == SRC_REF)) vec_each([&](int k) { o[k] = B[k]; });
+                if (in_memory(code)) vec_each([&](int k) { o[k] = B[k]; });
                 fetch_to(i + 2, code_r, arg_r, B);
                 if (code & BC_PUSH) {
                     if (sp < kVecLdsSlots) vec_each([&](int k) { stk[sp][k][lane] = top[k]; });
